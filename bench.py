@@ -1,0 +1,181 @@
+"""Throughput benchmark of the MI355X RHS + TVD-RK3 path (one GPU).
+
+Workload (default): the 2D shock-bubble, two fluids (water + air bubble),
+2048 x 2048 cells, WENO5 + HLLC + RK3, plain FP64, dt from the CFL 0.4
+reduction at t = 0 and then held fixed (the reference's dt is an input).
+A step is one full TVD-RK3 time step (three RHS evaluations with the stage
+updates) over the whole grid; inputs are resident in HBM before timing.
+
+Prints one JSON line with the throughput (cell-updates/s per RK3 step), the
+roofline of the dominant kernel (HIP events on the library's stream) and the
+CPU baseline (the FP64 oracle, one core, one RK3 step of the same grid).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def sweep_bytes_per_step(prm):
+    """Algorithmic HBM bytes per RK3 step of each sweep kernel, per cell:
+    the stencil input is read once (S doubles), rhs is written by the first
+    sweep, read and written by middle ones, read by the last, which reads the
+    RK base (stages 2 and 3) and writes the stage output."""
+    S, D = prm.sys_size, prm.num_dims
+    out = {}
+    names = ["sweep_x", "sweep_y", "sweep_z"][:D]
+    for d, nm in enumerate(names):
+        first, last = d == 0, d == D - 1
+        per_stage = []
+        for stage in range(3):
+            r = S  # stencil input
+            w = 0
+            if not first:
+                r += S  # rhs partial
+            if last:
+                if stage > 0:
+                    r += S  # RK base q1
+                w += S  # stage output
+            else:
+                w += S  # rhs
+            per_stage.append(8 * (r + w))
+        out[nm] = sum(per_stage)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--case", default="shock_bubble_2d", choices=["shock_bubble_2d", "shock_droplet_3d", "sod_1d"])
+    ap.add_argument("--N", type=int, default=0, help="cells per direction (default: the case's size)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_bulk"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU halo exchange is design-only "
+                                   "(see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from microfc_amd import PREC_FP32_BULK, PREC_FP64, RhsModule, ic
+
+    prec = PREC_FP64 if args.precision == "fp64" else PREC_FP32_BULK
+    if args.case == "shock_bubble_2d":
+        N = args.N or 2048
+        prm, grids, q = ic.shock_bubble_2d(N=N, precision=prec)
+        workload = f"2D shock-bubble water/air {N}x{N}, WENO5+HLLC+RK3"
+    elif args.case == "shock_droplet_3d":
+        N = args.N or 256
+        prm, grids, q = ic.shock_droplet_3d(N=N, precision=prec)
+        workload = f"3D shock-droplet water/air {N}^3, WENO5+HLLC+RK3"
+    else:
+        N = args.N or 1000
+        prm, grids, q = ic.sod_1d(m=N - 1)
+        prm.precision = prec
+        workload = f"1D Sod {N} cells, WENO5+HLLC+RK3"
+    ncell = (prm.m + 1) * (prm.n + 1) * (prm.p + 1)
+
+    mod = RhsModule(prm, grids)
+    q1 = torch.from_numpy(q).cuda()
+    q2 = mod.new_field()
+    rhs = mod.new_field(ghosted=False)
+    cfl = 0.4
+    dt = mod.cfl_dt(q1, cfl)
+    for t in range(args.warmup):
+        mod.s_tvd_rk(t, (q1, q2), None, rhs, dt)
+    mod.sync()
+    torch.cuda.synchronize()
+
+    mod.timing(enable=True, reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(args.steps):
+        mod.s_tvd_rk(args.warmup + t, (q1, q2), None, rhs, dt)
+    mod.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing = mod.timing(enable=False)
+    sec = t1 - t0
+    ms_per_step = 1e3 * sec / args.steps
+    value = ncell * args.steps / sec
+
+    assert torch.isfinite(q1).all().item(), "non-finite state after the timed steps"
+
+    # roofline of the dominant kernel
+    per_step = sweep_bytes_per_step(prm)
+    dom = max(per_step, key=lambda k: timing[k][0])
+    dom_ms, dom_n = timing[dom]
+    dom_bytes_per_launch = per_step[dom] * ncell / 3.0  # three launches per step
+    dom_avg_s = dom_ms / 1e3 / dom_n
+    achieved = dom_bytes_per_launch / dom_avg_s / 1e9
+    S, D = prm.sys_size, prm.num_dims
+    step_bytes = 64 * S * ncell
+    f_alg = 3 * (D * (85 * S + 180 + 4 * S + 10) + 20 + 4 * S)
+    out = {
+        "metric": "cell-updates/s per RK3 step",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == PREC_FP64 else "f32 bulk / f64 accumulate",
+        "data": "synthetic initial condition of the named case (generated, not read)",
+        "config": {"workload": workload, "cells": ncell, "sys_size": S, "cfl": cfl, "dt": dt,
+                   "precision": args.precision, "time_stepper": "TVD-RK3"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": dom_bytes_per_launch,
+                     "avg_launch_ms": dom_avg_s * 1e3},
+        "step_hbm": {"algorithmic_bytes_per_cell_step": 64 * S, "achieved_GBps": step_bytes * args.steps / sec / 1e9,
+                     "frac": step_bytes * args.steps / sec / 1e9 / HBM_PEAK_GBS},
+        "fp64_valu": {"nominal_flops_per_cell_step": f_alg, "achieved_TFLOPs": f_alg * value / 1e12,
+                      "peak_TFLOPs": FP64_PEAK_TFLOPS, "frac": f_alg * value / 1e12 / FP64_PEAK_TFLOPS},
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in timing.items() if v[1]},
+    }
+    mod.finalize()
+    del q1, q2, rhs
+    torch.cuda.empty_cache()
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args, prm, grids, q, np)
+    print(json.dumps(out))
+
+
+def cpu_baseline(args, prm, grids, q, np):
+    """The FP64 oracle on one host core: one TVD-RK3 step of the same grid
+    (bounded sample of the workload), same dt rule."""
+    from oracle import oracle as O
+    oc = O.Case(m=prm.m, n=prm.n, p=prm.p, num_fluids=prm.num_fluids, weno_order=prm.weno_order,
+                time_stepper=prm.time_stepper, weno_eps=prm.weno_eps, gammas=list(prm.gammas),
+                pi_infs=list(prm.pi_infs), bc_beg=prm.bc_beg, bc_end=prm.bc_end)
+    ncell = (prm.m + 1) * (prm.n + 1) * (prm.p + 1)
+    if ncell > 20_000_000:
+        return {"value": None, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                "sample": "skipped: grid too large for a bounded CPU sample"}
+    q1 = q.copy()
+    q2 = np.zeros_like(q1)
+    t0 = time.perf_counter()
+    O.tvd_rk_step(oc, grids, 1e-9, q1, q2, t_step=0)
+    sec = time.perf_counter() - t0
+    return {"value": ncell / sec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"one TVD-RK3 step of the same {ncell}-cell grid ({sec:.1f} s) on one host core"}
+
+
+if __name__ == "__main__":
+    main()

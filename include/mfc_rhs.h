@@ -1,0 +1,130 @@
+/*
+ * mfc_rhs.h -- C ABI of the MI355X (gfx950) implementation of MicroFC's
+ * simulation right-hand side and TVD Runge-Kutta time stepper.
+ *
+ * Every entry point returns 0 on success and a negative code on failure
+ * (mfc_last_error() then holds a message); nothing throws across the ABI.
+ * Field arguments are host arrays of sys_size DEVICE pointers, one per
+ * variable, each pointing at the first element of that variable's array,
+ * i.e. c_loc(q_cons_vf(i)%sf(-b, -b, -b)) on the Fortran side:
+ *   ghosted fields (q_cons_vf, q_prim_vf, q_cons_ts): (m+2b+1) x (n+2b+1) x
+ *     (p+2b+1) doubles, x fastest (a collapsed direction has extent 1);
+ *   rhs_vf: interior only, (m+1) x (n+1) x (p+1) doubles, x fastest.
+ * b = buff_size = (weno_order-1)/2 + 2.  All work is queued on the library's
+ * stream (mfc_stream()); calls return before the device has finished unless
+ * they return a host value.  Call mfc_sync() before reading results on the host.
+ */
+#ifndef MFC_RHS_H
+#define MFC_RHS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFC_NUM_FLUIDS_MAX 10
+
+/* Arithmetic of the WENO/HLLC bulk. */
+#define MFC_PREC_FP64 0      /* plain path: FP64 throughout, reference evaluation order */
+#define MFC_PREC_FP32_BULK 1 /* WENO and HLLC in FP32; cons->prim, flux divergence and RK in FP64 */
+
+/* Case parameters; names and meaning follow the namelist &user_inputs of
+ * src/simulation/m_start_up.fpp:73-83 (p and bc_z are the 3D extension). */
+typedef struct mfc_params {
+    int m, n, p;
+    int num_fluids;
+    int weno_order;
+    int time_stepper;
+    int t_step_stop;
+    int bc_x_beg, bc_x_end, bc_y_beg, bc_y_end, bc_z_beg, bc_z_end;
+    double weno_eps;
+    double gammas[MFC_NUM_FLUIDS_MAX];  /* fluid_pp(i)%gamma  */
+    double pi_infs[MFC_NUM_FLUIDS_MAX]; /* fluid_pp(i)%pi_inf */
+    int precision;                      /* MFC_PREC_* */
+} mfc_params;
+
+/* Replaces the module set-up chain s_initialize_global_parameters_module,
+ * s_initialize_variables_conversion_module, s_initialize_riemann_solvers_module,
+ * s_initialize_rhs_module and s_initialize_time_steppers_module
+ * (src/simulation/p_main.fpp:125-183) and the input checks of
+ * s_check_input_file (src/simulation/m_start_up.fpp:117-290). */
+int mfc_rhs_init(const mfc_params *prm);
+
+/* Replaces s_initialize_weno_module / s_compute_weno_coefficients
+ * (src/simulation/m_weno.fpp:103-363) for direction dir (0 = x, 1 = y, 2 = z):
+ * s_cb is the ghosted host array x_cb(-1-b : s+b) and ds the ghosted host
+ * array dx(-b : s+b) as left by s_populate_grid_variables_buffers
+ * (src/simulation/m_start_up.fpp:517-655). */
+int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds);
+
+/* Replaces s_compute_rhs(q_cons_vf, q_prim_vf, rhs_vf, t_step)
+ * (src/simulation/m_rhs.fpp:405-679).  The ghost cells of q_cons_vf are
+ * filled in place (the reference fills a private copy; the ghost layers are
+ * scratch in both).  q_prim_vf may be NULL; when given it receives the ghosted
+ * primitive field (the reference copies it out when run_time_info is set).
+ * When t_step == t_step_stop only the ghost fill happens, as in the reference's
+ * early return at m_rhs.fpp:452. */
+int mfc_compute_rhs(double *const *q_cons_vf, double *const *q_prim_vf, double *const *rhs_vf,
+                    int t_step);
+
+/* Replaces s_{1st,2nd,3rd}_order_tvd_rk(t_step, time_avg)
+ * (src/simulation/m_time_steppers.fpp:129-362) for the configured
+ * time_stepper: advances q_cons_ts1 in place by one step of size dt.
+ * q_cons_ts2 is the second state of RK2/RK3; rhs_vf is interior scratch.
+ * q_prim_vf (may be NULL) receives the primitive field of the first stage,
+ * which is what the reference's run-time information reads.  The RK stage
+ * update is fused into the last direction's sweep; RK3 uses one more ghosted
+ * state owned by the library.  Returns 1 without advancing when
+ * t_step == t_step_stop (m_time_steppers.fpp:296). */
+int mfc_tvd_rk_step(int t_step, double dt, double *const *q_cons_ts1, double *const *q_cons_ts2,
+                    double *const *q_prim_vf, double *const *rhs_vf);
+
+/* Replaces the conversion of s_convert_conservative_to_primitive_variables
+ * (src/common/m_variables_conversion.fpp:313-375) over the ghosted range. */
+int mfc_cons_to_prim(double *const *q_cons_vf, double *const *q_prim_vf);
+
+/* Maximum ICFL over the interior, from a primitive field, as computed by
+ * s_write_run_time_information (src/simulation/m_data_output.fpp:197-251).
+ * Blocks until the value is on the host.  NaN anywhere gives NaN. */
+int mfc_icfl_max(double *const *q_prim_vf, double dt, double *icfl_max);
+
+/* CFL time step: dt = cfl / max over cells of the ICFL at dt = 1, evaluated
+ * from a conservative field (cons->prim fused).  The reference has no adaptive
+ * dt (its dt is an input); this is the same max-reduction inverted. */
+int mfc_cfl_dt(double *const *q_cons_vf, double cfl, double *dt_out);
+
+/* Sizes implied by the initialised case. */
+int mfc_sizes(int *sys_size, int *buff_size, long *ghost_len, long *interior_len);
+
+/* Host copy of the WENO coefficient table of direction dir: 27 planes of
+ * (s+2b+1) doubles, cell j at offset j+b (plane layout in mfc_rhs.hip). */
+int mfc_weno_coefficients(int dir, double *out);
+
+/* Device-memory plumbing for hosts without their own HIP bindings (Fortran). */
+int mfc_device_alloc(size_t bytes, void **ptr);
+int mfc_device_free(void *ptr);
+int mfc_memcpy_h2d(void *dst_device, const void *src_host, size_t bytes);
+int mfc_memcpy_d2h(void *dst_host, const void *src_device, size_t bytes);
+int mfc_sync(void);
+void *mfc_stream(void);
+/* Queue all further work on a caller's HIP stream (NULL: the library's own). */
+int mfc_set_stream(void *stream);
+
+/* Per-kernel timing with HIP events recorded on the library's stream around
+ * every launch.  Kernel ids: 0 bc_fill, 1 cons_to_prim, 2 sweep_x, 3 sweep_y,
+ * 4 sweep_z, 5 icfl, 6 rk1_update.  The reference's only timing is the mean
+ * CPU time per step (src/simulation/m_time_steppers.fpp:281,352-358). */
+int mfc_timing_enable(int on);
+int mfc_timing_reset(void);
+int mfc_timing_read(int kernel, const char **name, double *total_ms, long *count);
+
+/* Replaces s_finalize_*_module (src/simulation/p_main.fpp:329-342). */
+int mfc_rhs_finalize(void);
+
+const char *mfc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,197 @@
+// mfc_device.hpp -- per-cell and per-face device arithmetic of the RHS.
+//
+// Every routine restates the reference's arithmetic in its evaluation order
+// (the library is built with -ffp-contract=off, so each multiply and add is
+// rounded on its own, like the Fortran statements).  R is the arithmetic type
+// of the WENO/HLLC bulk: double for the plain path, float for the FP32-bulk
+// form.  Conversion to primitive variables always runs in double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mfc {
+
+constexpr int kNumFluidsMax = 10;
+constexpr int kMaxSys = 2 * kNumFluidsMax + 3 + 1;
+
+// Index layout of the 5-equation system (m_global_parameters.fpp:302-310):
+// [alpha_i rho_i (NF), momentum (D), E, alpha_i (NF)], 0-based.
+template <int NF, int D>
+struct Sys {
+    static constexpr int S = 2 * NF + D + 1;
+    static constexpr int MOM = NF;
+    static constexpr int E = NF + D;
+    static constexpr int ADV = NF + D + 1;
+};
+
+struct Eos {
+    double gammas[kNumFluidsMax];
+    double pi_infs[kNumFluidsMax];
+};
+
+template <typename T>
+__device__ __forceinline__ T min_ref(T a, T b) { return (a < b) ? a : b; }
+template <typename T>
+__device__ __forceinline__ T max_ref(T a, T b) { return (a > b) ? a : b; }
+
+// Conservative -> primitive at one cell:
+// m_variables_conversion.fpp:329-370 with the mixture rule of :199-207 and
+// the pressure of :98-106.  rho is clamped at sgm_eps = 1e-16 (:353).
+template <int NF, int D>
+__device__ __forceinline__ void cons_to_prim(const double *c, double *p, const Eos &eos) {
+    using Y = Sys<NF, D>;
+    double rho = 0.0, gam = 0.0, pinf = 0.0, dyn = 0.0;
+#pragma unroll
+    for (int i = 0; i < NF; ++i) p[i] = c[i];
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+        rho = rho + c[i];
+        gam = gam + c[Y::ADV + i] * eos.gammas[i];
+        pinf = pinf + c[Y::ADV + i] * eos.pi_infs[i];
+    }
+    rho = max_ref(rho, 1e-16);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        p[Y::MOM + d] = c[Y::MOM + d] / rho;
+        dyn = dyn + 0.5 * c[Y::MOM + d] * p[Y::MOM + d];
+    }
+    p[Y::E] = (c[Y::E] - dyn - pinf) / gam;
+#pragma unroll
+    for (int i = 0; i < NF; ++i) p[Y::ADV + i] = c[Y::ADV + i];
+}
+
+// WENO coefficients of one cell, in the precision of the bulk.  Plane
+// numbering of the 27-entry table: poly_coef_cbL(r,q) at 2r+q, cbR at 6+2r+q,
+// d_cbL(r) at 12+r, d_cbR(r) at 15+r, beta_coef(r,q) at 18+3r+q.
+enum { kCbL = 0, kCbR = 6, kDL = 12, kDR = 15, kBeta = 18, kNCoef = 27 };
+
+// Smoothness indicators of cell j from its stencil (m_weno.fpp:476-506;
+// WENO3 :425-438).  d[] are the undivided differences.
+template <int WO, typename R>
+struct WenoCell {
+    R d[2 * ((WO - 1) / 2)]; // WENO5: d1, d0, dm1, dm2; WENO3: d0, dm1
+    R beta[(WO - 1) / 2 + 1];
+};
+
+// Reconstruction of one variable at one cell (m_weno.fpp:391-540): v[0..WO-1]
+// is the stencil j-polyn..j+polyn, cf the cell's coefficients.  Computes the
+// smoothness indicators once and evaluates the left (face j-1/2) and/or right
+// (face j+1/2) value.
+template <int WO, typename R, bool WANT_L, bool WANT_R>
+__device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL, R &vR) {
+    if constexpr (WO == 1) {
+        vL = v[0];
+        vR = v[0];
+    } else if constexpr (WO == 3) {
+        const R vj = v[1];
+        const R d0 = v[2] - v[1], dm1 = v[1] - v[0];
+        const R b0 = cf[kBeta + 0] * d0 * d0 + eps;
+        const R b1 = cf[kBeta + 3] * dm1 * dm1 + eps;
+        if constexpr (WANT_L) {
+            const R p0 = vj + cf[kCbL + 0] * d0;
+            const R p1 = vj + cf[kCbL + 2] * dm1;
+            const R a0 = cf[kDL + 0] / (b0 * b0);
+            const R a1 = cf[kDL + 1] / (b1 * b1);
+            const R s = a0 + a1;
+            vL = (a0 / s) * p0 + (a1 / s) * p1;
+        }
+        if constexpr (WANT_R) {
+            const R p0 = vj + cf[kCbR + 0] * d0;
+            const R p1 = vj + cf[kCbR + 2] * dm1;
+            const R a0 = cf[kDR + 0] / (b0 * b0);
+            const R a1 = cf[kDR + 1] / (b1 * b1);
+            const R s = a0 + a1;
+            vR = (a0 / s) * p0 + (a1 / s) * p1;
+        }
+    } else {
+        const R vj = v[2];
+        const R d1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];
+        const R b0 = cf[kBeta + 0] * d1 * d1 + cf[kBeta + 1] * d1 * d0 + cf[kBeta + 2] * d0 * d0 + eps;
+        const R b1 = cf[kBeta + 3] * d0 * d0 + cf[kBeta + 4] * d0 * dm1 + cf[kBeta + 5] * dm1 * dm1 + eps;
+        const R b2 = cf[kBeta + 6] * dm1 * dm1 + cf[kBeta + 7] * dm1 * dm2 + cf[kBeta + 8] * dm2 * dm2 + eps;
+        const R bb0 = b0 * b0, bb1 = b1 * b1, bb2 = b2 * b2;
+        if constexpr (WANT_L) {
+            const R p0 = vj + cf[kCbL + 0] * d1 + cf[kCbL + 1] * d0;
+            const R p1 = vj + cf[kCbL + 2] * d0 + cf[kCbL + 3] * dm1;
+            const R p2 = vj + cf[kCbL + 4] * dm1 + cf[kCbL + 5] * dm2;
+            const R a0 = cf[kDL + 0] / bb0, a1 = cf[kDL + 1] / bb1, a2 = cf[kDL + 2] / bb2;
+            const R s = a0 + a1 + a2;
+            vL = (a0 / s) * p0 + (a1 / s) * p1 + (a2 / s) * p2;
+        }
+        if constexpr (WANT_R) {
+            const R p0 = vj + cf[kCbR + 0] * d1 + cf[kCbR + 1] * d0;
+            const R p1 = vj + cf[kCbR + 2] * d0 + cf[kCbR + 3] * dm1;
+            const R p2 = vj + cf[kCbR + 4] * dm1 + cf[kCbR + 5] * dm2;
+            const R a0 = cf[kDR + 0] / bb0, a1 = cf[kDR + 1] / bb1, a2 = cf[kDR + 2] / bb2;
+            const R s = a0 + a1 + a2;
+            vR = (a0 / s) * p0 + (a1 / s) * p1 + (a2 / s) * p2;
+        }
+    }
+}
+
+// HLLC flux at one face, normal direction DIR
+// (m_riemann_solvers.fpp:134-325).  qL is the state left of the face (the
+// right-face reconstruction of the left cell), qR the state right of it.
+// flux[] gets sys_size values, ustar the contact velocity that drives the
+// alpha*div(u) source.
+template <int NF, int D, int DIR, typename R>
+__device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &eos, R *flux, R &ustar) {
+    using Y = Sys<NF, D>;
+    R vel_L_rms = R(0), vel_R_rms = R(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        vel_L_rms = vel_L_rms + qL[Y::MOM + d] * qL[Y::MOM + d];
+        vel_R_rms = vel_R_rms + qR[Y::MOM + d] * qR[Y::MOM + d];
+    }
+    const R pres_L = qL[Y::E], pres_R = qR[Y::E];
+    R rho_L = R(0), gamma_L = R(0), pi_inf_L = R(0);
+    R rho_R = R(0), gamma_R = R(0), pi_inf_R = R(0);
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+        rho_L = rho_L + qL[i];
+        gamma_L = gamma_L + qL[Y::ADV + i] * R(eos.gammas[i]);
+        pi_inf_L = pi_inf_L + qL[Y::ADV + i] * R(eos.pi_infs[i]);
+        rho_R = rho_R + qR[i];
+        gamma_R = gamma_R + qR[Y::ADV + i] * R(eos.gammas[i]);
+        pi_inf_R = pi_inf_R + qR[Y::ADV + i] * R(eos.pi_infs[i]);
+    }
+    const R E_L = gamma_L * pres_L + pi_inf_L + R(0.5) * rho_L * vel_L_rms;
+    const R E_R = gamma_R * pres_R + pi_inf_R + R(0.5) * rho_R * vel_R_rms;
+    const R H_L = (E_L + pres_L) / rho_L;
+    const R H_R = (E_R + pres_R) / rho_R;
+    const R c_L = sqrt((H_L - R(0.5) * vel_L_rms) / gamma_L);
+    const R c_R = sqrt((H_R - R(0.5) * vel_R_rms) / gamma_R);
+    const R uL = qL[Y::MOM + DIR], uR = qR[Y::MOM + DIR];
+    const R s_L = min_ref(uL - c_L, uR - c_R);
+    const R s_R = max_ref(uR + c_R, uL + c_L);
+    const R s_S = (pres_R - pres_L + rho_L * uL * (s_L - uL) - rho_R * uR * (s_R - uR)) /
+                  (rho_L * (s_L - uL) - rho_R * (s_R - uR));
+    const R s_M = min_ref(R(0), s_L);
+    const R s_P = max_ref(R(0), s_R);
+    const R xi_L = (s_L - uL) / (s_L - s_S);
+    const R xi_R = (s_R - uR) / (s_R - s_S);
+    // Fortran sign(0.5, s_S) takes the sign bit of s_S, so -0.0 gives -0.5.
+    const R hs = copysign(R(0.5), s_S);
+    const R xi_M = (R(0.5) + hs);
+    const R xi_P = (R(0.5) - hs);
+    const R mL = uL + s_M * (xi_L - R(1));
+    const R mR = uR + s_P * (xi_R - R(1));
+#pragma unroll
+    for (int i = 0; i < NF; ++i) flux[i] = xi_M * qL[i] * mL + xi_P * qR[i] * mR;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const R flg = (d == DIR) ? R(1) : R(0);
+        const R vLd = qL[Y::MOM + d], vRd = qR[Y::MOM + d];
+        flux[Y::MOM + d] =
+            xi_M * (rho_L * (uL * vLd + s_M * (xi_L * (flg * s_S + (R(1) - flg) * vLd) - vLd)) + flg * (pres_L)) +
+            xi_P * (rho_R * (uR * vRd + s_P * (xi_R * (flg * s_S + (R(1) - flg) * vRd) - vRd)) + flg * (pres_R));
+    }
+    flux[Y::E] = xi_M * (uL * (E_L + pres_L) + s_M * (xi_L * (E_L + (s_S - uL) * (rho_L * s_S + pres_L / (s_L - uL))) - E_L)) +
+                 xi_P * (uR * (E_R + pres_R) + s_P * (xi_R * (E_R + (s_S - uR) * (rho_R * s_S + pres_R / (s_R - uR))) - E_R));
+#pragma unroll
+    for (int i = 0; i < NF; ++i) flux[Y::ADV + i] = xi_M * qL[Y::ADV + i] * mL + xi_P * qR[Y::ADV + i] * mR;
+    // dir_flg = 1 for the normal component: xi_M*(u_L + 1*s_M*(xi_L-1)) + ...
+    ustar = xi_M * (uL + R(1) * s_M * (xi_L - R(1))) + xi_P * (uR + R(1) * s_P * (xi_R - R(1)));
+}
+
+} // namespace mfc

@@ -1,0 +1,58 @@
+// mfc_launch.hpp -- dispatch of the sweep kernels over (num_dims, weno_order,
+// precision) for one number of fluids.  Each mfc_sweep_nf<N>.hip instantiates
+// launch_sweep<N> so the kernel variants compile in parallel.
+#pragma once
+
+#include "mfc_sweep.hpp"
+
+namespace mfc {
+
+struct SweepLaunch {
+    int dir;       // 0, 1, 2
+    int num_dims;  // 1..3
+    int weno_order;
+    int precision; // MFC_PREC_*
+};
+
+template <int NF>
+int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
+
+#define MFC_SWEEP_LAUNCH_IMPL(NF)                                                                       \
+    template <int D, int WO, typename R>                                                                \
+    static int launch_nf##NF##_dwr(int dir, const SweepArgs &a, hipStream_t st) {                       \
+        const dim3 grid((unsigned)a.nblocks), block(64);                                                \
+        if (dir == 0) {                                                                                 \
+            hipLaunchKernelGGL((k_sweep_x<NF, D, WO, R>), grid, block, 0, st, a);                       \
+        } else if (dir == 1) {                                                                          \
+            if constexpr (D >= 2) hipLaunchKernelGGL((k_sweep_t<NF, D, 1, WO, R>), grid, block, 0, st, a); \
+            else return -1;                                                                             \
+        } else {                                                                                        \
+            if constexpr (D >= 3) hipLaunchKernelGGL((k_sweep_t<NF, D, 2, WO, R>), grid, block, 0, st, a); \
+            else return -1;                                                                             \
+        }                                                                                               \
+        return hipGetLastError() == hipSuccess ? 0 : -2;                                                \
+    }                                                                                                   \
+    template <int D, typename R>                                                                        \
+    static int launch_nf##NF##_dr(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {           \
+        switch (L.weno_order) {                                                                         \
+        case 1: return launch_nf##NF##_dwr<D, 1, R>(L.dir, a, st);                                      \
+        case 3: return launch_nf##NF##_dwr<D, 3, R>(L.dir, a, st);                                      \
+        case 5: return launch_nf##NF##_dwr<D, 5, R>(L.dir, a, st);                                      \
+        default: return -1;                                                                             \
+        }                                                                                               \
+    }                                                                                                   \
+    template <typename R>                                                                               \
+    static int launch_nf##NF##_r(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {            \
+        switch (L.num_dims) {                                                                           \
+        case 1: return launch_nf##NF##_dr<1, R>(L, a, st);                                              \
+        case 2: return launch_nf##NF##_dr<2, R>(L, a, st);                                              \
+        case 3: return launch_nf##NF##_dr<3, R>(L, a, st);                                              \
+        default: return -1;                                                                             \
+        }                                                                                               \
+    }                                                                                                   \
+    template <>                                                                                         \
+    int launch_sweep<NF>(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {                    \
+        return L.precision == 1 ? launch_nf##NF##_r<float>(L, a, st) : launch_nf##NF##_r<double>(L, a, st); \
+    }
+
+} // namespace mfc

@@ -1,0 +1,840 @@
+// mfc_rhs.hip -- C ABI of the MI355X RHS library (declared in include/mfc_rhs.h)
+// plus the small kernels around the sweeps: ghost-cell fill, cons->prim,
+// the CFL max-reduction and the pointwise RK update.
+#include "mfc_launch.hpp"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+extern "C" {
+#include "../../include/mfc_rhs.h"
+}
+
+namespace mfc {
+template <> int launch_sweep<1>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<2>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<3>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<4>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+} // namespace mfc
+
+using namespace mfc;
+
+namespace {
+
+constexpr int kNumFluidsBuilt = 4;
+constexpr int kReduceBlocks = 1024;
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return fail(-3, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct Dir {
+    int s = 0;
+    double *coef = nullptr;   // device, 27 x ncell
+    double *inv_ds = nullptr; // device, s+1
+    double *ds_dev = nullptr; // device, ds(0..s)
+    std::vector<double> coef_h, ds_h;
+    bool ready = false;
+};
+
+struct State {
+    bool init = false;
+    mfc_params prm{};
+    int D = 1, S = 0, b = 0, polyn = 0;
+    long nxg = 0, nyg = 0, nzg = 0, ng = 0, nint = 0;
+    Dir dir[3];
+    hipStream_t own = nullptr, stream = nullptr;
+    double *partials = nullptr; // kReduceBlocks doubles
+    double *scratch_state = nullptr; // S*ng doubles, RK3 stage buffer
+    int chunk = 32;
+} G;
+
+// ---------------------------------------------------------------------------
+// Optional per-kernel timing with HIP events on the library's stream.
+enum KernelId { kKBcFill = 0, kKConsToPrim, kKSweepX, kKSweepY, kKSweepZ, kKIcfl, kKRk1, kKCount };
+const char *const kKernelNames[kKCount] = {"bc_fill", "cons_to_prim", "sweep_x", "sweep_y", "sweep_z", "icfl",
+                                           "rk1_update"};
+
+struct Timing {
+    bool on = false;
+    struct Rec {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[kKCount] = {};
+    long count[kKCount] = {};
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+} T;
+
+struct TimedScope {
+    int id;
+    hipEvent_t a = nullptr;
+    explicit TimedScope(int k) : id(k) {
+        if (T.on && (a = T.get())) hipEventRecord(a, G.stream);
+    }
+    ~TimedScope() {
+        if (!a) return;
+        hipEvent_t b = T.get();
+        if (!b) return;
+        hipEventRecord(b, G.stream);
+        T.pending.push_back({id, a, b});
+    }
+};
+
+int timing_flush() {
+    if (T.pending.empty()) return 0;
+    HIPCHK(hipStreamSynchronize(G.stream));
+    for (auto &r : T.pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        T.total_ms[r.id] += ms;
+        T.count[r.id] += 1;
+        T.pool.push_back(r.a);
+        T.pool.push_back(r.b);
+    }
+    T.pending.clear();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// input checks: src/simulation/m_start_up.fpp:117-290 (the parts that apply)
+int check_params(const mfc_params &p) {
+    if (p.m <= 0) return fail(-1, "Unsupported value of m");
+    if (p.n < 0) return fail(-1, "Unsupported value of n");
+    if (p.p < 0) return fail(-1, "Unsupported value of p");
+    if (p.p > 0 && p.n == 0) return fail(-1, "p > 0 requires n > 0");
+    if (p.num_fluids < 1 || p.num_fluids > MFC_NUM_FLUIDS_MAX) return fail(-1, "Unsupported value of num_fluids");
+    if (p.num_fluids > kNumFluidsBuilt)
+        return fail(-1, "num_fluids = %d: kernels are built for 1..%d fluids", p.num_fluids, kNumFluidsBuilt);
+    if (p.weno_order != 1 && p.weno_order != 3 && p.weno_order != 5) return fail(-1, "Unsupported value of weno_order");
+    if (p.m + 1 < 5 * p.weno_order) return fail(-1, "Unsupported combination of values of m and weno_order");
+    if (p.n > 0 && p.n + 1 < 5 * p.weno_order) return fail(-1, "Unsupported combination of values of n and weno_order");
+    if (p.p > 0 && p.p + 1 < 5 * p.weno_order) return fail(-1, "Unsupported combination of values of p and weno_order");
+    if (!(p.weno_eps > 0.0) || p.weno_eps > 1e-6) return fail(-1, "Unsupported value of weno_eps");
+    if (p.time_stepper < 1 || p.time_stepper > 3) return fail(-1, "Unsupported value of time_stepper");
+    const int bcs[3][2] = {{p.bc_x_beg, p.bc_x_end}, {p.bc_y_beg, p.bc_y_end}, {p.bc_z_beg, p.bc_z_end}};
+    const char *nm[3] = {"bc_x", "bc_y", "bc_z"};
+    const int D = 1 + (p.n > 0) + (p.p > 0);
+    for (int d = 0; d < D; ++d) {
+        for (int e = 0; e < 2; ++e)
+            if (bcs[d][e] < -12 || bcs[d][e] > -1)
+                return fail(-1, "Unsupported value of %s%%%s", nm[d], e ? "end" : "beg");
+        if ((bcs[d][0] == -1) != (bcs[d][1] == -1))
+            return fail(-1, "Unsupported combination of values of %s%%beg and %s%%end", nm[d], nm[d]);
+    }
+    for (int i = 0; i < p.num_fluids; ++i) {
+        if (!(p.gammas[i] > 0.0)) return fail(-1, "Unsupported value of fluid_pp(%d)%%gamma", i + 1);
+        if (p.pi_infs[i] < 0.0) return fail(-1, "Unsupported value of fluid_pp(%d)%%pi_inf", i + 1);
+    }
+    if (p.precision != MFC_PREC_FP64 && p.precision != MFC_PREC_FP32_BULK) return fail(-1, "Unsupported precision");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// WENO coefficient tables on the host: src/simulation/m_weno.fpp:168-363.
+// Same plane layout as mfc_device.hpp; x**2d0 is evaluated as x*x.
+inline double sq(double x) { return x * x; }
+
+void weno_tables(int s, int b, int order, const double *cb0, std::vector<double> &out) {
+    const long nc = s + 2 * b + 1;
+    const double *cb = cb0 + 1 + b; // cb[i] = s_cb(i)
+    const int polyn = (order - 1) / 2;
+    out.assign(27 * nc, 0.0);
+    if (order == 1) return;
+    auto C = [&](int plane, int j) -> double & { return out[(long)plane * nc + j + b]; };
+    for (int i = -b - 1 + polyn; i <= s + b - 1 - polyn; ++i) {
+        const int j = i + 1;
+        const double S0 = cb[i], S1 = cb[i + 1], S2 = cb[i + 2], Sm1 = cb[i - 1];
+        if (order == 3) {
+            C(kCbR + 0, j) = (S0 - S1) / (S0 - S2);
+            C(kCbR + 2, j) = (S0 - S1) / (Sm1 - S1);
+            C(kCbL + 0, j) = -C(kCbR + 0, j);
+            C(kCbL + 2, j) = -C(kCbR + 2, j);
+            C(kDR + 0, j) = (Sm1 - S1) / (Sm1 - S2);
+            C(kDL + 0, j) = (Sm1 - S0) / (Sm1 - S2);
+            C(kDR + 1, j) = 1.0 - C(kDR + 0, j);
+            C(kDL + 1, j) = 1.0 - C(kDL + 0, j);
+            C(kBeta + 0, j) = 4.0 * sq(S0 - S1) / sq(S0 - S2);
+            C(kBeta + 3, j) = 4.0 * sq(S0 - S1) / sq(Sm1 - S1);
+            continue;
+        }
+        const double S3 = cb[i + 3], Sm2 = cb[i - 2];
+        const double h4 = 4.0 * sq(S0 - S1);
+        C(kCbR + 0, j) = ((S0 - S1) * (S1 - S2)) / ((S0 - S3) * (S3 - S1));
+        C(kCbR + 2, j) = ((Sm1 - S1) * (S1 - S0)) / ((Sm1 - S2) * (S2 - S0));
+        C(kCbR + 3, j) = ((S0 - S1) * (S1 - S2)) / ((Sm1 - S1) * (Sm1 - S2));
+        C(kCbR + 5, j) = ((S0 - S1) * (S1 - Sm1)) / ((Sm2 - S0) * (Sm2 - S1));
+        C(kCbL + 0, j) = ((S1 - S0) * (S0 - S2)) / ((S0 - S3) * (S3 - S1));
+        C(kCbL + 2, j) = ((S0 - Sm1) * (S0 - S1)) / ((Sm1 - S2) * (S0 - S2));
+        C(kCbL + 3, j) = ((S1 - S0) * (S0 - S2)) / ((Sm1 - S1) * (Sm1 - S2));
+        C(kCbL + 5, j) = ((Sm1 - S0) * (S0 - S1)) / ((Sm2 - S0) * (Sm2 - S1));
+        C(kCbR + 1, j) = ((S0 - S2) + (S1 - S3)) / ((S0 - S2) * (S0 - S3)) * ((S0 - S1));
+        C(kCbR + 4, j) = ((Sm2 - S1) + (Sm1 - S1)) / ((Sm1 - S1) * (S1 - Sm2)) * ((S1 - S0));
+        C(kCbL + 1, j) = ((S0 - S2) + (S0 - S3)) / ((S0 - S2) * (S0 - S3)) * ((S1 - S0));
+        C(kCbL + 4, j) = ((Sm2 - S0) + (Sm1 - S1)) / ((Sm2 - S1) * (S1 - Sm1)) * ((S0 - S1));
+        C(kDR + 0, j) = ((Sm2 - S1) * (S1 - Sm1)) / ((Sm2 - S3) * (S3 - Sm1));
+        C(kDR + 2, j) = ((S1 - S2) * (S1 - S3)) / ((Sm2 - S2) * (Sm2 - S3));
+        C(kDL + 0, j) = ((Sm2 - S0) * (S0 - Sm1)) / ((Sm2 - S3) * (S3 - Sm1));
+        C(kDL + 2, j) = ((S0 - S2) * (S0 - S3)) / ((Sm2 - S2) * (Sm2 - S3));
+        C(kDR + 1, j) = 1.0 - C(kDR + 0, j) - C(kDR + 2, j);
+        C(kDL + 1, j) = 1.0 - C(kDL + 0, j) - C(kDL + 2, j);
+        const double a10 = S1 - S0, a21 = S2 - S1, a20 = S2 - S0, a31 = S3 - S1, a0m = S0 - Sm1;
+        C(kBeta + 0, j) = h4 * (10.0 * sq(a10) + a10 * a21 + sq(a21)) / (sq(S0 - S3) * sq(S1 - S3));
+        C(kBeta + 1, j) = h4 * (19.0 * sq(a10) - a10 * a31 + 2.0 * a20 * (a20 + a31)) /
+                          ((S0 - S2) * sq(S0 - S3) * a31);
+        C(kBeta + 2, j) = h4 * (10.0 * sq(a10) + a10 * (a20 + a31) + sq(a20 + a31)) / (sq(S0 - S2) * sq(S0 - S3));
+        C(kBeta + 3, j) = h4 * (10.0 * sq(a10) + sq(a0m) + a0m * a10) / (sq(Sm1 - S2) * sq(S0 - S2));
+        C(kBeta + 4, j) = h4 * ((S0 - S1) * (a0m + 20.0 * a10) + (2.0 * a0m + a10) * a20) /
+                          ((S1 - Sm1) * sq(Sm1 - S2) * a20);
+        C(kBeta + 5, j) = h4 * (10.0 * sq(a10) + a10 * a21 + sq(a21)) / (sq(Sm1 - S1) * sq(Sm1 - S2));
+        const double c2m = (S0 - Sm2) + (S0 - Sm1);
+        C(kBeta + 6, j) = h4 * (12.0 * sq(a10) + sq(c2m) + 3.0 * c2m * a10) / (sq(Sm2 - S1) * sq(Sm1 - S1));
+        C(kBeta + 7, j) = h4 * (19.0 * sq(a10) + ((S0 - Sm2) * (S0 - S1)) + 2.0 * (S1 - Sm1) * ((S0 - Sm2) + (S1 - Sm1))) /
+                          ((Sm2 - S0) * sq(Sm2 - S1) * (S1 - Sm1));
+        C(kBeta + 8, j) = h4 * (10.0 * sq(a10) + sq(a0m) + a0m * a10) / (sq(Sm2 - S0) * sq(Sm2 - S1));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+
+struct Box {
+    int m, n, p, b, bj, bk;
+    long nxg, nyg;
+};
+
+__device__ __forceinline__ long box_off(const Box &x, int i, int j, int k) {
+    return (long)(i + x.b) + x.nxg * ((long)(j + x.bj) + x.nyg * (long)(k + x.bk));
+}
+
+// Source index of a ghost coordinate for one direction's boundary condition
+// (m_rhs.fpp:692-905): <= -3 extrapolation, -2 reflection, -1 periodic.
+__device__ __forceinline__ int bc_source(int bcb, int bce, int s, int i, bool &flip) {
+    if (i < 0) {
+        if (bcb <= -3) return 0;
+        if (bcb == -2) { flip = !flip; return -i - 1; }
+        return s + 1 + i;
+    }
+    if (i > s) {
+        if (bce <= -3) return s;
+        if (bce == -2) { flip = !flip; return 2 * s + 1 - i; }
+        return i - (s + 1);
+    }
+    return i;
+}
+
+// Ghost fill.  The reference fills x ghosts for interior rows, then y ghosts
+// over all x, then z ghosts over all x and y, so a corner holds the interior
+// cell reached by applying each direction's map in turn, with the momentum
+// component normal to each reflecting boundary negated.  Each thread here
+// fills one ghost cell with that composed map.  blockIdx.y selects the slab:
+// 0: x ghosts (all j, k), 1: y ghosts (interior i, all k), 2: z ghosts
+// (interior i, j); together they cover every ghost cell once.
+__global__ void k_bc_fill(Fields q, int S, int momxb, Box x, int bxb, int bxe, int byb, int bye, int bzb,
+                          int bze, long n0, long n1, long n2) {
+    const int slab = blockIdx.y;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    const int gx = 2 * x.b, gy = 2 * x.bj, gz = 2 * x.bk;
+    const int NY = x.n + 1 + gy, NX = x.m + 1;
+    if (slab == 0) {
+        if (t >= n0) return;
+        const int gi = (int)(t % gx);
+        const long r = t / gx;
+        const int jj = (int)(r % NY);
+        const int kk = (int)(r / NY);
+        i = gi < x.b ? gi - x.b : x.m + 1 + (gi - x.b);
+        j = jj - x.bj;
+        k = kk - x.bk;
+    } else if (slab == 1) {
+        if (t >= n1) return;
+        const int ii = (int)(t % NX);
+        const long r = t / NX;
+        const int gj = (int)(r % gy);
+        const int kk = (int)(r / gy);
+        i = ii;
+        j = gj < x.bj ? gj - x.bj : x.n + 1 + (gj - x.bj);
+        k = kk - x.bk;
+    } else {
+        if (t >= n2) return;
+        const int ii = (int)(t % NX);
+        const long r = t / NX;
+        const int jj = (int)(r % (x.n + 1));
+        const int gk = (int)(r / (x.n + 1));
+        i = ii;
+        j = jj;
+        k = gk < x.bk ? gk - x.bk : x.p + 1 + (gk - x.bk);
+    }
+    bool fx = false, fy = false, fz = false;
+    const int si = bc_source(bxb, bxe, x.m, i, fx);
+    const int sj = x.bj ? bc_source(byb, bye, x.n, j, fy) : j;
+    const int sk = x.bk ? bc_source(bzb, bze, x.p, k, fz) : k;
+    const long dst = box_off(x, i, j, k), src = box_off(x, si, sj, sk);
+    for (int v = 0; v < S; ++v) {
+        double val = q.v[v][src];
+        if ((fx && v == momxb) || (fy && v == momxb + 1) || (fz && v == momxb + 2)) val = -val;
+        q.v[v][dst] = val;
+    }
+}
+
+template <int NF, int D>
+__global__ void k_cons_to_prim(Fields qc, Fields qp, Eos eos, long ng) {
+    constexpr int S = Sys<NF, D>::S;
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= ng) return;
+    double c[S], p[S];
+#pragma unroll
+    for (int v = 0; v < S; ++v) c[v] = qc.v[v][o];
+    cons_to_prim<NF, D>(c, p, eos);
+#pragma unroll
+    for (int v = 0; v < S; ++v) qp.v[v][o] = p[v];
+}
+
+// NaN-propagating maximum: once a NaN is seen it stays.
+__device__ __forceinline__ double max_nan(double a, double b) {
+    return (a != a || b != b) ? (a != a ? a : b) : (a > b ? a : b);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max_nan(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Per-cell ICFL (m_data_output.fpp:197-244) and a block max-reduction by
+// wavefront shuffles; block results go to partials[blockIdx.x].  FROM_CONS
+// converts the conservative field in place of reading q_prim_vf; the values
+// are identical because the reference's q_prim_vf is that same conversion.
+template <int NF, int D, bool FROM_CONS>
+__global__ __launch_bounds__(256) void k_icfl(Fields q, Eos eos, Box x, const double *dx, const double *dy,
+                                              const double *dz, double dt, long nint, double *partials) {
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    double mx = -INFINITY;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nint; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % (x.m + 1));
+        const long r = t / (x.m + 1);
+        const int j = (int)(r % (x.n + 1));
+        const int k = (int)(r / (x.n + 1));
+        const long o = box_off(x, i, j, k);
+        double pr[S];
+        if constexpr (FROM_CONS) {
+            double c[S];
+#pragma unroll
+            for (int v = 0; v < S; ++v) c[v] = q.v[v][o];
+            cons_to_prim<NF, D>(c, pr, eos);
+        } else {
+#pragma unroll
+            for (int v = 0; v < S; ++v) pr[v] = q.v[v][o];
+        }
+        double rho = 0.0, gam = 0.0, pinf = 0.0;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            rho = rho + pr[f];
+            gam = gam + pr[Y::ADV + f] * eos.gammas[f];
+            pinf = pinf + pr[Y::ADV + f] * eos.pi_infs[f];
+        }
+        const double pres = pr[Y::E];
+        const double c = sqrt(((gam + 1.0) * pres + pinf) / (gam * rho));
+        double icfl;
+        if constexpr (D == 1) {
+            icfl = (dt / dx[i]) * (fabs(pr[Y::MOM]) + c);
+        } else if constexpr (D == 2) {
+            icfl = dt / min_ref(dx[i] / (fabs(pr[Y::MOM]) + c), dy[j] / (fabs(pr[Y::MOM + 1]) + c));
+        } else {
+            icfl = dt / min_ref(min_ref(dx[i] / (fabs(pr[Y::MOM]) + c), dy[j] / (fabs(pr[Y::MOM + 1]) + c)),
+                                dz[k] / (fabs(pr[Y::MOM + 2]) + c));
+        }
+        mx = max_nan(mx, icfl);
+    }
+    __shared__ double red[4];
+    mx = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = max_nan(v, red[w]);
+        partials[blockIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_max(double *partials, int n) {
+    double mx = -INFINITY;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) mx = max_nan(mx, partials[t]);
+    __shared__ double red[4];
+    mx = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = max_nan(v, red[w]);
+        partials[0] = v;
+    }
+}
+
+// Pointwise RK1 update q = q + dt*rhs on the interior
+// (m_time_steppers.fpp:165-175); in place, so it cannot be fused into a
+// stencil sweep that reads q.
+__global__ void k_rk1_update(Fields q, Fields rhs, int S, Box x, double dt, long nint) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nint) return;
+    const int i = (int)(t % (x.m + 1));
+    const long r = t / (x.m + 1);
+    const int j = (int)(r % (x.n + 1));
+    const int k = (int)(r / (x.n + 1));
+    const long o = box_off(x, i, j, k);
+    for (int v = 0; v < S; ++v) q.v[v][o] = q.v[v][o] + dt * rhs.v[v][t];
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+
+Box box() {
+    Box x;
+    x.m = G.prm.m;
+    x.n = G.prm.n;
+    x.p = G.prm.p;
+    x.b = G.b;
+    x.bj = G.prm.n > 0 ? G.b : 0;
+    x.bk = G.prm.p > 0 ? G.b : 0;
+    x.nxg = G.nxg;
+    x.nyg = G.nyg;
+    return x;
+}
+
+Eos eos() {
+    Eos e;
+    for (int i = 0; i < kNumFluidsMax; ++i) {
+        e.gammas[i] = G.prm.gammas[i];
+        e.pi_infs[i] = G.prm.pi_infs[i];
+    }
+    return e;
+}
+
+int fields(double *const *ptrs, Fields &f, const char *what) {
+    if (!ptrs) return fail(-1, "%s: NULL field array", what);
+    for (int v = 0; v < kMaxSys; ++v) f.v[v] = nullptr;
+    for (int v = 0; v < G.S; ++v) {
+        if (!ptrs[v]) return fail(-1, "%s: NULL pointer for variable %d", what, v + 1);
+        f.v[v] = ptrs[v];
+    }
+    return 0;
+}
+
+int require_ready() {
+    if (!G.init) return fail(-4, "mfc_rhs_init has not been called");
+    for (int d = 0; d < G.D; ++d)
+        if (!G.dir[d].ready) return fail(-4, "mfc_rhs_set_grid has not been called for direction %d", d);
+    return 0;
+}
+
+template <typename F>
+int dispatch_nf_d(F &&f) {
+    const int nf = G.prm.num_fluids, D = G.D;
+#define MFC_CASE(NF, DD) \
+    if (nf == NF && D == DD) return f(std::integral_constant<int, NF>{}, std::integral_constant<int, DD>{});
+    MFC_CASE(1, 1) MFC_CASE(1, 2) MFC_CASE(1, 3)
+    MFC_CASE(2, 1) MFC_CASE(2, 2) MFC_CASE(2, 3)
+    MFC_CASE(3, 1) MFC_CASE(3, 2) MFC_CASE(3, 3)
+    MFC_CASE(4, 1) MFC_CASE(4, 2) MFC_CASE(4, 3)
+#undef MFC_CASE
+    return fail(-1, "no kernels for num_fluids=%d, num_dims=%d", nf, D);
+}
+
+int bc_fill(const Fields &q) {
+    const Box x = box();
+    const long n0 = (long)(2 * x.b) * (x.n + 1 + 2 * x.bj) * (x.p + 1 + 2 * x.bk);
+    const long n1 = (long)(x.m + 1) * (2 * x.bj) * (x.p + 1 + 2 * x.bk);
+    const long n2 = (long)(x.m + 1) * (x.n + 1) * (2 * x.bk);
+    const long nmax = std::max(n0, std::max(n1, n2));
+    const dim3 grid((unsigned)((nmax + 255) / 256), 1 + (G.D >= 2) + (G.D >= 3)), block(256);
+    TimedScope ts(kKBcFill);
+    hipLaunchKernelGGL(k_bc_fill, grid, block, 0, G.stream, q, G.S, G.prm.num_fluids, x, G.prm.bc_x_beg,
+                       G.prm.bc_x_end, G.prm.bc_y_beg, G.prm.bc_y_end, G.prm.bc_z_beg, G.prm.bc_z_end, n0, n1, n2);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int cons_to_prim_launch(const Fields &qc, const Fields &qp) {
+    const Eos e = eos();
+    TimedScope ts(kKConsToPrim);
+    return dispatch_nf_d([&](auto NF, auto D) -> int {
+        hipLaunchKernelGGL((k_cons_to_prim<decltype(NF)::value, decltype(D)::value>),
+                           dim3((unsigned)((G.ng + 255) / 256)), dim3(256), 0, G.stream, qc, qp, e, G.ng);
+        HIPCHK(hipGetLastError());
+        return 0;
+    });
+}
+
+SweepArgs sweep_args(int d) {
+    SweepArgs a;
+    memset(&a, 0, sizeof a);
+    a.coef = G.dir[d].coef;
+    a.inv_ds = G.dir[d].inv_ds;
+    a.eos = eos();
+    a.eps = G.prm.weno_eps;
+    a.nxg = G.nxg;
+    a.nyg = G.nyg;
+    a.ncell = G.dir[d].s + 2 * G.b + 1;
+    a.m = G.prm.m;
+    a.n = G.prm.n;
+    a.p = G.prm.p;
+    a.b = G.b;
+    a.bj = G.prm.n > 0 ? G.b : 0;
+    a.bk = G.prm.p > 0 ? G.b : 0;
+    const int bcs[3][2] = {{G.prm.bc_x_beg, G.prm.bc_x_end}, {G.prm.bc_y_beg, G.prm.bc_y_end},
+                           {G.prm.bc_z_beg, G.prm.bc_z_end}};
+    a.bc_beg = bcs[d][0];
+    a.bc_end = bcs[d][1];
+    if (d == 0) {
+        a.ntile = (G.prm.m + 1 + 62) / 63;
+        a.nblocks = (long)a.ntile * (G.prm.n + 1) * (G.prm.p + 1);
+    } else {
+        const int s = G.dir[d].s;
+        a.ntile = (G.prm.m + 1 + 63) / 64;
+        a.chunk = G.chunk;
+        a.nchunk = (s + 1 + a.chunk - 1) / a.chunk;
+        const int other = d == 1 ? G.prm.p + 1 : G.prm.n + 1;
+        a.nblocks = (long)a.ntile * a.nchunk * other;
+    }
+    return a;
+}
+
+int sweep(int d, SweepArgs &a) {
+    SweepLaunch L{d, G.D, G.prm.weno_order, G.prm.precision};
+    TimedScope ts(kKSweepX + d);
+    int rc;
+    switch (G.prm.num_fluids) {
+    case 1: rc = launch_sweep<1>(L, a, G.stream); break;
+    case 2: rc = launch_sweep<2>(L, a, G.stream); break;
+    case 3: rc = launch_sweep<3>(L, a, G.stream); break;
+    case 4: rc = launch_sweep<4>(L, a, G.stream); break;
+    default: rc = -1;
+    }
+    if (rc != 0) return fail(-3, "sweep launch failed (dir %d): %s", d, hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+// All directional sweeps of one RHS evaluation on the ghost-filled field qin.
+// The last sweep either writes rhs (rk = kRkNone) or applies the RK stage.
+int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, const Fields *qout, double dt) {
+    for (int d = 0; d < G.D; ++d) {
+        SweepArgs a = sweep_args(d);
+        a.qin = qin;
+        a.rhs = rhs;
+        a.accumulate = d > 0;
+        a.dt = dt;
+        const bool last = d == G.D - 1;
+        a.rk = last ? rk : kRkNone;
+        if (last && rk != kRkNone) {
+            a.qbase = *qbase;
+            a.qout = *qout;
+        }
+        if (int rc = sweep(d, a)) return rc;
+    }
+    return 0;
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+const char *mfc_last_error(void) { return g_err.c_str(); }
+
+int mfc_rhs_init(const mfc_params *prm) {
+    if (!prm) return fail(-1, "NULL params");
+    if (int rc = check_params(*prm)) return rc;
+    if (G.init) mfc_rhs_finalize();
+    G.prm = *prm;
+    G.D = 1 + (prm->n > 0) + (prm->p > 0);
+    G.S = 2 * prm->num_fluids + G.D + 1;
+    G.polyn = (prm->weno_order - 1) / 2;
+    G.b = G.polyn + 2;
+    G.nxg = prm->m + 2 * G.b + 1;
+    G.nyg = prm->n > 0 ? prm->n + 2 * G.b + 1 : 1;
+    G.nzg = prm->p > 0 ? prm->p + 2 * G.b + 1 : 1;
+    G.ng = G.nxg * G.nyg * G.nzg;
+    G.nint = (long)(prm->m + 1) * (prm->n + 1) * (prm->p + 1);
+    G.dir[0].s = prm->m;
+    G.dir[1].s = prm->n;
+    G.dir[2].s = prm->p;
+    if (const char *e = getenv("MFC_MARCH_CHUNK")) {
+        const int c = atoi(e);
+        if (c >= 1 && c <= 4096) G.chunk = c;
+    }
+    HIPCHK(hipStreamCreateWithFlags(&G.own, hipStreamNonBlocking));
+    G.stream = G.own;
+    HIPCHK(hipMalloc(&G.partials, sizeof(double) * kReduceBlocks));
+    G.init = true;
+    return 0;
+}
+
+int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds) {
+    if (!G.init) return fail(-4, "mfc_rhs_init has not been called");
+    if (dir < 0 || dir >= G.D) return fail(-1, "direction %d outside 0..%d", dir, G.D - 1);
+    if (!s_cb || !ds) return fail(-1, "NULL grid array");
+    Dir &D = G.dir[dir];
+    const int s = D.s, b = G.b;
+    for (int i = 0; i <= s; ++i)
+        if (!(ds[i + b] > 0.0)) return fail(-1, "non-positive cell width at index %d of direction %d", i, dir);
+    weno_tables(s, b, G.prm.weno_order, s_cb, D.coef_h);
+    std::vector<double> inv(s + 1);
+    for (int i = 0; i <= s; ++i) inv[i] = 1.0 / ds[i + b];
+    D.ds_h.assign(ds, ds + s + 2 * b + 1);
+    if (D.coef) HIPCHK(hipFree(D.coef));
+    if (D.inv_ds) HIPCHK(hipFree(D.inv_ds));
+    if (D.ds_dev) HIPCHK(hipFree(D.ds_dev));
+    D.coef = nullptr;
+    D.inv_ds = nullptr;
+    D.ds_dev = nullptr;
+    HIPCHK(hipMalloc(&D.coef, sizeof(double) * D.coef_h.size()));
+    HIPCHK(hipMalloc(&D.inv_ds, sizeof(double) * inv.size()));
+    HIPCHK(hipMalloc(&D.ds_dev, sizeof(double) * (s + 1)));
+    HIPCHK(hipMemcpy(D.ds_dev, ds + b, sizeof(double) * (s + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.coef, D.coef_h.data(), sizeof(double) * D.coef_h.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.inv_ds, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+    D.ready = true;
+    return 0;
+}
+
+int mfc_compute_rhs(double *const *q_cons_vf, double *const *q_prim_vf, double *const *rhs_vf, int t_step) {
+    if (int rc = require_ready()) return rc;
+    Fields qc, qp, rhs;
+    if (int rc = fields(q_cons_vf, qc, "q_cons_vf")) return rc;
+    if (int rc = bc_fill(qc)) return rc;
+    if (t_step == G.prm.t_step_stop) return 1;
+    if (int rc = fields(rhs_vf, rhs, "rhs_vf")) return rc;
+    if (q_prim_vf) {
+        if (int rc = fields(q_prim_vf, qp, "q_prim_vf")) return rc;
+        if (int rc = cons_to_prim_launch(qc, qp)) return rc;
+    }
+    return sweeps(qc, rhs, kRkNone, nullptr, nullptr, 0.0);
+}
+
+int mfc_tvd_rk_step(int t_step, double dt, double *const *q_cons_ts1, double *const *q_cons_ts2,
+                    double *const *q_prim_vf, double *const *rhs_vf) {
+    if (int rc = require_ready()) return rc;
+    Fields q1, q2, qp, rhs;
+    if (int rc = fields(q_cons_ts1, q1, "q_cons_ts(1)")) return rc;
+    if (int rc = fields(rhs_vf, rhs, "rhs_vf")) return rc;
+    if (G.prm.time_stepper > 1)
+        if (int rc = fields(q_cons_ts2, q2, "q_cons_ts(2)")) return rc;
+    if (int rc = bc_fill(q1)) return rc;
+    if (t_step == G.prm.t_step_stop) return 1;
+    if (q_prim_vf) {
+        if (int rc = fields(q_prim_vf, qp, "q_prim_vf")) return rc;
+        if (int rc = cons_to_prim_launch(q1, qp)) return rc;
+    }
+    if (G.prm.time_stepper == 1) {
+        if (int rc = sweeps(q1, rhs, kRkNone, nullptr, nullptr, dt)) return rc;
+        const Box x = box();
+        TimedScope ts(kKRk1);
+        hipLaunchKernelGGL(k_rk1_update, dim3((unsigned)((G.nint + 255) / 256)), dim3(256), 0, G.stream, q1, rhs,
+                           G.S, x, dt, G.nint);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    if (G.prm.time_stepper == 2) {
+        if (int rc = sweeps(q1, rhs, kRkEuler, &q1, &q2, dt)) return rc;
+        if (int rc = bc_fill(q2)) return rc;
+        return sweeps(q2, rhs, kRk2Final, &q1, &q1, dt);
+    }
+    if (!G.scratch_state) HIPCHK(hipMalloc(&G.scratch_state, sizeof(double) * G.S * G.ng));
+    Fields q3;
+    for (int v = 0; v < kMaxSys; ++v) q3.v[v] = v < G.S ? G.scratch_state + (long)v * G.ng : nullptr;
+    // stage 1: q3 = q1 + dt*L(q1); stage 2: q2 = (3 q1 + q3 + dt*L(q3))/4;
+    // stage 3: q1 = (q1 + 2 q2 + 2 dt*L(q2))/3.  Each stage reads its input's
+    // stencil and writes a different array, or only its own cell of q1.
+    if (int rc = sweeps(q1, rhs, kRkEuler, &q1, &q3, dt)) return rc;
+    if (int rc = bc_fill(q3)) return rc;
+    if (int rc = sweeps(q3, rhs, kRk3Stage2, &q1, &q2, dt)) return rc;
+    if (int rc = bc_fill(q2)) return rc;
+    return sweeps(q2, rhs, kRk3Stage3, &q1, &q1, dt);
+}
+
+int mfc_cons_to_prim(double *const *q_cons_vf, double *const *q_prim_vf) {
+    if (!G.init) return fail(-4, "mfc_rhs_init has not been called");
+    Fields qc, qp;
+    if (int rc = fields(q_cons_vf, qc, "q_cons_vf")) return rc;
+    if (int rc = fields(q_prim_vf, qp, "q_prim_vf")) return rc;
+    return cons_to_prim_launch(qc, qp);
+}
+
+static int icfl_reduce(double *const *q, bool from_cons, double dt, double *out) {
+    if (int rc = require_ready()) return rc;
+    if (!out) return fail(-1, "NULL output");
+    Fields f;
+    if (int rc = fields(q, f, from_cons ? "q_cons_vf" : "q_prim_vf")) return rc;
+    const Box x = box();
+    const Eos e = eos();
+    double *dxs[3] = {G.dir[0].ds_dev, G.dir[1].ds_dev, G.dir[2].ds_dev};
+    const long want = (G.nint + 255) / 256;
+    const int nb = (int)std::min<long>(kReduceBlocks, std::max<long>(1, want));
+    TimedScope ts(kKIcfl);
+    int rc = dispatch_nf_d([&](auto NF, auto D) -> int {
+        if (from_cons)
+            hipLaunchKernelGGL((k_icfl<decltype(NF)::value, decltype(D)::value, true>), dim3(nb), dim3(256), 0,
+                               G.stream, f, e, x, dxs[0], dxs[1], dxs[2], dt, G.nint, G.partials);
+        else
+            hipLaunchKernelGGL((k_icfl<decltype(NF)::value, decltype(D)::value, false>), dim3(nb), dim3(256), 0,
+                               G.stream, f, e, x, dxs[0], dxs[1], dxs[2], dt, G.nint, G.partials);
+        HIPCHK(hipGetLastError());
+        return 0;
+    });
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, G.stream, G.partials, nb);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, G.partials, sizeof(double), hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    return 0;
+}
+
+int mfc_icfl_max(double *const *q_prim_vf, double dt, double *icfl_max) {
+    return icfl_reduce(q_prim_vf, false, dt, icfl_max);
+}
+
+int mfc_cfl_dt(double *const *q_cons_vf, double cfl, double *dt_out) {
+    if (!(cfl > 0.0)) return fail(-1, "cfl must be positive");
+    double mx = 0.0;
+    if (int rc = icfl_reduce(q_cons_vf, true, 1.0, &mx)) return rc;
+    if (!(mx > 0.0) || mx != mx) return fail(-5, "CFL reduction gave %g (NaN or non-positive wave speed)", mx);
+    *dt_out = cfl / mx;
+    return 0;
+}
+
+int mfc_sizes(int *sys_size, int *buff_size, long *ghost_len, long *interior_len) {
+    if (!G.init) return fail(-4, "mfc_rhs_init has not been called");
+    if (sys_size) *sys_size = G.S;
+    if (buff_size) *buff_size = G.b;
+    if (ghost_len) *ghost_len = G.ng;
+    if (interior_len) *interior_len = G.nint;
+    return 0;
+}
+
+int mfc_weno_coefficients(int dir, double *out) {
+    if (!G.init) return fail(-4, "mfc_rhs_init has not been called");
+    if (dir < 0 || dir >= G.D || !G.dir[dir].ready) return fail(-1, "no grid for direction %d", dir);
+    memcpy(out, G.dir[dir].coef_h.data(), sizeof(double) * G.dir[dir].coef_h.size());
+    return 0;
+}
+
+int mfc_device_alloc(size_t bytes, void **ptr) {
+    if (!ptr) return fail(-1, "NULL pointer");
+    HIPCHK(hipMalloc(ptr, bytes));
+    return 0;
+}
+
+int mfc_device_free(void *ptr) {
+    HIPCHK(hipFree(ptr));
+    return 0;
+}
+
+int mfc_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    hipStream_t st = G.init ? G.stream : nullptr;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int mfc_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    hipStream_t st = G.init ? G.stream : nullptr;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int mfc_sync(void) {
+    if (!G.init) return 0;
+    HIPCHK(hipStreamSynchronize(G.stream));
+    return 0;
+}
+
+void *mfc_stream(void) { return G.init ? (void *)G.stream : nullptr; }
+
+int mfc_timing_enable(int on) {
+    if (!G.init) return fail(-4, "mfc_rhs_init has not been called");
+    if (int rc = timing_flush()) return rc;
+    T.on = on != 0;
+    return 0;
+}
+
+int mfc_timing_reset(void) {
+    if (G.init)
+        if (int rc = timing_flush()) return rc;
+    for (int k = 0; k < kKCount; ++k) {
+        T.total_ms[k] = 0.0;
+        T.count[k] = 0;
+    }
+    return 0;
+}
+
+int mfc_timing_read(int kernel, const char **name, double *total_ms, long *count) {
+    if (kernel < 0 || kernel >= kKCount) return fail(-1, "kernel id %d outside 0..%d", kernel, kKCount - 1);
+    if (G.init)
+        if (int rc = timing_flush()) return rc;
+    if (name) *name = kKernelNames[kernel];
+    if (total_ms) *total_ms = T.total_ms[kernel];
+    if (count) *count = T.count[kernel];
+    return 0;
+}
+
+int mfc_set_stream(void *stream) {
+    if (!G.init) return fail(-4, "mfc_rhs_init has not been called");
+    G.stream = stream ? (hipStream_t)stream : G.own;
+    return 0;
+}
+
+int mfc_rhs_finalize(void) {
+    if (!G.init) return 0;
+    hipStreamSynchronize(G.stream);
+    timing_flush();
+    for (auto e : T.pool) hipEventDestroy(e);
+    T = Timing();
+    for (int d = 0; d < 3; ++d) {
+        if (G.dir[d].coef) hipFree(G.dir[d].coef);
+        if (G.dir[d].inv_ds) hipFree(G.dir[d].inv_ds);
+        if (G.dir[d].ds_dev) hipFree(G.dir[d].ds_dev);
+        G.dir[d] = Dir();
+    }
+    if (G.partials) hipFree(G.partials);
+    if (G.scratch_state) hipFree(G.scratch_state);
+    if (G.own) hipStreamDestroy(G.own);
+    const int chunk = G.chunk;
+    G = State();
+    G.chunk = chunk;
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,323 @@
+// mfc_sweep.hpp -- the directional sweep kernels of the RHS.
+//
+// One sweep per direction fuses: conservative -> primitive conversion of the
+// stencil cells, WENO reconstruction of both face states, the HLLC flux and
+// contact velocity, the flux divergence and the alpha*div(u) source, and --
+// in the last direction of a time step -- the TVD-RK stage update.  Face
+// states and fluxes never touch HBM.  Per cell and direction the HBM traffic
+// is the stencil read of the S conservative variables (halo reads mostly hit
+// L2) plus the rhs read-modify-write, or the RK base/output in the fused last
+// sweep.
+//
+//  * k_sweep_x (x, the contiguous direction): one wave per 64 faces of one
+//    grid row.  The 64+2*polyn+1 stencil cells are converted to primitive and
+//    staged in LDS; lane l computes face x0-1/2+l; fluxes go through LDS so
+//    lane l can difference faces l and l+1 for cell x0+l (63 cells per wave).
+//  * k_sweep_t (y or z, strided): one wave per 64 consecutive x-columns,
+//    marching along the sweep direction over a chunk of cells.  Each lane
+//    keeps the 2*polyn+1 stencil cells, the previous cell's right-face state
+//    and the previous face's flux in registers, so each step loads one cell
+//    (coalesced along x), reconstructs one cell and solves one face.
+#pragma once
+
+#include "mfc_device.hpp"
+
+namespace mfc {
+
+struct Fields {
+    double *v[kMaxSys];
+};
+
+// Stage kinds of the TVD-RK update fused into the last sweep
+// (m_time_steppers.fpp:129-362).  base is q_cons_ts(1), in is the stage input.
+enum RkKind {
+    kRkNone = 0,   // write rhs_vf
+    kRkEuler = 1,  // out = base + dt*rhs
+    kRk2Final = 2, // out = (base + in + dt*rhs)/2
+    kRk3Stage2 = 3,// out = (3*base + in + dt*rhs)/4
+    kRk3Stage3 = 4 // out = (base + 2*in + 2*dt*rhs)/3
+};
+
+struct SweepArgs {
+    Fields qin, qbase, qout, rhs;
+    const double *coef;   // 27 planes of ncell entries for this direction
+    const double *inv_ds; // 1/ds(i), i = 0..s
+    Eos eos;
+    double eps, dt;
+    long nxg, nyg;        // ghosted pitches in elements
+    long ncell;           // s + 2b + 1
+    int m, n, p, b, bj, bk;
+    int bc_beg, bc_end;   // of this direction
+    int rk;               // RkKind
+    int accumulate;       // 0: first direction assigns rhs, 1: adds
+    int chunk, nchunk;    // march chunking (k_sweep_t)
+    int ntile;            // x tiles (k_sweep_x) or x column groups (k_sweep_t)
+    long nblocks;
+};
+
+__device__ __forceinline__ long ghost_off(const SweepArgs &a, int i, int j, int k) {
+    return (long)(i + a.b) + a.nxg * ((long)(j + a.bj) + a.nyg * (long)(k + a.bk));
+}
+__device__ __forceinline__ long interior_off(const SweepArgs &a, int i, int j, int k) {
+    return (long)i + (long)(a.m + 1) * ((long)j + (long)(a.n + 1) * (long)k);
+}
+
+// Blocks are dealt to the 8 XCDs round-robin by id.  Renumber so that each
+// XCD gets a contiguous range of logical blocks: neighbouring tiles, which
+// share stencil halos, then run on the same XCD and share its L2.
+__device__ __forceinline__ long xcd_logical_block(long bid, long nblocks) {
+    constexpr long kXcd = 8;
+    const long full = (nblocks / kXcd) * kXcd;
+    if (bid >= full) return bid;
+    const long per = full / kXcd;
+    return (bid % kXcd) * per + bid / kXcd;
+}
+
+template <typename R>
+__device__ __forceinline__ double to_d(R x) { return (double)x; }
+
+// Finishes one cell: flux divergence, alpha*div(u) source, then either the
+// rhs store or the fused RK stage (m_rhs.fpp:565-652, m_time_steppers.fpp).
+template <int NF, int D, typename R>
+__device__ __forceinline__ void finish_cell(const SweepArgs &a, long go, long ro, double inv,
+                                            const R *Fm, const R *Fp, R usm, R usp) {
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    double r[S];
+#pragma unroll
+    for (int v = 0; v < S; ++v) {
+        const double df = to_d(Fm[v]) - to_d(Fp[v]);
+        r[v] = a.accumulate ? a.rhs.v[v][ro] + inv * df : inv * df;
+    }
+    const double dus = to_d(usp) - to_d(usm);
+#pragma unroll
+    for (int i = 0; i < NF; ++i) r[Y::ADV + i] = r[Y::ADV + i] + inv * a.qin.v[Y::ADV + i][go] * dus;
+    switch (a.rk) {
+    case kRkNone:
+#pragma unroll
+        for (int v = 0; v < S; ++v) a.rhs.v[v][ro] = r[v];
+        break;
+    case kRkEuler:
+#pragma unroll
+        for (int v = 0; v < S; ++v) a.qout.v[v][go] = a.qbase.v[v][go] + a.dt * r[v];
+        break;
+    case kRk2Final:
+#pragma unroll
+        for (int v = 0; v < S; ++v) a.qout.v[v][go] = (a.qbase.v[v][go] + a.qin.v[v][go] + a.dt * r[v]) / 2.0;
+        break;
+    case kRk3Stage2:
+#pragma unroll
+        for (int v = 0; v < S; ++v)
+            a.qout.v[v][go] = (3.0 * a.qbase.v[v][go] + a.qin.v[v][go] + a.dt * r[v]) / 4.0;
+        break;
+    default:
+#pragma unroll
+        for (int v = 0; v < S; ++v)
+            a.qout.v[v][go] = (a.qbase.v[v][go] + 2.0 * a.qin.v[v][go] + 2.0 * a.dt * r[v]) / 3.0;
+        break;
+    }
+}
+
+template <int NF, int D>
+__device__ __forceinline__ void load_prim(const SweepArgs &a, long go, double *p) {
+    constexpr int S = Sys<NF, D>::S;
+    double c[S];
+#pragma unroll
+    for (int v = 0; v < S; ++v) c[v] = a.qin.v[v][go];
+    cons_to_prim<NF, D>(c, p, a.eos);
+}
+
+template <typename R, int WO>
+__device__ __forceinline__ void load_coef(const SweepArgs &a, int cell, R *cf) {
+    const double *t = a.coef + (long)(cell + a.b);
+    if constexpr (WO > 1) {
+#pragma unroll
+        for (int q = 0; q < kNCoef; ++q) cf[q] = (R)t[(long)q * a.ncell];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x sweep
+template <int NF, int D, int WO, typename R>
+__global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    constexpr int PN = (WO - 1) / 2;
+    constexpr int NC = 64 + 2 * PN + 1;
+    __shared__ R sp[S][NC];
+    __shared__ R sf[S + 1][64];
+
+    const long bid = xcd_logical_block(blockIdx.x, a.nblocks);
+    const int tile = (int)(bid % a.ntile);
+    const long row = bid / a.ntile;
+    const int j = (int)(row % (a.n + 1));
+    const int k = (int)(row / (a.n + 1));
+    const int lane = threadIdx.x;
+    const int x0 = tile * 63;
+
+    // 1. stage primitive stencil cells x0-1-PN .. x0+63+PN
+    for (int c = lane; c < NC; c += 64) {
+        const int x = x0 - 1 - PN + c;
+        double pr[S];
+        if (x <= a.m + a.b) {
+            load_prim<NF, D>(a, ghost_off(a, x, j, k), pr);
+        } else {
+#pragma unroll
+            for (int v = 0; v < S; ++v) pr[v] = 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < S; ++v) sp[v][c] = (R)pr[v];
+    }
+    __syncthreads();
+
+    // 2. face x0-1+lane + 1/2
+    const int jf = x0 - 1 + lane;
+    if (jf <= a.m) {
+        R cfR[kNCoef], cfL[kNCoef];
+        load_coef<R, WO>(a, jf, cfR);
+        load_coef<R, WO>(a, jf + 1, cfL);
+        R qL[S], qR[S], f[S], us;
+        const R eps = (R)a.eps;
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            R st[WO], dummy;
+#pragma unroll
+            for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + w];
+            weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
+#pragma unroll
+            for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + 1 + w];
+            weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
+        }
+        if (jf == -1 && a.bc_beg == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qL[v] = qR[v];
+        }
+        if (jf == a.m && a.bc_end == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qR[v] = qL[v];
+        }
+        hllc_face<NF, D, 0, R>(qL, qR, a.eos, f, us);
+#pragma unroll
+        for (int v = 0; v < S; ++v) sf[v][lane] = f[v];
+        sf[S][lane] = us;
+    }
+    __syncthreads();
+
+    // 3. cell x0+lane
+    const int i = x0 + lane;
+    if (lane < 63 && i <= a.m) {
+        R Fm[S], Fp[S];
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            Fm[v] = sf[v][lane];
+            Fp[v] = sf[v][lane + 1];
+        }
+        finish_cell<NF, D, R>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), a.inv_ds[i], Fm, Fp,
+                              sf[S][lane], sf[S][lane + 1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y / z sweep (DIR = 1 or 2): march along the sweep direction
+template <int NF, int D, int DIR, int WO, typename R>
+__global__ __launch_bounds__(64) void k_sweep_t(const SweepArgs a) {
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    constexpr int PN = (WO - 1) / 2;
+
+    const long bid = xcd_logical_block(blockIdx.x, a.nblocks);
+    const int xt = (int)(bid % a.ntile);
+    const long rest = bid / a.ntile;
+    const int chunk = (int)(rest % a.nchunk);
+    const int t = (int)(rest / a.nchunk);
+    const int lane = threadIdx.x;
+    const int i_raw = xt * 64 + lane;
+    const bool active = i_raw <= a.m;
+    const int i = active ? i_raw : a.m;
+    const int s = (DIR == 1) ? a.n : a.p;
+    const int j0 = chunk * a.chunk;
+    const int j1 = min(j0 + a.chunk - 1, s);
+
+    // ghosted offset of the cell at sweep index c of this column
+    auto gcell = [&](int c) -> long {
+        return (DIR == 1) ? ghost_off(a, i, c, t) : ghost_off(a, i, t, c);
+    };
+    auto rcell = [&](int c) -> long {
+        return (DIR == 1) ? interior_off(a, i, c, t) : interior_off(a, i, t, c);
+    };
+
+    R W[WO][S];
+    R vRp[S], Fprev[S], usprev = R(0);
+    const R eps = (R)a.eps;
+#pragma unroll
+    for (int w = 0; w < WO; ++w) {
+        double pr[S];
+        load_prim<NF, D>(a, gcell(j0 - 1 - PN + w), pr);
+#pragma unroll
+        for (int v = 0; v < S; ++v) W[w][v] = (R)pr[v];
+    }
+    {
+        R cf[kNCoef];
+        load_coef<R, WO>(a, j0 - 1, cf);
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            R st[WO], dummy;
+#pragma unroll
+            for (int w = 0; w < WO; ++w) st[w] = W[w][v];
+            weno_cell<WO, R, false, true>(st, cf, eps, dummy, vRp[v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < S; ++v) Fprev[v] = R(0);
+
+    for (int c = j0; c <= j1 + 1; ++c) {
+        // shift the window and bring in cell c+PN
+#pragma unroll
+        for (int w = 0; w < WO - 1; ++w)
+#pragma unroll
+            for (int v = 0; v < S; ++v) W[w][v] = W[w + 1][v];
+        {
+            double pr[S];
+            load_prim<NF, D>(a, gcell(c + PN), pr);
+#pragma unroll
+            for (int v = 0; v < S; ++v) W[WO - 1][v] = (R)pr[v];
+        }
+        R vL[S], vR[S];
+        {
+            R cf[kNCoef];
+            load_coef<R, WO>(a, c, cf);
+#pragma unroll
+            for (int v = 0; v < S; ++v) {
+                R st[WO];
+#pragma unroll
+                for (int w = 0; w < WO; ++w) st[w] = W[w][v];
+                weno_cell<WO, R, true, true>(st, cf, eps, vL[v], vR[v]);
+            }
+        }
+        // face (c-1)+1/2: left state = right face of cell c-1, right state = left face of cell c
+        R qL[S], F[S], us;
+#pragma unroll
+        for (int v = 0; v < S; ++v) qL[v] = vRp[v];
+        if (c - 1 == -1 && a.bc_beg == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qL[v] = vL[v];
+        }
+        if (c - 1 == s && a.bc_end == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) vL[v] = qL[v];
+        }
+        hllc_face<NF, D, DIR, R>(qL, vL, a.eos, F, us);
+        if (c - 1 >= j0 && active) {
+            const int cc = c - 1;
+            finish_cell<NF, D, R>(a, gcell(cc), rcell(cc), a.inv_ds[cc], Fprev, F, usprev, us);
+        }
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            Fprev[v] = F[v];
+            vRp[v] = vR[v];
+        }
+        usprev = us;
+    }
+}
+
+} // namespace mfc

@@ -1,0 +1,195 @@
+"""Initial conditions for the benchmark cases, written straight into the
+ghosted conservative layout the library consumes.
+
+prim_to_cons restates s_convert_primitive_to_conservative_variables
+(src/common/m_variables_conversion.fpp:385-454), the pre_process direction.
+The smoothed circle/sphere patch follows the tanh smoothing of
+src/pre_process/m_create_patches.fpp:122-128 (eta = tanh(smooth_coeff/min(dx)
+*(r - R))*(-0.5) + 0.5) and the mixing rule of m_assign_patches.fpp:54-165
+(a smoothed patch mixes the patch and background primitive states by eta).
+Configurations are the benchmark's cases; each returns (params, grids, q)
+with q a float64 numpy array of the ghosted shape (ghost cells zero).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .rhs import Params, populate_grid_buffers, uniform_cell_boundaries
+
+# Stiffened-gas inputs in MFC's convention (examples/1D_kapilashocktube/case.py:60-63):
+# gamma = 1/(g-1), pi_inf = g*p_inf/(g-1).
+def eos_inputs(g: float, p_inf: float = 0.0):
+    return 1.0 / (g - 1.0), g * p_inf / (g - 1.0)
+
+
+WATER = eos_inputs(4.4, 6.0e8)
+AIR = eos_inputs(1.4, 0.0)
+
+
+def prim_to_cons(params: Params, prim):
+    """Primitive -> conservative, vectorised over the trailing axes."""
+    nf, D = params.num_fluids, params.num_dims
+    q = np.array(prim, dtype=np.float64, copy=True)
+    rho = np.zeros(q.shape[1:])
+    gam = np.zeros(q.shape[1:])
+    pinf = np.zeros(q.shape[1:])
+    for i in range(nf):
+        rho = rho + prim[i]
+        gam = gam + prim[nf + D + 1 + i] * params.gammas[i]
+        pinf = pinf + prim[nf + D + 1 + i] * params.pi_infs[i]
+    dyn = np.zeros(q.shape[1:])
+    for i in range(nf, nf + D):
+        q[i] = rho * prim[i]
+        dyn = dyn + q[i] * prim[i] / 2.0
+    q[nf + D] = gam * prim[nf + D] + dyn + pinf
+    return q
+
+
+def uniform_grids(params: Params, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    grids = []
+    for d, s in enumerate(params.extents()):
+        cbi = uniform_cell_boundaries(s, lo[d], hi[d])
+        grids.append(populate_grid_buffers(cbi, params.buff_size, params.bc_beg[d], params.bc_end[d]))
+    return grids
+
+
+def cell_centres(params: Params, grids):
+    b = params.buff_size
+    out = []
+    for d, s in enumerate(params.extents()):
+        cb = grids[d][0]
+        out.append(0.5 * (cb[b: b + s + 1] + cb[b + 1: b + s + 2]))
+    return out
+
+
+def mesh(params: Params, grids):
+    """Cell-centre coordinate arrays broadcast to (p+1, n+1, m+1)."""
+    cc = cell_centres(params, grids)
+    shp = params.interior_shape[1:]
+    X = np.broadcast_to(cc[0][None, None, :], shp)
+    Y = np.broadcast_to(cc[1][None, :, None], shp) if params.num_dims > 1 else None
+    Z = np.broadcast_to(cc[2][:, None, None], shp) if params.num_dims > 2 else None
+    return X, Y, Z
+
+
+def ghosted(params: Params, interior_cons):
+    q = np.zeros(params.ghost_shape)
+    b = params.buff_size
+    bj = b if params.n > 0 else 0
+    bk = b if params.p > 0 else 0
+    q[:, bk:bk + params.p + 1, bj:bj + params.n + 1, b:b + params.m + 1] = interior_cons
+    return q
+
+
+def sod_1d(m: int = 999, weno_order: int = 5):
+    """1D Sod: x < 0.5 rho=1, p=1; x >= 0.5 rho=0.125, p=0.1; gamma 1.4."""
+    g, pi = AIR
+    prm = Params(m=m, num_fluids=1, weno_order=weno_order, gammas=[g], pi_infs=[pi])
+    grids = uniform_grids(prm)
+    X, _, _ = mesh(prm, grids)
+    prim = np.zeros(prm.interior_shape)
+    left = X < 0.5
+    prim[0] = np.where(left, 1.0, 0.125)
+    prim[1] = 0.0
+    prim[2] = np.where(left, 1.0, 0.1)
+    prim[3] = 1.0
+    return prm, grids, ghosted(prm, prim_to_cons(prm, prim))
+
+
+def stiffened_hugoniot_density(rho1, p1, p2, g, p_inf):
+    """Post-shock density of a stiffened gas from the Rankine-Hugoniot relation."""
+    a = (p2 + p_inf) / (p1 + p_inf)
+    return rho1 * ((g + 1.0) * a + (g - 1.0)) / ((g - 1.0) * a + (g + 1.0))
+
+
+def _smooth_indicator(r, R, dxmin, smooth_cells=2.0):
+    """eta = 1 inside, 0 outside, tanh profile of about smooth_cells cells."""
+    return 0.5 - 0.5 * np.tanh((r - R) / (0.5 * smooth_cells * dxmin))
+
+
+def shock_bubble_2d(N: int = 2048, weno_order: int = 5, alpha_min: float = 1e-6, precision: int = 0):
+    """Water column with an air bubble (R = 0.1 at (0.5, 0.5)) hit by a planar
+    water shock at x = 0.3 (post-shock p = 1e9, u = 200; p = 1e5 ahead).
+    Fluid 1 water, fluid 2 air; extrapolation boundaries."""
+    gw, piw = WATER
+    ga, pia = AIR
+    prm = Params(m=N - 1, n=N - 1, num_fluids=2, weno_order=weno_order, gammas=[gw, ga], pi_infs=[piw, pia],
+                 bc_beg=(-3, -3, -3), bc_end=(-3, -3, -3), precision=precision)
+    grids = uniform_grids(prm)
+    X, Y, _ = mesh(prm, grids)
+    dxmin = 1.0 / N
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2)
+    eta = _smooth_indicator(r, 0.1, dxmin)
+    a_air = alpha_min + (1.0 - 2.0 * alpha_min) * eta
+    a_wat = 1.0 - a_air
+    rho_w1, p1 = 1000.0, 1.0e5
+    rho_w2 = stiffened_hugoniot_density(rho_w1, p1, 1.0e9, 4.4, 6.0e8)
+    shocked = X < 0.3
+    prim = np.zeros(prm.interior_shape)
+    prim[0] = a_wat * np.where(shocked, rho_w2, rho_w1)
+    prim[1] = a_air * 1.0
+    prim[2] = np.where(shocked, 200.0, 0.0)
+    prim[3] = 0.0
+    prim[4] = np.where(shocked, 1.0e9, p1)
+    prim[5] = a_wat
+    prim[6] = a_air
+    return prm, grids, ghosted(prm, prim_to_cons(prm, prim))
+
+
+def shock_droplet_3d(N: int = 512, weno_order: int = 5, alpha_min: float = 1e-6, precision: int = 0):
+    """Water droplet (R = 0.1, centre of [0,1]^3) in air (rho 1.2, p 1e5) hit
+    by a Mach 2.4 air shock at x = 0.2 (Rankine-Hugoniot post-shock state)."""
+    gw, piw = WATER
+    ga, pia = AIR
+    prm = Params(m=N - 1, n=N - 1, p=N - 1, num_fluids=2, weno_order=weno_order, gammas=[gw, ga],
+                 pi_infs=[piw, pia], precision=precision)
+    grids = uniform_grids(prm)
+    X, Y, Z = mesh(prm, grids)
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2)
+    eta = _smooth_indicator(r, 0.1, 1.0 / N)
+    a_wat = alpha_min + (1.0 - 2.0 * alpha_min) * eta
+    a_air = 1.0 - a_wat
+    g, M, rho1, p1 = 1.4, 2.4, 1.2, 1.0e5
+    c1 = math.sqrt(g * p1 / rho1)
+    p2 = p1 * (1.0 + 2.0 * g / (g + 1.0) * (M * M - 1.0))
+    rho2 = rho1 * (g + 1.0) * M * M / ((g - 1.0) * M * M + 2.0)
+    u2 = M * c1 * (1.0 - rho1 / rho2)
+    shocked = X < 0.2
+    prim = np.zeros(prm.interior_shape)
+    prim[0] = a_wat * 1000.0
+    prim[1] = a_air * np.where(shocked, rho2, rho1)
+    prim[2] = np.where(shocked, u2, 0.0)
+    prim[5] = np.where(shocked, p2, p1)
+    prim[6] = a_wat
+    prim[7] = a_air
+    return prm, grids, ghosted(prm, prim_to_cons(prm, prim))
+
+
+def shear_layer_2d(N: int = 4096, weno_order: int = 5, seed: int = 0, modes: int = 16, delta: float = 0.01,
+                   precision: int = 0):
+    """Kelvin-Helmholtz shear layer, one gas (gamma 1.4): u = tanh((y-0.5)/delta),
+    rho = 1 below / 0.5 above y = 0.5 (smoothed by the same tanh), p = 1, and a
+    seeded multimode v perturbation (amplitudes U(0,1e-3), random phases,
+    Gaussian envelope around the layer).  Periodic in x, extrapolation in y."""
+    g, pi = AIR
+    prm = Params(m=N - 1, n=N - 1, num_fluids=1, weno_order=weno_order, gammas=[g], pi_infs=[pi],
+                 bc_beg=(-1, -3, -3), bc_end=(-1, -3, -3), precision=precision)
+    grids = uniform_grids(prm)
+    X, Y, _ = mesh(prm, grids)
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(0.0, 1e-3, modes)
+    ph = rng.uniform(0.0, 2.0 * math.pi, modes)
+    v = np.zeros(prm.interior_shape[1:])
+    for k in range(modes):
+        v = v + amp[k] * np.sin(2.0 * math.pi * (k + 1) * X + ph[k])
+    v = v * np.exp(-((Y - 0.5) / (4.0 * delta)) ** 2)
+    t = np.tanh((Y - 0.5) / delta)
+    prim = np.zeros(prm.interior_shape)
+    prim[0] = 1.0 - 0.25 * (1.0 + t)
+    prim[1] = t
+    prim[2] = v
+    prim[3] = 1.0
+    prim[4] = 1.0
+    return prm, grids, ghosted(prm, prim_to_cons(prm, prim))

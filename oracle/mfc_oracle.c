@@ -1,0 +1,612 @@
+/*
+ * mfc_oracle.c -- CPU FP64 restatement of MicroFC's simulation RHS.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  Compile with
+ * -ffp-contract=off so that every multiply and add rounds separately, as the
+ * reference's Fortran statements do.  Every function cites the reference
+ * file:line whose arithmetic it restates; evaluation order follows the
+ * Fortran left-to-right rule for operators of equal precedence, and x**2d0 is
+ * evaluated as x*x.
+ */
+#include "mfc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SGM_EPS 1e-16 /* m_global_parameters.fpp:39 */
+
+/* ------------------------------------------------------------------------ */
+/* sizes (m_global_parameters.fpp:285-381)                                   */
+
+int orc_num_dims(const orc_case *c) { return 1 + (c->n > 0) + (c->p > 0); }
+int orc_sys_size(const orc_case *c) { return 2 * c->num_fluids + orc_num_dims(c) + 1; }
+int orc_buff_size(const orc_case *c) { return (c->weno_order - 1) / 2 + 2; }
+
+typedef struct {
+    int m, n, p, b, bj, bk;
+    long nxg, nyg, nzg, ng;
+    int D, S, nf;
+    int contxe, momxb, E_idx, advxb; /* 0-based */
+} geom;
+
+static void mkgeom(const orc_case *c, geom *g) {
+    g->m = c->m;
+    g->n = c->n;
+    g->p = c->p;
+    g->b = orc_buff_size(c);
+    g->bj = c->n > 0 ? g->b : 0;
+    g->bk = c->p > 0 ? g->b : 0;
+    g->nxg = c->m + 2 * g->b + 1;
+    g->nyg = c->n + 2 * g->bj + 1;
+    g->nzg = c->p + 2 * g->bk + 1;
+    g->ng = g->nxg * g->nyg * g->nzg;
+    g->D = orc_num_dims(c);
+    g->nf = c->num_fluids;
+    g->S = 2 * g->nf + g->D + 1;
+    g->contxe = g->nf - 1;
+    g->momxb = g->nf;
+    g->E_idx = g->nf + g->D;
+    g->advxb = g->E_idx + 1;
+}
+
+long orc_ghost_len(const orc_case *c) {
+    geom g;
+    mkgeom(c, &g);
+    return g.ng;
+}
+
+static inline long gidx(const geom *g, int i, int j, int k) {
+    return (long)(i + g->b) + g->nxg * ((long)(j + g->bj) + g->nyg * (long)(k + g->bk));
+}
+
+static inline long ridx(const geom *g, int i, int j, int k) {
+    return (long)i + (long)(g->m + 1) * ((long)j + (long)(g->n + 1) * (long)k);
+}
+
+/* ------------------------------------------------------------------------ */
+/* grid ghost buffers: m_start_up.fpp:517-655                                */
+
+int orc_grid_buffers(int s, int b, int bc_beg, int bc_end, double *cb_, double *ds_, double *cc_) {
+    double *cb = cb_ + 1 + b; /* cb[i] = s_cb(i), i in [-1-b, s+b] */
+    double *ds = ds_ + b;     /* ds[i] = ds(i),   i in [-b, s+b]   */
+    double *cc = cc_ + b;
+    int i;
+    /* m_start_up.fpp:356-359 */
+    for (i = 0; i <= s; i++) ds[i] = cb[i] - cb[i - 1];
+    for (i = 0; i <= s; i++) cc[i] = cb[i - 1] + ds[i] / 2.0;
+    /* beginning: m_start_up.fpp:527-550 */
+    if (bc_beg <= -3) {
+        for (i = 1; i <= b; i++) ds[-i] = ds[0];
+    } else if (bc_beg == -2) {
+        for (i = 1; i <= b; i++) ds[-i] = ds[i - 1];
+    } else if (bc_beg == -1) {
+        for (i = 1; i <= b; i++) ds[-i] = ds[s - (i - 1)];
+    } else {
+        return -1; /* processor boundaries are not part of a single-domain run */
+    }
+    for (i = 1; i <= b; i++) cb[-1 - i] = cb[-i] - ds[-i];
+    for (i = 1; i <= b; i++) cc[-i] = cc[1 - i] - (ds[1 - i] + ds[-i]) / 2.0;
+    /* end: m_start_up.fpp:557-582 */
+    if (bc_end <= -3) {
+        for (i = 1; i <= b; i++) ds[s + i] = ds[s];
+    } else if (bc_end == -2) {
+        for (i = 1; i <= b; i++) ds[s + i] = ds[s - (i - 1)];
+    } else if (bc_end == -1) {
+        for (i = 1; i <= b; i++) ds[s + i] = ds[i - 1];
+    } else {
+        return -1;
+    }
+    for (i = 1; i <= b; i++) cb[s + i] = cb[s + (i - 1)] + ds[s + i];
+    for (i = 1; i <= b; i++) cc[s + i] = cc[s + (i - 1)] + (ds[s + (i - 1)] + ds[s + i]) / 2.0;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO coefficients: m_weno.fpp:168-363                                     */
+
+static inline double sq(double x) { return x * x; }
+
+int orc_weno_coefficients(int s, int b, int weno_order, const double *cb_, double *out) {
+    const double *cb = cb_ + 1 + b;
+    const long nc = s + 2 * b + 1;
+    const int polyn = (weno_order - 1) / 2;
+    int i;
+    memset(out, 0, sizeof(double) * 27 * (size_t)nc);
+    if (weno_order == 1) return 0;
+#define C(plane, j) out[(long)(plane) * nc + (long)((j) + b)]
+    if (weno_order == 3) {
+        /* m_weno.fpp:194-217 */
+        for (i = -b - 1 + polyn; i <= s + b - 1 - polyn; i++) {
+            const double S0 = cb[i], S1 = cb[i + 1], S2 = cb[i + 2], Sm1 = cb[i - 1];
+            const int j = i + 1;
+            C(6 + 0, j) = (S0 - S1) / (S0 - S2);  /* cbR(0,0) */
+            C(6 + 2, j) = (S0 - S1) / (Sm1 - S1); /* cbR(1,0) */
+            C(0 + 0, j) = -C(6 + 0, j);
+            C(0 + 2, j) = -C(6 + 2, j);
+            C(15 + 0, j) = (Sm1 - S1) / (Sm1 - S2); /* d_cbR(0) */
+            C(12 + 0, j) = (Sm1 - S0) / (Sm1 - S2); /* d_cbL(0) */
+            C(15 + 1, j) = 1.0 - C(15 + 0, j);
+            C(12 + 1, j) = 1.0 - C(12 + 0, j);
+            C(18 + 0, j) = 4.0 * sq(S0 - S1) / sq(S0 - S2);  /* beta(0,0) */
+            C(18 + 3, j) = 4.0 * sq(S0 - S1) / sq(Sm1 - S1); /* beta(1,0) */
+        }
+        return 0;
+    }
+    /* WENO5: m_weno.fpp:223-346 */
+    for (i = -b - 1 + polyn; i <= s + b - 1 - polyn; i++) {
+        const double S0 = cb[i], S1 = cb[i + 1], S2 = cb[i + 2], S3 = cb[i + 3];
+        const double Sm1 = cb[i - 1], Sm2 = cb[i - 2];
+        const int j = i + 1;
+        const double h4 = 4.0 * sq(S0 - S1);
+        /* poly_coef_cbR(j,0,0), (1,0), (1,1), (2,1) */
+        C(6 + 0, j) = ((S0 - S1) * (S1 - S2)) / ((S0 - S3) * (S3 - S1));
+        C(6 + 2, j) = ((Sm1 - S1) * (S1 - S0)) / ((Sm1 - S2) * (S2 - S0));
+        C(6 + 3, j) = ((S0 - S1) * (S1 - S2)) / ((Sm1 - S1) * (Sm1 - S2));
+        C(6 + 5, j) = ((S0 - S1) * (S1 - Sm1)) / ((Sm2 - S0) * (Sm2 - S1));
+        /* poly_coef_cbL(j,0,0), (1,0), (1,1), (2,1) */
+        C(0 + 0, j) = ((S1 - S0) * (S0 - S2)) / ((S0 - S3) * (S3 - S1));
+        C(0 + 2, j) = ((S0 - Sm1) * (S0 - S1)) / ((Sm1 - S2) * (S0 - S2));
+        C(0 + 3, j) = ((S1 - S0) * (S0 - S2)) / ((Sm1 - S1) * (Sm1 - S2));
+        C(0 + 5, j) = ((Sm1 - S0) * (S0 - S1)) / ((Sm2 - S0) * (Sm2 - S1));
+        /* poly_coef_cbR(j,0,1), (2,0); cbL(j,0,1), (2,0) */
+        C(6 + 1, j) = ((S0 - S2) + (S1 - S3)) / ((S0 - S2) * (S0 - S3)) * ((S0 - S1));
+        C(6 + 4, j) = ((Sm2 - S1) + (Sm1 - S1)) / ((Sm1 - S1) * (S1 - Sm2)) * ((S1 - S0));
+        C(0 + 1, j) = ((S0 - S2) + (S0 - S3)) / ((S0 - S2) * (S0 - S3)) * ((S1 - S0));
+        C(0 + 4, j) = ((Sm2 - S0) + (Sm1 - S1)) / ((Sm2 - S1) * (S1 - Sm1)) * ((S0 - S1));
+        /* ideal weights d_cbR(0), d_cbR(2), d_cbL(0), d_cbL(2), then (1) */
+        C(15 + 0, j) = ((Sm2 - S1) * (S1 - Sm1)) / ((Sm2 - S3) * (S3 - Sm1));
+        C(15 + 2, j) = ((S1 - S2) * (S1 - S3)) / ((Sm2 - S2) * (Sm2 - S3));
+        C(12 + 0, j) = ((Sm2 - S0) * (S0 - Sm1)) / ((Sm2 - S3) * (S3 - Sm1));
+        C(12 + 2, j) = ((S0 - S2) * (S0 - S3)) / ((Sm2 - S2) * (Sm2 - S3));
+        C(15 + 1, j) = 1.0 - C(15 + 0, j) - C(15 + 2, j);
+        C(12 + 1, j) = 1.0 - C(12 + 0, j) - C(12 + 2, j);
+        /* smoothness-indicator coefficients beta_coef(j,r,q) */
+        C(18 + 0, j) = h4 * (10.0 * sq(S1 - S0) + (S1 - S0) * (S2 - S1) + sq(S2 - S1)) /
+                       (sq(S0 - S3) * sq(S1 - S3));
+        C(18 + 1, j) = h4 *
+                       (19.0 * sq(S1 - S0) - (S1 - S0) * (S3 - S1) +
+                        2.0 * (S2 - S0) * ((S2 - S0) + (S3 - S1))) /
+                       ((S0 - S2) * sq(S0 - S3) * (S3 - S1));
+        C(18 + 2, j) = h4 *
+                       (10.0 * sq(S1 - S0) + (S1 - S0) * ((S2 - S0) + (S3 - S1)) +
+                        sq((S2 - S0) + (S3 - S1))) /
+                       (sq(S0 - S2) * sq(S0 - S3));
+        C(18 + 3, j) = h4 * (10.0 * sq(S1 - S0) + sq(S0 - Sm1) + (S0 - Sm1) * (S1 - S0)) /
+                       (sq(Sm1 - S2) * sq(S0 - S2));
+        C(18 + 4, j) = h4 *
+                       ((S0 - S1) * ((S0 - Sm1) + 20.0 * (S1 - S0)) +
+                        (2.0 * (S0 - Sm1) + (S1 - S0)) * (S2 - S0)) /
+                       ((S1 - Sm1) * sq(Sm1 - S2) * (S2 - S0));
+        C(18 + 5, j) = h4 * (10.0 * sq(S1 - S0) + (S1 - S0) * (S2 - S1) + sq(S2 - S1)) /
+                       (sq(Sm1 - S1) * sq(Sm1 - S2));
+        C(18 + 6, j) = h4 *
+                       (12.0 * sq(S1 - S0) + sq((S0 - Sm2) + (S0 - Sm1)) +
+                        3.0 * ((S0 - Sm2) + (S0 - Sm1)) * (S1 - S0)) /
+                       (sq(Sm2 - S1) * sq(Sm1 - S1));
+        C(18 + 7, j) = h4 *
+                       (19.0 * sq(S1 - S0) + ((S0 - Sm2) * (S0 - S1)) +
+                        2.0 * (S1 - Sm1) * ((S0 - Sm2) + (S1 - Sm1))) /
+                       ((Sm2 - S0) * sq(Sm2 - S1) * (S1 - Sm1));
+        C(18 + 8, j) = h4 * (10.0 * sq(S1 - S0) + sq(S0 - Sm1) + (S0 - Sm1) * (S1 - S0)) /
+                       (sq(Sm2 - S0) * sq(Sm2 - S1));
+    }
+#undef C
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO reconstruction of one variable at one cell: m_weno.fpp:391-540       */
+
+void orc_weno_cell(int weno_order, double weno_eps, const double *cf, const double *v, double *vL,
+                   double *vR) {
+    const double vj = v[2];
+    if (weno_order == 1) { /* m_weno.fpp:391-414 */
+        *vL = vj;
+        *vR = vj;
+        return;
+    }
+    if (weno_order == 3) { /* m_weno.fpp:416-465 */
+        const double d0 = v[3] - v[2], dm1 = v[2] - v[1];
+        double poly0, poly1, beta0, beta1, a0, a1, sum;
+        poly0 = vj + cf[0] * d0;
+        poly1 = vj + cf[2] * dm1;
+        beta0 = cf[18] * d0 * d0 + weno_eps;
+        beta1 = cf[21] * dm1 * dm1 + weno_eps;
+        a0 = cf[12] / (beta0 * beta0);
+        a1 = cf[13] / (beta1 * beta1);
+        sum = a0 + a1;
+        *vL = (a0 / sum) * poly0 + (a1 / sum) * poly1;
+        poly0 = vj + cf[6] * d0;
+        poly1 = vj + cf[8] * dm1;
+        a0 = cf[15] / (beta0 * beta0);
+        a1 = cf[16] / (beta1 * beta1);
+        sum = a0 + a1;
+        *vR = (a0 / sum) * poly0 + (a1 / sum) * poly1;
+        return;
+    }
+    /* WENO5: m_weno.fpp:476-531 */
+    {
+        const double d1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];
+        double p0, p1, p2, b0, b1, b2, a0, a1, a2, sum, w0, w1, w2;
+        p0 = vj + cf[0] * d1 + cf[1] * d0;
+        p1 = vj + cf[2] * d0 + cf[3] * dm1;
+        p2 = vj + cf[4] * dm1 + cf[5] * dm2;
+        b0 = cf[18] * d1 * d1 + cf[19] * d1 * d0 + cf[20] * d0 * d0 + weno_eps;
+        b1 = cf[21] * d0 * d0 + cf[22] * d0 * dm1 + cf[23] * dm1 * dm1 + weno_eps;
+        b2 = cf[24] * dm1 * dm1 + cf[25] * dm1 * dm2 + cf[26] * dm2 * dm2 + weno_eps;
+        a0 = cf[12] / (b0 * b0);
+        a1 = cf[13] / (b1 * b1);
+        a2 = cf[14] / (b2 * b2);
+        sum = a0 + a1 + a2;
+        w0 = a0 / sum;
+        w1 = a1 / sum;
+        w2 = a2 / sum;
+        *vL = w0 * p0 + w1 * p1 + w2 * p2;
+        p0 = vj + cf[6] * d1 + cf[7] * d0;
+        p1 = vj + cf[8] * d0 + cf[9] * dm1;
+        p2 = vj + cf[10] * dm1 + cf[11] * dm2;
+        a0 = cf[15] / (b0 * b0);
+        a1 = cf[16] / (b1 * b1);
+        a2 = cf[17] / (b2 * b2);
+        sum = a0 + a1 + a2;
+        w0 = a0 / sum;
+        w1 = a1 / sum;
+        w2 = a2 / sum;
+        *vR = w0 * p0 + w1 * p1 + w2 * p2;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* HLLC at one face: m_riemann_solvers.fpp:134-325                           */
+
+static inline double fmin_ref(double a, double b) { return (a < b) ? a : b; }
+static inline double fmax_ref(double a, double b) { return (a > b) ? a : b; }
+
+void orc_hllc_face(const orc_case *c, int dir, const double *qL, const double *qR, double *flux,
+                   double *ustar) {
+    const int nf = c->num_fluids, D = orc_num_dims(c);
+    const int contxe = nf - 1, E = nf + D, advxb = nf + D + 1;
+    double velL[3], velR[3];
+    double vel_L_rms = 0.0, vel_R_rms = 0.0;
+    double rho_L = 0.0, gamma_L = 0.0, pi_inf_L = 0.0;
+    double rho_R = 0.0, gamma_R = 0.0, pi_inf_R = 0.0;
+    double pres_L, pres_R, E_L, E_R, H_L, H_R, c_L, c_R;
+    double s_L, s_R, s_S, s_M, s_P, xi_L, xi_R, xi_M, xi_P;
+    double uL, uR;
+    int i;
+    for (i = 0; i < D; i++) { /* :139-145 */
+        velL[i] = qL[contxe + 1 + i];
+        velR[i] = qR[contxe + 1 + i];
+        vel_L_rms = vel_L_rms + velL[i] * velL[i];
+        vel_R_rms = vel_R_rms + velR[i] * velR[i];
+    }
+    pres_L = qL[E];
+    pres_R = qR[E];
+    for (i = 0; i < nf; i++) { /* :158-167 */
+        rho_L = rho_L + qL[i];
+        gamma_L = gamma_L + qL[E + 1 + i] * c->gammas[i];
+        pi_inf_L = pi_inf_L + qL[E + 1 + i] * c->pi_infs[i];
+        rho_R = rho_R + qR[i];
+        gamma_R = gamma_R + qR[E + 1 + i] * c->gammas[i];
+        pi_inf_R = pi_inf_R + qR[E + 1 + i] * c->pi_infs[i];
+    }
+    E_L = gamma_L * pres_L + pi_inf_L + 0.5 * rho_L * vel_L_rms; /* :202-204 */
+    E_R = gamma_R * pres_R + pi_inf_R + 0.5 * rho_R * vel_R_rms;
+    H_L = (E_L + pres_L) / rho_L;
+    H_R = (E_R + pres_R) / rho_R;
+    /* c_avg (:209-218) is computed by the reference but never used. */
+    c_L = ((H_L - 0.5 * vel_L_rms) / gamma_L); /* :220-223 */
+    c_R = ((H_R - 0.5 * vel_R_rms) / gamma_R);
+    c_L = sqrt(c_L);
+    c_R = sqrt(c_R);
+    uL = velL[dir];
+    uR = velR[dir];
+    s_L = fmin_ref(uL - c_L, uR - c_R); /* :232-233 */
+    s_R = fmax_ref(uR + c_R, uL + c_L);
+    s_S = (pres_R - pres_L + rho_L * uL * (s_L - uL) - rho_R * uR * (s_R - uR)) /
+          (rho_L * (s_L - uL) - rho_R * (s_R - uR)); /* :235-240 */
+    s_M = fmin_ref(0.0, s_L);
+    s_P = fmax_ref(0.0, s_R);
+    xi_L = (s_L - uL) / (s_L - s_S);
+    xi_R = (s_R - uR) / (s_R - s_S);
+    xi_M = (0.5 + copysign(0.5, s_S)); /* sign(5d-1, s_S): -0.0 gives -0.5 */
+    xi_P = (0.5 - copysign(0.5, s_S));
+    for (i = 0; i <= contxe; i++) /* :258-264 */
+        flux[i] = xi_M * qL[i] * (uL + s_M * (xi_L - 1.0)) + xi_P * qR[i] * (uR + s_P * (xi_R - 1.0));
+    for (i = 0; i < D; i++) { /* :270-286 */
+        const double flg = (i == dir) ? 1.0 : 0.0;
+        flux[contxe + 1 + i] =
+            xi_M * (rho_L * (uL * velL[i] + s_M * (xi_L * (flg * s_S + (1.0 - flg) * velL[i]) - velL[i])) +
+                    flg * (pres_L)) +
+            xi_P * (rho_R * (uR * velR[i] + s_P * (xi_R * (flg * s_S + (1.0 - flg) * velR[i]) - velR[i])) +
+                    flg * (pres_R));
+    }
+    flux[E] = xi_M * (uL * (E_L + pres_L) + s_M * (xi_L * (E_L + (s_S - uL) * (rho_L * s_S + pres_L / (s_L - uL))) - E_L)) +
+              xi_P * (uR * (E_R + pres_R) + s_P * (xi_R * (E_R + (s_S - uR) * (rho_R * s_S + pres_R / (s_R - uR))) - E_R));
+    for (i = advxb; i < advxb + nf; i++) /* :304-310 */
+        flux[i] = xi_M * qL[i] * (uL + s_M * (xi_L - 1.0)) + xi_P * qR[i] * (uR + s_P * (xi_R - 1.0));
+    /* contact velocity, :314-325 (dir_flg = 1 for the normal component) */
+    *ustar = xi_M * (uL + 1.0 * s_M * (xi_L - 1.0)) + xi_P * (uR + 1.0 * s_P * (xi_R - 1.0));
+}
+
+/* ------------------------------------------------------------------------ */
+/* cons -> prim: m_variables_conversion.fpp:313-375 (+ :187-227, :98-106)     */
+
+static void cons_to_prim_cell(const orc_case *c, const geom *g, const double *qc, double *qp,
+                              long o) {
+    double rho_K = 0.0, gamma_K = 0.0, pi_inf_K = 0.0, dyn_pres_K = 0.0;
+    int i;
+    for (i = 0; i <= g->contxe; i++) qp[i * g->ng + o] = qc[i * g->ng + o];
+    for (i = 0; i < g->nf; i++) {
+        rho_K = rho_K + qc[i * g->ng + o];
+        gamma_K = gamma_K + qc[(g->advxb + i) * g->ng + o] * c->gammas[i];
+        pi_inf_K = pi_inf_K + qc[(g->advxb + i) * g->ng + o] * c->pi_infs[i];
+    }
+    rho_K = fmax_ref(rho_K, SGM_EPS);
+    for (i = g->momxb; i < g->momxb + g->D; i++) {
+        qp[i * g->ng + o] = qc[i * g->ng + o] / rho_K;
+        dyn_pres_K = dyn_pres_K + 0.5 * qc[i * g->ng + o] * qp[i * g->ng + o];
+    }
+    qp[g->E_idx * g->ng + o] = (qc[g->E_idx * g->ng + o] - dyn_pres_K - pi_inf_K) / gamma_K;
+    for (i = g->advxb; i < g->advxb + g->nf; i++) qp[i * g->ng + o] = qc[i * g->ng + o];
+}
+
+int orc_cons_to_prim(const orc_case *c, const double *q_cons, double *q_prim) {
+    geom g;
+    long o;
+    mkgeom(c, &g);
+    for (o = 0; o < g.ng; o++) cons_to_prim_cell(c, &g, q_cons, q_prim, o);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ghost fill: m_rhs.fpp:686-908 (x over interior rows, then y over all x,    */
+/* then z over all x and y -- the z part is the 3D generalisation)           */
+
+static int bc_src(int bc_beg, int bc_end, int s, int i, int *flip) {
+    *flip = 0;
+    if (i < 0) {
+        if (bc_beg <= -3) return 0;
+        if (bc_beg == -2) { *flip = 1; return -i - 1; }
+        return s + 1 + i; /* periodic: q(-j) = q(s-(j-1)) */
+    }
+    if (bc_end <= -3) return s;
+    if (bc_end == -2) { *flip = 1; return 2 * s + 1 - i; }
+    return i - (s + 1); /* periodic: q(s+j) = q(j-1) */
+}
+
+int orc_bc_fill(const orc_case *c, double *q) {
+    geom g;
+    int i, j, k, v, flip, src;
+    mkgeom(c, &g);
+    for (i = 0; i < 3; i++) {
+        if (i < g.D && (c->bc_beg[i] < -12 || c->bc_beg[i] > -1 || c->bc_end[i] < -12 || c->bc_end[i] > -1))
+            return -1;
+    }
+    /* x: rows j in [0,n], k in [0,p] */
+    for (k = 0; k <= g.p; k++)
+        for (j = 0; j <= g.n; j++)
+            for (i = -g.b; i <= g.m + g.b; i++) {
+                if (i >= 0 && i <= g.m) continue;
+                src = bc_src(c->bc_beg[0], c->bc_end[0], g.m, i, &flip);
+                for (v = 0; v < g.S; v++) {
+                    double val = q[v * g.ng + gidx(&g, src, j, k)];
+                    q[v * g.ng + gidx(&g, i, j, k)] = (flip && v == g.momxb) ? -val : val;
+                }
+            }
+    if (g.D < 2) return 0;
+    /* y: all x in [-b, m+b], k in [0,p] */
+    for (k = 0; k <= g.p; k++)
+        for (j = -g.b; j <= g.n + g.b; j++) {
+            if (j >= 0 && j <= g.n) continue;
+            src = bc_src(c->bc_beg[1], c->bc_end[1], g.n, j, &flip);
+            for (i = -g.b; i <= g.m + g.b; i++)
+                for (v = 0; v < g.S; v++) {
+                    double val = q[v * g.ng + gidx(&g, i, src, k)];
+                    q[v * g.ng + gidx(&g, i, j, k)] = (flip && v == g.momxb + 1) ? -val : val;
+                }
+        }
+    if (g.D < 3) return 0;
+    /* z: all x and y including ghosts */
+    for (k = -g.b; k <= g.p + g.b; k++) {
+        if (k >= 0 && k <= g.p) continue;
+        src = bc_src(c->bc_beg[2], c->bc_end[2], g.p, k, &flip);
+        for (j = -g.b; j <= g.n + g.b; j++)
+            for (i = -g.b; i <= g.m + g.b; i++)
+                for (v = 0; v < g.S; v++) {
+                    double val = q[v * g.ng + gidx(&g, i, j, src)];
+                    q[v * g.ng + gidx(&g, i, j, k)] = (flip && v == g.momxb + 2) ? -val : val;
+                }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* s_compute_rhs: m_rhs.fpp:405-679                                          */
+
+int orc_compute_rhs(const orc_case *c, const double *const cb[3], const double *const ds[3],
+                    const double *q_cons, double *q_prim, double *rhs, int t_step) {
+    geom g;
+    double *qc, *qp, *coef, *vL, *vR, *flux, *ustar;
+    int dir, v;
+    mkgeom(c, &g);
+    if (c->weno_order != 1 && c->weno_order != 3 && c->weno_order != 5) return -1;
+    qc = (double *)malloc(sizeof(double) * g.S * g.ng);
+    qp = (double *)malloc(sizeof(double) * g.S * g.ng);
+    if (!qc || !qp) { free(qc); free(qp); return -2; }
+    /* :425-432 private copy, :435 ghost fill, :445-447 conversion */
+    memcpy(qc, q_cons, sizeof(double) * g.S * g.ng);
+    if (orc_bc_fill(c, qc) != 0) { free(qc); free(qp); return -1; }
+    orc_cons_to_prim(c, qc, qp);
+    if (t_step == c->t_step_stop) { free(qc); free(qp); return 1; } /* :452 */
+
+    for (dir = 0; dir < g.D; dir++) {
+        /* Extents along the sweep direction (s) and the two transverse ones. */
+        const int s = dir == 0 ? g.m : (dir == 1 ? g.n : g.p);
+        const int t1 = dir == 0 ? g.n : g.m;                  /* first transverse  */
+        const int t2 = dir == 2 ? g.n : g.p;                  /* second transverse */
+        const long nc = s + 2 * g.b + 1;
+        const long ns = s + 3;                                /* cells -1..s+1 */
+        const int a = 0;
+        int j, k1, k2, q;
+        coef = (double *)malloc(sizeof(double) * 27 * nc);
+        vL = (double *)malloc(sizeof(double) * g.S * ns);
+        vR = (double *)malloc(sizeof(double) * g.S * ns);
+        flux = (double *)malloc(sizeof(double) * g.S * (s + 2));
+        ustar = (double *)malloc(sizeof(double) * (s + 2));
+        orc_weno_coefficients(s, g.b, c->weno_order, cb[dir], coef);
+        (void)a;
+        for (k2 = 0; k2 <= t2; k2++)
+            for (k1 = 0; k1 <= t1; k1++) {
+                /* physical (i,j,k) of sweep position jj */
+#define POS(jj, I, J, K)                                                   \
+    do {                                                                   \
+        if (dir == 0) { I = (jj); J = k1; K = k2; }                        \
+        else if (dir == 1) { I = k1; J = (jj); K = k2; }                   \
+        else { I = k1; J = k2; K = (jj); }                                 \
+    } while (0)
+                /* WENO (m_weno.fpp:470-536) for cells -1..s+1; the reference
+                 * also reconstructs further ghost cells and ghost rows whose
+                 * values never reach the interior fluxes. */
+                for (j = -1; j <= s + 1; j++) {
+                    double cf[27];
+                    for (q = 0; q < 27; q++) cf[q] = coef[q * nc + j + g.b];
+                    for (v = 0; v < g.S; v++) {
+                        double st[5];
+                        int r;
+                        for (r = -2; r <= 2; r++) {
+                            int I, J, K, jj = j + r;
+                            if (c->weno_order == 1 && r != 0) { st[r + 2] = 0.0; continue; }
+                            if (c->weno_order == 3 && (r < -1 || r > 1)) { st[r + 2] = 0.0; continue; }
+                            POS(jj, I, J, K);
+                            st[r + 2] = qp[v * g.ng + gidx(&g, I, J, K)];
+                        }
+                        orc_weno_cell(c->weno_order, c->weno_eps, cf, st, &vL[v * ns + (j + 1)],
+                                      &vR[v * ns + (j + 1)]);
+                    }
+                }
+                /* Riemann-state buffers for bc = -4 (m_riemann_solvers.fpp:480-550) */
+                if (c->bc_beg[dir] == -4)
+                    for (v = 0; v < g.S; v++) vR[v * ns + 0] = vL[v * ns + 1];
+                if (c->bc_end[dir] == -4)
+                    for (v = 0; v < g.S; v++) vL[v * ns + (s + 2)] = vR[v * ns + (s + 1)];
+                /* HLLC at faces j+1/2, j in [-1, s]; the caller passes the
+                 * right-face value of cell j as the solver's left state
+                 * (m_rhs.fpp:545-556). */
+                for (j = -1; j <= s; j++) {
+                    double ql[3 * ORC_NUM_FLUIDS_MAX + 4], qr[3 * ORC_NUM_FLUIDS_MAX + 4], f[3 * ORC_NUM_FLUIDS_MAX + 4];
+                    for (v = 0; v < g.S; v++) {
+                        ql[v] = vR[v * ns + (j + 1)];
+                        qr[v] = vL[v * ns + (j + 2)];
+                    }
+                    orc_hllc_face(c, dir, ql, qr, f, &ustar[j + 1]);
+                    for (v = 0; v < g.S; v++) flux[v * (s + 2) + (j + 1)] = f[v];
+                }
+                /* flux divergence and alpha*div(u) (m_rhs.fpp:565-652) */
+                for (v = 0; v < g.S; v++)
+                    for (j = 0; j <= s; j++) {
+                        int I, J, K;
+                        long r;
+                        POS(j, I, J, K);
+                        r = (long)v * (g.m + 1) * (g.n + 1) * (g.p + 1) + ridx(&g, I, J, K);
+                        if (dir == 0)
+                            rhs[r] = 1.0 / ds[dir][j + g.b] * (flux[v * (s + 2) + j] - flux[v * (s + 2) + j + 1]);
+                        else
+                            rhs[r] = rhs[r] + 1.0 / ds[dir][j + g.b] * (flux[v * (s + 2) + j] - flux[v * (s + 2) + j + 1]);
+                    }
+                for (v = g.advxb; v < g.advxb + g.nf; v++)
+                    for (j = 0; j <= s; j++) {
+                        int I, J, K;
+                        long r;
+                        POS(j, I, J, K);
+                        r = (long)v * (g.m + 1) * (g.n + 1) * (g.p + 1) + ridx(&g, I, J, K);
+                        rhs[r] = rhs[r] + 1.0 / ds[dir][j + g.b] * qc[v * g.ng + gidx(&g, I, J, K)] *
+                                              (ustar[j + 1] - ustar[j]);
+                    }
+#undef POS
+            }
+        free(coef); free(vL); free(vR); free(flux); free(ustar);
+    }
+    if (q_prim) memcpy(q_prim, qp, sizeof(double) * g.S * g.ng); /* :659-675 */
+    free(qc);
+    free(qp);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* TVD RK: m_time_steppers.fpp:129-362                                       */
+
+int orc_tvd_rk_step(const orc_case *c, const double *const cb[3], const double *const ds[3],
+                    double dt, double *q1, double *q2, double *q_prim, double *rhs, int t_step) {
+    geom g;
+    int v, i, j, k, rc;
+    mkgeom(c, &g);
+    const long nr = (long)(g.m + 1) * (g.n + 1) * (g.p + 1);
+#define LOOP for (v = 0; v < g.S; v++) for (k = 0; k <= g.p; k++) for (j = 0; j <= g.n; j++) for (i = 0; i <= g.m; i++)
+#define Q(a) a[v * g.ng + gidx(&g, i, j, k)]
+#define R rhs[v * nr + ridx(&g, i, j, k)]
+    rc = orc_compute_rhs(c, cb, ds, q1, q_prim, rhs, t_step);
+    if (rc < 0) return rc;
+    if (t_step == c->t_step_stop) return 1;
+    if (c->time_stepper == 1) {
+        LOOP Q(q1) = Q(q1) + dt * R;
+        return 0;
+    }
+    LOOP Q(q2) = Q(q1) + dt * R;
+    rc = orc_compute_rhs(c, cb, ds, q2, q_prim, rhs, t_step);
+    if (rc < 0) return rc;
+    if (c->time_stepper == 2) {
+        LOOP Q(q1) = (Q(q1) + Q(q2) + dt * R) / 2.0;
+        return 0;
+    }
+    LOOP Q(q2) = (3.0 * Q(q1) + Q(q2) + dt * R) / 4.0;
+    rc = orc_compute_rhs(c, cb, ds, q2, q_prim, rhs, t_step);
+    if (rc < 0) return rc;
+    LOOP Q(q1) = (Q(q1) + 2.0 * Q(q2) + 2.0 * dt * R) / 3.0;
+#undef LOOP
+#undef Q
+#undef R
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ICFL: m_data_output.fpp:197-251                                           */
+
+int orc_icfl_max(const orc_case *c, const double *const ds[3], const double *q_prim, double dt,
+                 double *icfl_max) {
+    geom g;
+    int i, j, k, f;
+    double mx = 0.0;
+    int any = 0, nan = 0;
+    mkgeom(c, &g);
+    for (k = 0; k <= g.p; k++)
+        for (j = 0; j <= g.n; j++)
+            for (i = 0; i <= g.m; i++) {
+                const long o = gidx(&g, i, j, k);
+                double rho = 0.0, gamma = 0.0, pi_inf = 0.0, vel[3], pres, cc, icfl;
+                for (f = 0; f < g.nf; f++) {
+                    rho = rho + q_prim[f * g.ng + o];
+                    gamma = gamma + q_prim[(g.advxb + f) * g.ng + o] * c->gammas[f];
+                    pi_inf = pi_inf + q_prim[(g.advxb + f) * g.ng + o] * c->pi_infs[f];
+                }
+                for (f = 0; f < g.D; f++) vel[f] = q_prim[(g.momxb + f) * g.ng + o];
+                pres = q_prim[g.E_idx * g.ng + o];
+                cc = (((gamma + 1.0) * pres + pi_inf) / (gamma * rho));
+                cc = sqrt(cc);
+                if (g.D == 1) {
+                    icfl = (dt / ds[0][i + g.b]) * (fabs(vel[0]) + cc);
+                } else if (g.D == 2) {
+                    icfl = dt / fmin_ref(ds[0][i + g.b] / (fabs(vel[0]) + cc),
+                                         ds[1][j + g.b] / (fabs(vel[1]) + cc));
+                } else {
+                    icfl = dt / fmin_ref(fmin_ref(ds[0][i + g.b] / (fabs(vel[0]) + cc),
+                                                  ds[1][j + g.b] / (fabs(vel[1]) + cc)),
+                                         ds[2][k + g.b] / (fabs(vel[2]) + cc));
+                }
+                if (icfl != icfl) nan = 1;
+                else if (!any || icfl > mx) { mx = icfl; any = 1; }
+            }
+    *icfl_max = nan ? NAN : mx;
+    return 0;
+}

@@ -1,0 +1,100 @@
+"""Shared helpers for the parity tests: case builders and the comparison.
+
+The comparison metric is the per-variable relative L-infinity error
+max|a - b| / max|b| (a zero reference variable compares absolutely), with
+the FP64 bar 1e-12 per step; the number of differing bits is reported too.
+"""
+import sys
+
+import numpy as np
+
+from microfc_amd import ic
+from microfc_amd.rhs import Params
+from oracle import oracle as O
+
+FP64_TOL = 1e-12
+
+
+def to_oracle(prm: Params) -> O.Case:
+    return O.Case(m=prm.m, n=prm.n, p=prm.p, num_fluids=prm.num_fluids, weno_order=prm.weno_order,
+                  time_stepper=prm.time_stepper, t_step_stop=prm.t_step_stop, weno_eps=prm.weno_eps,
+                  gammas=list(prm.gammas), pi_infs=list(prm.pi_infs), bc_beg=tuple(prm.bc_beg),
+                  bc_end=tuple(prm.bc_end))
+
+
+def rel_linf(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    out = []
+    for v in range(a.shape[0]):
+        den = np.max(np.abs(b[v]))
+        num = np.max(np.abs(a[v] - b[v]))
+        out.append(num / den if den > 0 else num)
+    return np.array(out)
+
+
+def ulp_diff(a, b):
+    ai = np.asarray(a, dtype=np.float64).view(np.int64)
+    bi = np.asarray(b, dtype=np.float64).view(np.int64)
+    ai = np.where(ai < 0, np.int64(-2**63) - ai, ai)
+    bi = np.where(bi < 0, np.int64(-2**63) - bi, bi)
+    return int(np.max(np.abs(ai - bi))) if ai.size else 0
+
+
+def assert_parity(a, b, tol=FP64_TOL, what=""):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(b)), f"{what}: non-finite values"
+    err = rel_linf(a, b)
+    bitwise = bool(np.array_equal(a.view(np.int64), b.view(np.int64)))
+    print(f"  {what}: rel_linf max {err.max():.3e} bitwise={bitwise} max_ulp={ulp_diff(a, b)}", flush=True)
+    assert err.max() <= tol, f"{what}: rel L-inf {err} > {tol}"
+    return bitwise
+
+
+def perturbed(prm, q, seed=1, amp=1e-3):
+    """Multiplies the interior partial densities and energy by 1 + amp*U(-1,1)
+    so every cell carries distinct values (all variables stay physical)."""
+    rng = np.random.default_rng(seed)
+    qi = prm.interior(q)
+    nf, D = prm.num_fluids, prm.num_dims
+    for v in list(range(nf)) + [nf + D]:
+        qi[v] *= 1.0 + amp * rng.uniform(-1.0, 1.0, qi[v].shape)
+    return q
+
+
+def small_cases():
+    """(name, params, grids, q) for the parity matrix; sizes keep the oracle
+    to well under a second per RHS."""
+    out = []
+    prm, g, q = ic.sod_1d(m=199)
+    out.append(("sod1d_w5_rk3", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.sod_1d(m=99, weno_order=3)
+    prm.time_stepper = 2
+    out.append(("sod1d_w3_rk2", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.sod_1d(m=49, weno_order=1)
+    prm.time_stepper = 1
+    out.append(("sod1d_w1_rk1", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.sod_1d(m=127)
+    prm.bc_beg = (-2, -3, -3)
+    prm.bc_end = (-4, -3, -3)
+    g = ic.uniform_grids(prm)
+    out.append(("sod1d_reflect_riemannextrap", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shock_bubble_2d(N=64)
+    out.append(("bubble2d_64", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shock_bubble_2d(N=48)
+    prm.bc_beg = (-1, -2, -3)
+    prm.bc_end = (-1, -4, -3)
+    g = ic.uniform_grids(prm)
+    out.append(("bubble2d_periodic_reflect", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shear_layer_2d(N=64)
+    out.append(("shear2d_64_periodic", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shock_droplet_3d(N=28)
+    out.append(("droplet3d_28", prm, g, perturbed(prm, q)))
+    return out
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
