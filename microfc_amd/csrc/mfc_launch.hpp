@@ -20,14 +20,14 @@ int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
 #define MFC_SWEEP_LAUNCH_IMPL(NF)                                                                       \
     template <int D, int WO, typename R>                                                                \
     static int launch_nf##NF##_dwr(int dir, const SweepArgs &a, hipStream_t st) {                       \
-        const dim3 grid((unsigned)a.nblocks), block(64);                                                \
+        const dim3 grid((unsigned)a.nblocks);                                                           \
         if (dir == 0) {                                                                                 \
-            hipLaunchKernelGGL((k_sweep_x<NF, D, WO, R>), grid, block, 0, st, a);                       \
+            hipLaunchKernelGGL((k_sweep_x<NF, D, WO, R>), grid, dim3(64), 0, st, a);                    \
         } else if (dir == 1) {                                                                          \
-            if constexpr (D >= 2) hipLaunchKernelGGL((k_sweep_t<NF, D, 1, WO, R>), grid, block, 0, st, a); \
+            if constexpr (D >= 2) hipLaunchKernelGGL((k_sweep_t<NF, D, 1, WO, R>), grid, dim3(256), 0, st, a); \
             else return -1;                                                                             \
         } else {                                                                                        \
-            if constexpr (D >= 3) hipLaunchKernelGGL((k_sweep_t<NF, D, 2, WO, R>), grid, block, 0, st, a); \
+            if constexpr (D >= 3) hipLaunchKernelGGL((k_sweep_t<NF, D, 2, WO, R>), grid, dim3(256), 0, st, a); \
             else return -1;                                                                             \
         }                                                                                               \
         return hipGetLastError() == hipSuccess ? 0 : -2;                                                \

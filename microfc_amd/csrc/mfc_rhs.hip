@@ -64,7 +64,6 @@ struct State {
     hipStream_t own = nullptr, stream = nullptr;
     double *partials = nullptr; // kReduceBlocks doubles
     double *scratch_state = nullptr; // S*ng doubles, RK3 stage buffer
-    int chunk = 32;
 } G;
 
 // ---------------------------------------------------------------------------
@@ -520,8 +519,8 @@ SweepArgs sweep_args(int d) {
         a.nblocks = (long)a.ntile * (G.prm.n + 1) * (G.prm.p + 1);
     } else {
         const int s = G.dir[d].s;
-        a.ntile = (G.prm.m + 1 + 63) / 64;
-        a.chunk = G.chunk;
+        a.ntile = (G.prm.m + 1 + kTX - 1) / kTX;
+        a.chunk = kTS - 1;
         a.nchunk = (s + 1 + a.chunk - 1) / a.chunk;
         const int other = d == 1 ? G.prm.p + 1 : G.prm.n + 1;
         a.nblocks = (long)a.ntile * a.nchunk * other;
@@ -590,10 +589,6 @@ int mfc_rhs_init(const mfc_params *prm) {
     G.dir[0].s = prm->m;
     G.dir[1].s = prm->n;
     G.dir[2].s = prm->p;
-    if (const char *e = getenv("MFC_MARCH_CHUNK")) {
-        const int c = atoi(e);
-        if (c >= 1 && c <= 4096) G.chunk = c;
-    }
     HIPCHK(hipStreamCreateWithFlags(&G.own, hipStreamNonBlocking));
     G.stream = G.own;
     HIPCHK(hipMalloc(&G.partials, sizeof(double) * kReduceBlocks));
@@ -831,9 +826,7 @@ int mfc_rhs_finalize(void) {
     if (G.partials) hipFree(G.partials);
     if (G.scratch_state) hipFree(G.scratch_state);
     if (G.own) hipStreamDestroy(G.own);
-    const int chunk = G.chunk;
     G = State();
-    G.chunk = chunk;
     return 0;
 }
 

@@ -13,11 +13,10 @@
 //    grid row.  The 64+2*polyn+1 stencil cells are converted to primitive and
 //    staged in LDS; lane l computes face x0-1/2+l; fluxes go through LDS so
 //    lane l can difference faces l and l+1 for cell x0+l (63 cells per wave).
-//  * k_sweep_t (y or z, strided): one wave per 64 consecutive x-columns,
-//    marching along the sweep direction over a chunk of cells.  Each lane
-//    keeps the 2*polyn+1 stencil cells, the previous cell's right-face state
-//    and the previous face's flux in registers, so each step loads one cell
-//    (coalesced along x), reconstructs one cell and solves one face.
+//  * k_sweep_t (y or z, strided): one 256-thread block per tile of 16
+//    x-columns by 16 faces along the sweep (15 cells).  Stencil rows of 16
+//    consecutive x are staged in LDS, thread (x, f) solves face f of column
+//    x, and the fluxes go back through LDS for the divergence.
 #pragma once
 
 #include "mfc_device.hpp"
@@ -50,7 +49,7 @@ struct SweepArgs {
     int bc_beg, bc_end;   // of this direction
     int rk;               // RkKind
     int accumulate;       // 0: first direction assigns rhs, 1: adds
-    int chunk, nchunk;    // march chunking (k_sweep_t)
+    int chunk, nchunk;    // cells per tile along the sweep and tile count (k_sweep_t)
     int ntile;            // x tiles (k_sweep_x) or x column groups (k_sweep_t)
     long nblocks;
 };
@@ -218,105 +217,108 @@ __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// y / z sweep (DIR = 1 or 2): march along the sweep direction
+// y / z sweep (DIR = 1 or 2): a TX x TS tile of faces per 256-thread block.
+// Thread (tx, ts) computes face ts of column x0+tx; the stencil rows are
+// staged in LDS (rows of TX consecutive x, so every global load and store
+// is a contiguous 8*TX-byte segment) and the fluxes reuse the same LDS.
+constexpr int kTX = 16, kTS = 16;
+
 template <int NF, int D, int DIR, int WO, typename R>
-__global__ __launch_bounds__(64) void k_sweep_t(const SweepArgs a) {
+__global__ __launch_bounds__(256) void k_sweep_t(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int PN = (WO - 1) / 2;
+    constexpr int NR = kTS + 2 * PN + 1; // staged rows
+    constexpr int PRIM_WORDS = S * NR * kTX;
+    constexpr int FLUX_WORDS = (S + 1) * kTS * kTX;
+    __shared__ R lds[PRIM_WORDS > FLUX_WORDS ? PRIM_WORDS : FLUX_WORDS];
+    R(*sp)[NR][kTX] = reinterpret_cast<R(*)[NR][kTX]>(lds);
+    R(*sf)[kTS][kTX] = reinterpret_cast<R(*)[kTS][kTX]>(lds);
 
     const long bid = xcd_logical_block(blockIdx.x, a.nblocks);
     const int xt = (int)(bid % a.ntile);
     const long rest = bid / a.ntile;
     const int chunk = (int)(rest % a.nchunk);
     const int t = (int)(rest / a.nchunk);
-    const int lane = threadIdx.x;
-    const int i_raw = xt * 64 + lane;
-    const bool active = i_raw <= a.m;
-    const int i = active ? i_raw : a.m;
     const int s = (DIR == 1) ? a.n : a.p;
-    const int j0 = chunk * a.chunk;
-    const int j1 = min(j0 + a.chunk - 1, s);
+    const int x0 = xt * kTX;
+    const int y0 = chunk * (kTS - 1); // first cell of this tile along the sweep
+    const int tid = threadIdx.x;
+    const int tx = tid % kTX, ts = tid / kTX;
 
-    // ghosted offset of the cell at sweep index c of this column
-    auto gcell = [&](int c) -> long {
+    auto gcell = [&](int i, int c) -> long {
         return (DIR == 1) ? ghost_off(a, i, c, t) : ghost_off(a, i, t, c);
     };
-    auto rcell = [&](int c) -> long {
+    auto rcell = [&](int i, int c) -> long {
         return (DIR == 1) ? interior_off(a, i, c, t) : interior_off(a, i, t, c);
     };
 
-    R W[WO][S];
-    R vRp[S], Fprev[S], usprev = R(0);
-    const R eps = (R)a.eps;
-#pragma unroll
-    for (int w = 0; w < WO; ++w) {
+    // 1. stage primitive rows y0-1-PN .. y0+kTS-1+PN of columns x0..x0+kTX-1
+    for (int e = tid; e < NR * kTX; e += 256) {
+        const int ex = e % kTX, er = e / kTX;
+        const int c = y0 - 1 - PN + er;
+        const int i = min(x0 + ex, a.m);
         double pr[S];
-        load_prim<NF, D>(a, gcell(j0 - 1 - PN + w), pr);
+        if (c <= s + a.b) {
+            load_prim<NF, D>(a, gcell(i, c), pr);
+        } else {
 #pragma unroll
-        for (int v = 0; v < S; ++v) W[w][v] = (R)pr[v];
+            for (int v = 0; v < S; ++v) pr[v] = 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < S; ++v) sp[v][er][ex] = (R)pr[v];
     }
-    {
-        R cf[kNCoef];
-        load_coef<R, WO>(a, j0 - 1, cf);
+    __syncthreads();
+
+    // 2. face jf+1/2 with jf = y0-1+ts
+    const int jf = y0 - 1 + ts;
+    R f[S], us = R(0);
+    const bool face_ok = jf <= s;
+    if (face_ok) {
+        R cfR[kNCoef], cfL[kNCoef];
+        load_coef<R, WO>(a, jf, cfR);
+        load_coef<R, WO>(a, jf + 1, cfL);
+        R qL[S], qR[S];
+        const R eps = (R)a.eps;
 #pragma unroll
         for (int v = 0; v < S; ++v) {
             R st[WO], dummy;
 #pragma unroll
-            for (int w = 0; w < WO; ++w) st[w] = W[w][v];
-            weno_cell<WO, R, false, true>(st, cf, eps, dummy, vRp[v]);
+            for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + w][tx];
+            weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
+#pragma unroll
+            for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + 1 + w][tx];
+            weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
         }
+        if (jf == -1 && a.bc_beg == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qL[v] = qR[v];
+        }
+        if (jf == s && a.bc_end == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qR[v] = qL[v];
+        }
+        hllc_face<NF, D, DIR, R>(qL, qR, a.eos, f, us);
     }
+    __syncthreads(); // the stencil is dead; the fluxes take its place
+    if (face_ok) {
 #pragma unroll
-    for (int v = 0; v < S; ++v) Fprev[v] = R(0);
+        for (int v = 0; v < S; ++v) sf[v][ts][tx] = f[v];
+        sf[S][ts][tx] = us;
+    }
+    __syncthreads();
 
-    for (int c = j0; c <= j1 + 1; ++c) {
-        // shift the window and bring in cell c+PN
-#pragma unroll
-        for (int w = 0; w < WO - 1; ++w)
-#pragma unroll
-            for (int v = 0; v < S; ++v) W[w][v] = W[w + 1][v];
-        {
-            double pr[S];
-            load_prim<NF, D>(a, gcell(c + PN), pr);
-#pragma unroll
-            for (int v = 0; v < S; ++v) W[WO - 1][v] = (R)pr[v];
-        }
-        R vL[S], vR[S];
-        {
-            R cf[kNCoef];
-            load_coef<R, WO>(a, c, cf);
-#pragma unroll
-            for (int v = 0; v < S; ++v) {
-                R st[WO];
-#pragma unroll
-                for (int w = 0; w < WO; ++w) st[w] = W[w][v];
-                weno_cell<WO, R, true, true>(st, cf, eps, vL[v], vR[v]);
-            }
-        }
-        // face (c-1)+1/2: left state = right face of cell c-1, right state = left face of cell c
-        R qL[S], F[S], us;
-#pragma unroll
-        for (int v = 0; v < S; ++v) qL[v] = vRp[v];
-        if (c - 1 == -1 && a.bc_beg == -4) {
-#pragma unroll
-            for (int v = 0; v < S; ++v) qL[v] = vL[v];
-        }
-        if (c - 1 == s && a.bc_end == -4) {
-#pragma unroll
-            for (int v = 0; v < S; ++v) vL[v] = qL[v];
-        }
-        hllc_face<NF, D, DIR, R>(qL, vL, a.eos, F, us);
-        if (c - 1 >= j0 && active) {
-            const int cc = c - 1;
-            finish_cell<NF, D, R>(a, gcell(cc), rcell(cc), a.inv_ds[cc], Fprev, F, usprev, us);
-        }
+    // 3. cell y0+ts (ts < kTS-1)
+    const int i = x0 + tx, cc = y0 + ts;
+    if (ts < kTS - 1 && i <= a.m && cc <= s) {
+        R Fm[S], Fp[S];
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            Fprev[v] = F[v];
-            vRp[v] = vR[v];
+            Fm[v] = sf[v][ts][tx];
+            Fp[v] = sf[v][ts + 1][tx];
         }
-        usprev = us;
+        finish_cell<NF, D, R>(a, gcell(i, cc), rcell(i, cc), a.inv_ds[cc], Fm, Fp, sf[S][ts][tx],
+                              sf[S][ts + 1][tx]);
     }
 }
 
