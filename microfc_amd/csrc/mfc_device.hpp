@@ -29,6 +29,14 @@ struct Eos {
     double pi_infs[kNumFluidsMax];
 };
 
+// Division in the bulk precision: IEEE correctly rounded in FP64 (the plain
+// path must reproduce the reference bit for bit); in the FP32 bulk a
+// reciprocal-multiply (about 1 ulp), which is what that form trades for speed.
+__device__ __forceinline__ double bdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float bdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double bsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float bsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 template <typename T>
 __device__ __forceinline__ T min_ref(T a, T b) { return (a < b) ? a : b; }
 template <typename T>
@@ -90,18 +98,18 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
         if constexpr (WANT_L) {
             const R p0 = vj + cf[kCbL + 0] * d0;
             const R p1 = vj + cf[kCbL + 2] * dm1;
-            const R a0 = cf[kDL + 0] / (b0 * b0);
-            const R a1 = cf[kDL + 1] / (b1 * b1);
+            const R a0 = bdiv(cf[kDL + 0], b0 * b0);
+            const R a1 = bdiv(cf[kDL + 1], b1 * b1);
             const R s = a0 + a1;
-            vL = (a0 / s) * p0 + (a1 / s) * p1;
+            vL = bdiv(a0, s) * p0 + bdiv(a1, s) * p1;
         }
         if constexpr (WANT_R) {
             const R p0 = vj + cf[kCbR + 0] * d0;
             const R p1 = vj + cf[kCbR + 2] * dm1;
-            const R a0 = cf[kDR + 0] / (b0 * b0);
-            const R a1 = cf[kDR + 1] / (b1 * b1);
+            const R a0 = bdiv(cf[kDR + 0], b0 * b0);
+            const R a1 = bdiv(cf[kDR + 1], b1 * b1);
             const R s = a0 + a1;
-            vR = (a0 / s) * p0 + (a1 / s) * p1;
+            vR = bdiv(a0, s) * p0 + bdiv(a1, s) * p1;
         }
     } else {
         const R vj = v[2];
@@ -114,17 +122,17 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
             const R p0 = vj + cf[kCbL + 0] * d1 + cf[kCbL + 1] * d0;
             const R p1 = vj + cf[kCbL + 2] * d0 + cf[kCbL + 3] * dm1;
             const R p2 = vj + cf[kCbL + 4] * dm1 + cf[kCbL + 5] * dm2;
-            const R a0 = cf[kDL + 0] / bb0, a1 = cf[kDL + 1] / bb1, a2 = cf[kDL + 2] / bb2;
+            const R a0 = bdiv(cf[kDL + 0], bb0), a1 = bdiv(cf[kDL + 1], bb1), a2 = bdiv(cf[kDL + 2], bb2);
             const R s = a0 + a1 + a2;
-            vL = (a0 / s) * p0 + (a1 / s) * p1 + (a2 / s) * p2;
+            vL = bdiv(a0, s) * p0 + bdiv(a1, s) * p1 + bdiv(a2, s) * p2;
         }
         if constexpr (WANT_R) {
             const R p0 = vj + cf[kCbR + 0] * d1 + cf[kCbR + 1] * d0;
             const R p1 = vj + cf[kCbR + 2] * d0 + cf[kCbR + 3] * dm1;
             const R p2 = vj + cf[kCbR + 4] * dm1 + cf[kCbR + 5] * dm2;
-            const R a0 = cf[kDR + 0] / bb0, a1 = cf[kDR + 1] / bb1, a2 = cf[kDR + 2] / bb2;
+            const R a0 = bdiv(cf[kDR + 0], bb0), a1 = bdiv(cf[kDR + 1], bb1), a2 = bdiv(cf[kDR + 2], bb2);
             const R s = a0 + a1 + a2;
-            vR = (a0 / s) * p0 + (a1 / s) * p1 + (a2 / s) * p2;
+            vR = bdiv(a0, s) * p0 + bdiv(a1, s) * p1 + bdiv(a2, s) * p2;
         }
     }
 }
@@ -157,19 +165,19 @@ __device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &e
     }
     const R E_L = gamma_L * pres_L + pi_inf_L + R(0.5) * rho_L * vel_L_rms;
     const R E_R = gamma_R * pres_R + pi_inf_R + R(0.5) * rho_R * vel_R_rms;
-    const R H_L = (E_L + pres_L) / rho_L;
-    const R H_R = (E_R + pres_R) / rho_R;
-    const R c_L = sqrt((H_L - R(0.5) * vel_L_rms) / gamma_L);
-    const R c_R = sqrt((H_R - R(0.5) * vel_R_rms) / gamma_R);
+    const R H_L = bdiv(E_L + pres_L, rho_L);
+    const R H_R = bdiv(E_R + pres_R, rho_R);
+    const R c_L = bsqrt(bdiv(H_L - R(0.5) * vel_L_rms, gamma_L));
+    const R c_R = bsqrt(bdiv(H_R - R(0.5) * vel_R_rms, gamma_R));
     const R uL = qL[Y::MOM + DIR], uR = qR[Y::MOM + DIR];
     const R s_L = min_ref(uL - c_L, uR - c_R);
     const R s_R = max_ref(uR + c_R, uL + c_L);
-    const R s_S = (pres_R - pres_L + rho_L * uL * (s_L - uL) - rho_R * uR * (s_R - uR)) /
-                  (rho_L * (s_L - uL) - rho_R * (s_R - uR));
+    const R s_S = bdiv(pres_R - pres_L + rho_L * uL * (s_L - uL) - rho_R * uR * (s_R - uR),
+                       rho_L * (s_L - uL) - rho_R * (s_R - uR));
     const R s_M = min_ref(R(0), s_L);
     const R s_P = max_ref(R(0), s_R);
-    const R xi_L = (s_L - uL) / (s_L - s_S);
-    const R xi_R = (s_R - uR) / (s_R - s_S);
+    const R xi_L = bdiv(s_L - uL, s_L - s_S);
+    const R xi_R = bdiv(s_R - uR, s_R - s_S);
     // Fortran sign(0.5, s_S) takes the sign bit of s_S, so -0.0 gives -0.5.
     const R hs = copysign(R(0.5), s_S);
     const R xi_M = (R(0.5) + hs);
@@ -186,8 +194,8 @@ __device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &e
             xi_M * (rho_L * (uL * vLd + s_M * (xi_L * (flg * s_S + (R(1) - flg) * vLd) - vLd)) + flg * (pres_L)) +
             xi_P * (rho_R * (uR * vRd + s_P * (xi_R * (flg * s_S + (R(1) - flg) * vRd) - vRd)) + flg * (pres_R));
     }
-    flux[Y::E] = xi_M * (uL * (E_L + pres_L) + s_M * (xi_L * (E_L + (s_S - uL) * (rho_L * s_S + pres_L / (s_L - uL))) - E_L)) +
-                 xi_P * (uR * (E_R + pres_R) + s_P * (xi_R * (E_R + (s_S - uR) * (rho_R * s_S + pres_R / (s_R - uR))) - E_R));
+    flux[Y::E] = xi_M * (uL * (E_L + pres_L) + s_M * (xi_L * (E_L + (s_S - uL) * (rho_L * s_S + bdiv(pres_L, s_L - uL))) - E_L)) +
+                 xi_P * (uR * (E_R + pres_R) + s_P * (xi_R * (E_R + (s_S - uR) * (rho_R * s_S + bdiv(pres_R, s_R - uR))) - E_R));
 #pragma unroll
     for (int i = 0; i < NF; ++i) flux[Y::ADV + i] = xi_M * qL[Y::ADV + i] * mL + xi_P * qR[Y::ADV + i] * mR;
     // dir_flg = 1 for the normal component: xi_M*(u_L + 1*s_M*(xi_L-1)) + ...

@@ -21,6 +21,8 @@
 
 #include "mfc_device.hpp"
 
+#include <type_traits>
+
 namespace mfc {
 
 struct Fields {
@@ -75,6 +77,29 @@ __device__ __forceinline__ long xcd_logical_block(long bid, long nblocks) {
 template <typename R>
 __device__ __forceinline__ double to_d(R x) { return (double)x; }
 
+// TVD-RK stage combination of the base, the stage input and rhs.
+template <int S>
+__device__ __forceinline__ void rk_combine(const SweepArgs &a, long go, const double *r, double *o) {
+    switch (a.rk) {
+    case kRkEuler:
+#pragma unroll
+        for (int v = 0; v < S; ++v) o[v] = a.qbase.v[v][go] + a.dt * r[v];
+        break;
+    case kRk2Final:
+#pragma unroll
+        for (int v = 0; v < S; ++v) o[v] = (a.qbase.v[v][go] + a.qin.v[v][go] + a.dt * r[v]) / 2.0;
+        break;
+    case kRk3Stage2:
+#pragma unroll
+        for (int v = 0; v < S; ++v) o[v] = (3.0 * a.qbase.v[v][go] + a.qin.v[v][go] + a.dt * r[v]) / 4.0;
+        break;
+    default:
+#pragma unroll
+        for (int v = 0; v < S; ++v) o[v] = (a.qbase.v[v][go] + 2.0 * a.qin.v[v][go] + 2.0 * a.dt * r[v]) / 3.0;
+        break;
+    }
+}
+
 // Finishes one cell: flux divergence, alpha*div(u) source, then either the
 // rhs store or the fused RK stage (m_rhs.fpp:565-652, m_time_steppers.fpp).
 template <int NF, int D, typename R>
@@ -91,6 +116,19 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, long go, long ro
     const double dus = to_d(usp) - to_d(usm);
 #pragma unroll
     for (int i = 0; i < NF; ++i) r[Y::ADV + i] = r[Y::ADV + i] + inv * a.qin.v[Y::ADV + i][go] * dus;
+    if constexpr (!std::is_same<R, double>::value) {
+        // FP32 bulk: the RK stage accumulates in FP64 and clips the volume
+        // fractions to [0, 1] (positivity; the reference has no clip).
+        if (a.rk != kRkNone) {
+            double o[S];
+            rk_combine<S>(a, go, r, o);
+#pragma unroll
+            for (int i = 0; i < NF; ++i) o[Y::ADV + i] = min(max(o[Y::ADV + i], 0.0), 1.0);
+#pragma unroll
+            for (int v = 0; v < S; ++v) a.qout.v[v][go] = o[v];
+            return;
+        }
+    }
     switch (a.rk) {
     case kRkNone:
 #pragma unroll

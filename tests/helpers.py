@@ -33,6 +33,29 @@ def rel_linf(a, b):
     return np.array(out)
 
 
+def scaled_linf(prm, a, b):
+    """Per-variable L-inf error of a against b, normalised by a physical
+    scale: max|b_v| for partial densities and energy, the largest momentum
+    component for every momentum component, and the velocity-divergence
+    scale max|b_mass| / max|rho| for the volume fractions (whose reference
+    rhs is round-off small in single-fluid regions)."""
+    nf, D = prm.num_fluids, prm.num_dims
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    mom = np.max(np.abs(b[nf:nf + D]))
+    div = max(np.max(np.abs(b[i])) for i in range(nf))
+    out = []
+    for v in range(a.shape[0]):
+        if nf <= v < nf + D:
+            den = mom
+        elif v > nf + D:
+            den = max(np.max(np.abs(b[v])), div)
+        else:
+            den = np.max(np.abs(b[v]))
+        out.append(np.max(np.abs(a[v] - b[v])) / den)
+    return np.array(out)
+
+
 def ulp_diff(a, b):
     ai = np.asarray(a, dtype=np.float64).view(np.int64)
     bi = np.asarray(b, dtype=np.float64).view(np.int64)

@@ -1,0 +1,120 @@
+"""FP32-bulk form (WENO and HLLC in FP32 with a reciprocal-multiply division;
+cons->prim, flux divergence and the RK stages in FP64; volume fractions
+clipped to [0, 1] at each stage) against the FP64 oracle.
+
+The error against FP64 is printed and bounded per result:
+  * one RHS: per-variable L-inf error <= 1e-3 of the variable's scale
+    (helpers.scaled_linf: max|rhs| of the variable, the largest momentum
+    component for momenta, the velocity-divergence scale for alpha);
+  * 20 RK3 steps: per-variable L1 error <= 1e-5 of the variable's L1 norm
+    (momentum components share one norm);
+  * discrete conservation of mass/momentum/energy with periodic boundaries
+    holds to FP64 round-off (each FP32 face flux enters both neighbours);
+  * alpha stays in [0, 1]."""
+import numpy as np
+import pytest
+
+from microfc_amd import PREC_FP32_BULK, RhsModule, ic
+from oracle import oracle as O
+
+from helpers import perturbed, say, scaled_linf, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    out = []
+    prm, g, q = ic.shock_bubble_2d(N=64)
+    out.append(("bubble2d_64", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shear_layer_2d(N=64)
+    out.append(("shear2d_64", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.sod_1d(m=199)
+    out.append(("sod1d_200", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shock_droplet_3d(N=32)
+    out.append(("droplet3d_32", prm, g, perturbed(prm, q)))
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fp32_bulk_rhs_error(gpu, case):
+    torch = gpu
+    name, prm, grids, q = case
+    say(f"[fp32 rhs] {name}")
+    ref = O.compute_rhs(to_oracle(prm), grids, q)
+    prm.precision = PREC_FP32_BULK
+    mod = RhsModule(prm, grids)
+    try:
+        qd = torch.from_numpy(q.copy()).cuda()
+        rd = mod.new_field(ghosted=False)
+        mod.s_compute_rhs(qd, None, rd, 0)
+        mod.sync()
+        err = scaled_linf(prm, rd.cpu().numpy(), ref)
+        say(f"  {name} FP32-bulk rhs scaled L-inf error vs FP64 per variable: {np.array2string(err, precision=2)}")
+        assert np.all(np.isfinite(err)) and err.max() <= 1e-3
+    finally:
+        prm.precision = 0
+        mod.finalize()
+
+
+@pytest.mark.parametrize("case", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_fp32_bulk_steps_error(gpu, case):
+    torch = gpu
+    name, prm, grids, q = case
+    nsteps = 20
+    oc = to_oracle(prm)
+    prm.precision = PREC_FP32_BULK
+    mod = RhsModule(prm, grids)
+    try:
+        qd = torch.from_numpy(q.copy()).cuda()
+        dt = 0.4 * mod.cfl_dt(qd, 1.0)
+        say(f"[fp32 steps] {name} dt={dt:.3e} steps={nsteps}")
+        q2d = mod.new_field()
+        rd = mod.new_field(ghosted=False)
+        qo = q.copy()
+        q2o = np.zeros_like(qo)
+        for t in range(nsteps):
+            mod.s_tvd_rk(t, (qd, q2d), None, rd, dt)
+            O.tvd_rk_step(oc, grids, dt, qo, q2o, t_step=t)
+        mod.sync()
+        got = prm.interior(qd.cpu().numpy())
+        ref = prm.interior(qo)
+        nf, D = prm.num_fluids, prm.num_dims
+        mom = sum(np.sum(np.abs(ref[v])) for v in range(nf, nf + D))
+        l1 = np.array([np.sum(np.abs(got[v] - ref[v])) / (mom if nf <= v < nf + D else np.sum(np.abs(ref[v])))
+                       for v in range(prm.sys_size)])
+        say(f"  {name} FP32-bulk after {nsteps} RK3 steps, rel L1 vs FP64 per variable: "
+            f"{np.array2string(l1, precision=2)}")
+        nf, D = prm.num_fluids, prm.num_dims
+        alphas = got[nf + D + 1:]
+        assert alphas.min() >= 0.0 and alphas.max() <= 1.0
+        assert np.all(np.isfinite(got))
+        assert l1.max() <= 1e-5
+    finally:
+        prm.precision = 0
+        mod.finalize()
+
+
+def test_fp32_bulk_conserves_to_fp64_roundoff(gpu):
+    torch = gpu
+    prm, grids, q = ic.shear_layer_2d(N=64)
+    prm.bc_beg = (-1, -1, -1)
+    prm.bc_end = (-1, -1, -1)
+    grids = ic.uniform_grids(prm)
+    prm.precision = PREC_FP32_BULK
+    say("[fp32 conservation] shear2d_64 periodic")
+    mod = RhsModule(prm, grids)
+    try:
+        qd = torch.from_numpy(q.copy()).cuda()
+        rd = mod.new_field(ghosted=False)
+        mod.s_compute_rhs(qd, None, rd, 0)
+        mod.sync()
+        r = rd.cpu().numpy()
+        for v in range(4):
+            tot, mag = abs(np.sum(r[v])), np.sum(np.abs(r[v]))
+            say(f"  var {v}: |sum rhs| / sum|rhs| = {tot / mag:.2e}")
+            assert tot <= 1e-13 * mag
+    finally:
+        mod.finalize()
