@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--N", type=int, default=0, help="cells per direction (default: the case's size)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_bulk"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 2 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -87,6 +89,12 @@ def main():
         workload = f"1D Sod {N} cells, WENO5+HLLC+RK3"
     ncell = (prm.m + 1) * (prm.n + 1) * (prm.p + 1)
 
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     mod = RhsModule(prm, grids)
     q1 = torch.from_numpy(q).cuda()
     q2 = mod.new_field()
