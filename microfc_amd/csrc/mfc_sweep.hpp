@@ -273,13 +273,15 @@ __global__ __launch_bounds__(256) void k_sweep_t(const SweepArgs a) {
     R(*sp)[NR][kTX] = reinterpret_cast<R(*)[NR][kTX]>(lds);
     R(*sf)[kTS][kTX] = reinterpret_cast<R(*)[kTS][kTX]>(lds);
 
-    // Tiles along the sweep are the fastest block index, so the tiles that
-    // share stencil rows run back to back on the same XCD.
+    // x tiles are the fastest block index: neighbouring tiles share the
+    // 128-byte lines their 16-wide row segments straddle, so they run back to
+    // back on one XCD (ordering by sweep tile instead doubled the fetched
+    // bytes on MI355X).
     const long bid = xcd_logical_block(blockIdx.x, a.nblocks);
-    const int chunk = (int)(bid % a.nchunk);
-    const long rest = bid / a.nchunk;
-    const int xt = (int)(rest % a.ntile);
-    const int t = (int)(rest / a.ntile);
+    const int xt = (int)(bid % a.ntile);
+    const long rest = bid / a.ntile;
+    const int chunk = (int)(rest % a.nchunk);
+    const int t = (int)(rest / a.nchunk);
     const int s = (DIR == 1) ? a.n : a.p;
     const int x0 = xt * kTX;
     const int y0 = chunk * (kTS - 1); // first cell of this tile along the sweep
