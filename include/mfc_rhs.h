@@ -28,6 +28,19 @@ extern "C" {
 /* Arithmetic of the WENO/HLLC bulk. */
 #define MFC_PREC_FP64 0      /* plain path: FP64 throughout, reference evaluation order */
 #define MFC_PREC_FP32_BULK 1 /* WENO and HLLC in FP32; cons->prim, flux divergence and RK in FP64 */
+#define MFC_PREC_FP32_LEARNED 2 /* FP32 bulk with learned WENO5 weights (needs mfc_mlp_init) */
+
+/* Learned WENO5 weights: a 5 -> 32 -> 32 -> 3 MLP (ReLU) whose logits z
+ * modulate the plain weights, w_k ~ w_plain_k exp(z_k).  Parameters are one
+ * FP32 block of MFC_MLP_NPARAM floats: W1[16][32] (input rows >= 5 zero),
+ * b1[32], W2[32][32], b2[32], W3[32][16] (output columns >= 3 zero), b3[16].
+ * A training sample is 12 floats (structure of arrays, 12 planes of n): the
+ * normalised stencil x[5], the normalised candidate polynomials p[3], the
+ * plain weights w_plain[3] and the normalised fine-grid face value t; the
+ * loss is mean((sum_k w_k p_k - t)^2).  None of this has a counterpart in the
+ * reference; it replaces the nonlinear weights of src/simulation/m_weno.fpp:508-528. */
+#define MFC_MLP_NPARAM 2128
+#define MFC_MLP_SAMPLE_FIELDS 12
 
 /* Case parameters; names and meaning follow the namelist &user_inputs of
  * src/simulation/m_start_up.fpp:73-83 (p and bc_z are the 3D extension). */
@@ -110,6 +123,28 @@ int mfc_sync(void);
 void *mfc_stream(void);
 /* Queue all further work on a caller's HIP stream (NULL: the library's own). */
 int mfc_set_stream(void *stream);
+
+/* Loads parameters (host, MFC_MLP_NPARAM floats), resets the Adam state and
+ * packs the BF16 operand copies used by training and by the learned sweeps. */
+int mfc_mlp_init(const float *params);
+int mfc_mlp_get(float *params);
+/* Gradient (host, MFC_MLP_NPARAM floats) and loss of n device samples. */
+int mfc_mlp_gradient(const float *samples, long n, float *grad, float *loss);
+/* One Adam step (beta 0.9/0.999) on n device samples; loss before the step. */
+int mfc_mlp_train_step(const float *samples, long n, float lr, float *loss);
+/* Inference on n device samples: out (device) = 4 planes: w0, w1, w2, value. */
+int mfc_mlp_infer(const float *samples, long n, float *out);
+/* Training pairs from the plain solver: coarse cell averages of a 2x finer
+ * field (one variable, ghosted layouts with b ghost layers; n = 0 for 1D). */
+int mfc_ml_restrict(int m, int n, int b, const double *fine, double *coarse);
+/* Samples of direction dir (0 x, 1 y) for every coarse interior cell and
+ * both of its faces: out (device) has 13 planes of 2(m+1)(n+1) floats, the
+ * 12 sample fields and a validity flag (the stencil's largest
+ * neighbour difference exceeds 1e-6 of its largest value).  coef27: one cell's
+ * WENO5 coefficients of the uniform coarse grid (mfc_weno_coefficients layout). */
+int mfc_ml_samples(int dir, int m, int n, int b, const double *coarse, const double *fine, const double *coef27,
+                   double eps, float *out);
+const char *mfc_mlp_last_error(void);
 
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * every launch.  Kernel ids: 0 bc_fill, 1 cons_to_prim, 2 sweep_x, 3 sweep_y,

@@ -15,6 +15,7 @@ extern "C" {
 }
 
 namespace mfc {
+const MlpFrags *mlp_frags();
 template <> int launch_sweep<1>(const SweepLaunch &, const SweepArgs &, hipStream_t);
 template <> int launch_sweep<2>(const SweepLaunch &, const SweepArgs &, hipStream_t);
 template <> int launch_sweep<3>(const SweepLaunch &, const SweepArgs &, hipStream_t);
@@ -154,7 +155,10 @@ int check_params(const mfc_params &p) {
         if (!(p.gammas[i] > 0.0)) return fail(-1, "Unsupported value of fluid_pp(%d)%%gamma", i + 1);
         if (p.pi_infs[i] < 0.0) return fail(-1, "Unsupported value of fluid_pp(%d)%%pi_inf", i + 1);
     }
-    if (p.precision != MFC_PREC_FP64 && p.precision != MFC_PREC_FP32_BULK) return fail(-1, "Unsupported precision");
+    if (p.precision != MFC_PREC_FP64 && p.precision != MFC_PREC_FP32_BULK && p.precision != MFC_PREC_FP32_LEARNED)
+        return fail(-1, "Unsupported precision");
+    if (p.precision == MFC_PREC_FP32_LEARNED && p.weno_order != 5)
+        return fail(-1, "learned WENO weights need weno_order = 5");
     return 0;
 }
 
@@ -530,6 +534,10 @@ SweepArgs sweep_args(int d) {
 
 int sweep(int d, SweepArgs &a) {
     SweepLaunch L{d, G.D, G.prm.weno_order, G.prm.precision};
+    if (G.prm.precision == MFC_PREC_FP32_LEARNED) {
+        a.mlp = mlp_frags();
+        if (!a.mlp) return fail(-4, "learned weights requested but mfc_mlp_init has not been called");
+    }
     TimedScope ts(kKSweepX + d);
     int rc;
     switch (G.prm.num_fluids) {

@@ -20,6 +20,7 @@
 #pragma once
 
 #include "mfc_device.hpp"
+#include "mfc_mlp.hpp"
 
 #include <type_traits>
 
@@ -54,6 +55,7 @@ struct SweepArgs {
     int chunk, nchunk;    // cells per tile along the sweep and tile count (k_sweep_t)
     int ntile;            // x tiles (k_sweep_x) or x column groups (k_sweep_t)
     long nblocks;
+    const MlpFrags *mlp;  // learned WENO5 weights (MFC_PREC_FP32_LEARNED)
 };
 
 __device__ __forceinline__ long ghost_off(const SweepArgs &a, int i, int j, int k) {
@@ -174,8 +176,102 @@ __device__ __forceinline__ void load_coef(const SweepArgs &a, int cell, R *cf) {
 }
 
 // ---------------------------------------------------------------------------
+// Learned WENO5 weights inside a sweep (FP32 bulk).  Per variable each lane
+// owns two reconstructions: the right face of its left cell and the left
+// face of its right cell.  Both are oriented so the face lies between
+// stencil entries 2 and 3 (the left-face stencil is reversed, its candidates
+// and weights listed in the reversed order), their features go to LDS, the
+// wave evaluates the MLP on 8 tiles of 16 with MFMA, and each lane combines
+// the learned weights with its candidates.  Per face the plain weights are
+// the fallback when the learned value is not finite or breaks positivity
+// (partial densities and volume fractions >= 0, volume fractions <= 1,
+// pressure > 0 where the plain value is).
+
+struct MlLds {
+    unsigned short x[128][16];
+    MlpActLds act;
+    float z[128][4];
+};
+
+// Candidates and plain weights of one side of a WENO5 cell (FP32 bulk; the
+// same arithmetic as weno_cell), in the oriented order.
+template <bool RIGHT>
+__device__ __forceinline__ void weno5_parts(const float *v, const float *cf, float eps, float *p, float *w) {
+    const float vj = v[2];
+    const float d1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];
+    const float b0 = cf[kBeta + 0] * d1 * d1 + cf[kBeta + 1] * d1 * d0 + cf[kBeta + 2] * d0 * d0 + eps;
+    const float b1 = cf[kBeta + 3] * d0 * d0 + cf[kBeta + 4] * d0 * dm1 + cf[kBeta + 5] * dm1 * dm1 + eps;
+    const float b2 = cf[kBeta + 6] * dm1 * dm1 + cf[kBeta + 7] * dm1 * dm2 + cf[kBeta + 8] * dm2 * dm2 + eps;
+    const int C = RIGHT ? kCbR : kCbL, Dd = RIGHT ? kDR : kDL;
+    float pp[3], a[3];
+    pp[0] = vj + cf[C + 0] * d1 + cf[C + 1] * d0;
+    pp[1] = vj + cf[C + 2] * d0 + cf[C + 3] * dm1;
+    pp[2] = vj + cf[C + 4] * dm1 + cf[C + 5] * dm2;
+    a[0] = bdiv(cf[Dd + 0], b0 * b0);
+    a[1] = bdiv(cf[Dd + 1], b1 * b1);
+    a[2] = bdiv(cf[Dd + 2], b2 * b2);
+    const float s = a[0] + a[1] + a[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        // the left face's candidate k is the oriented candidate 2-k
+        const int o = RIGHT ? k : 2 - k;
+        p[o] = pp[k];
+        w[o] = bdiv(a[k], s);
+    }
+}
+
+// Features of an oriented stencil into one row of the LDS tile.
+__device__ __forceinline__ float ml_features(const float *v, unsigned short *row) {
+    float sc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sc = fmaxf(sc, fabsf(v[k + 1] - v[k]));
+    const float inv = sc > 0.f ? 1.f / sc : 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) row[k] = k < kMlpIn ? f2bf((v[k] - v[2]) * inv) : (unsigned short)0;
+    return sc;
+}
+
+template <int NF, int D>
+__device__ __forceinline__ float ml_value(int var, const float *z, const float *p, const float *wp) {
+    using Y = Sys<NF, D>;
+    float w[3];
+    mlp_weights(z, wp, w);
+    const float learned = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
+    const float plain = wp[0] * p[0] + wp[1] * p[1] + wp[2] * p[2];
+    bool ok = isfinite(learned);
+    if (var < NF || var >= Y::ADV) ok = ok && !(learned < 0.f && plain >= 0.f);
+    if (var >= Y::ADV) ok = ok && !(learned > 1.f && plain <= 1.f);
+    if (var == Y::E) ok = ok && !(learned <= 0.f && plain > 0.f);
+    return ok ? learned : plain;
+}
+
+// Both face states of one variable for the wave's 64 faces.  stR: stencil of
+// the cell left of the lane's face (cells j-2..j+2), stL: of the cell right
+// of it (cells j-1..j+3); cfR/cfL their coefficients.
+template <int NF, int D>
+__device__ __forceinline__ void ml_face_states(int var, const float *stR, const float *stL, const float *cfR,
+                                               const float *cfL, float eps, const MlpRegs &Rg, MlLds &L, int lane,
+                                               float &qLv, float &qRv) {
+    float pR[3], wR[3], pL[3], wL[3], vo[5];
+    weno5_parts<true>(stR, cfR, eps, pR, wR);
+    weno5_parts<false>(stL, cfL, eps, pL, wL);
+    ml_features(stR, L.x[lane]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) vo[k] = stL[4 - k];
+    ml_features(vo, L.x[64 + lane]);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        MlpFwdState st;
+        mlp_forward_tile(Rg, L.x + 16 * t, L.act, L.z + 16 * t, lane, st);
+    }
+    qLv = ml_value<NF, D>(var, L.z[lane], pR, wR);
+    qRv = ml_value<NF, D>(var, L.z[64 + lane], pL, wL);
+    wave_lds_sync();
+}
+
+// ---------------------------------------------------------------------------
 // x sweep
-template <int NF, int D, int WO, typename R>
+template <int NF, int D, int WO, typename R, bool ML>
 __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
@@ -209,21 +305,44 @@ __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
 
     // 2. face x0-1+lane + 1/2
     const int jf = x0 - 1 + lane;
-    if (jf <= a.m) {
+    R qL[S], qR[S];
+    if constexpr (ML) {
+        // every lane takes part in the wave's MFMA tiles, valid face or not
+        __shared__ MlLds ml[1];
+        MlpRegs Rg;
+        mlp_load_regs(a.mlp, lane, Rg);
         R cfR[kNCoef], cfL[kNCoef];
-        load_coef<R, WO>(a, jf, cfR);
-        load_coef<R, WO>(a, jf + 1, cfL);
-        R qL[S], qR[S], f[S], us;
+        load_coef<R, WO>(a, min(jf, a.m), cfR);
+        load_coef<R, WO>(a, min(jf, a.m) + 1, cfL);
         const R eps = (R)a.eps;
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            R st[WO], dummy;
+            R stR[5], stL[5];
 #pragma unroll
-            for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + w];
-            weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
+            for (int w = 0; w < 5; ++w) {
+                stR[w] = sp[v][lane + w];
+                stL[w] = sp[v][lane + 1 + w];
+            }
+            ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, Rg, ml[0], lane, qL[v], qR[v]);
+        }
+    }
+    if (jf <= a.m) {
+        R f[S], us;
+        if constexpr (!ML) {
+            R cfR[kNCoef], cfL[kNCoef];
+            load_coef<R, WO>(a, jf, cfR);
+            load_coef<R, WO>(a, jf + 1, cfL);
+            const R eps = (R)a.eps;
 #pragma unroll
-            for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + 1 + w];
-            weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
+            for (int v = 0; v < S; ++v) {
+                R st[WO], dummy;
+#pragma unroll
+                for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + w];
+                weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
+#pragma unroll
+                for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + 1 + w];
+                weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
+            }
         }
         if (jf == -1 && a.bc_beg == -4) {
 #pragma unroll
@@ -261,7 +380,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
 // is a contiguous 8*TX-byte segment) and the fluxes reuse the same LDS.
 constexpr int kTX = 16, kTS = 16;
 
-template <int NF, int D, int DIR, int WO, typename R>
+template <int NF, int D, int DIR, int WO, typename R, bool ML>
 __global__ __launch_bounds__(256) void k_sweep_t(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
@@ -316,21 +435,43 @@ __global__ __launch_bounds__(256) void k_sweep_t(const SweepArgs a) {
     const int jf = y0 - 1 + ts;
     R f[S], us = R(0);
     const bool face_ok = jf <= s;
-    if (face_ok) {
+    R qL[S], qR[S];
+    if constexpr (ML) {
+        __shared__ MlLds ml[4];
+        const int lane = tid & 63, wv = tid >> 6;
+        MlpRegs Rg;
+        mlp_load_regs(a.mlp, lane, Rg);
         R cfR[kNCoef], cfL[kNCoef];
-        load_coef<R, WO>(a, jf, cfR);
-        load_coef<R, WO>(a, jf + 1, cfL);
-        R qL[S], qR[S];
+        load_coef<R, WO>(a, min(jf, s), cfR);
+        load_coef<R, WO>(a, min(jf, s) + 1, cfL);
         const R eps = (R)a.eps;
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            R st[WO], dummy;
+            R stR[5], stL[5];
 #pragma unroll
-            for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + w][tx];
-            weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
+            for (int w = 0; w < 5; ++w) {
+                stR[w] = sp[v][ts + w][tx];
+                stL[w] = sp[v][ts + 1 + w][tx];
+            }
+            ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, Rg, ml[wv], lane, qL[v], qR[v]);
+        }
+    }
+    if (face_ok) {
+        if constexpr (!ML) {
+            R cfR[kNCoef], cfL[kNCoef];
+            load_coef<R, WO>(a, jf, cfR);
+            load_coef<R, WO>(a, jf + 1, cfL);
+            const R eps = (R)a.eps;
 #pragma unroll
-            for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + 1 + w][tx];
-            weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
+            for (int v = 0; v < S; ++v) {
+                R st[WO], dummy;
+#pragma unroll
+                for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + w][tx];
+                weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
+#pragma unroll
+                for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + 1 + w][tx];
+                weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
+            }
         }
         if (jf == -1 && a.bc_beg == -4) {
 #pragma unroll

@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(_HERE, "libmfc_rhs.so")
 NUM_FLUIDS_MAX = 10
 PREC_FP64 = 0
 PREC_FP32_BULK = 1
+PREC_FP32_LEARNED = 2
 
 EXPORTED = [
     "mfc_rhs_init", "mfc_rhs_set_grid", "mfc_compute_rhs", "mfc_tvd_rk_step",
@@ -34,6 +35,8 @@ EXPORTED = [
     "mfc_memcpy_h2d", "mfc_memcpy_d2h", "mfc_sync", "mfc_stream",
     "mfc_set_stream", "mfc_rhs_finalize", "mfc_last_error",
     "mfc_timing_enable", "mfc_timing_reset", "mfc_timing_read",
+    "mfc_mlp_init", "mfc_mlp_get", "mfc_mlp_gradient", "mfc_mlp_train_step", "mfc_mlp_infer",
+    "mfc_ml_restrict", "mfc_ml_samples", "mfc_mlp_last_error",
 ]
 KERNEL_IDS = {"bc_fill": 0, "cons_to_prim": 1, "sweep_x": 2, "sweep_y": 3, "sweep_z": 4, "icfl": 5,
               "rk1_update": 6}
