@@ -258,6 +258,127 @@ void orc_weno_cell(int weno_order, double weno_eps, const double *cf, const doub
 }
 
 /* ------------------------------------------------------------------------ */
+/* Learned WENO5 weights.  No reference counterpart: the model defined in
+ * include/mfc_rhs.h replaces w_k of m_weno.fpp:508-510,526-528 by
+ * w_plain_k exp(z_k) / sum_m w_plain_m exp(z_m), z = MLP(stencil features),
+ * and keeps the plain value where the learned one is not finite or breaks a
+ * bound the plain one keeps.                                                */
+
+static int g_learned = 0;
+static double g_w1[5][32], g_b1[32], g_w2[32][32], g_b2[32], g_w3[32][3], g_b3[3];
+
+/* FP64 -> FP32 -> BF16, round to nearest even (finite values). */
+static double bf16r(double a) {
+    float f = (float)a;
+    unsigned u;
+    memcpy(&u, &f, sizeof u);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    memcpy(&f, &u, sizeof f);
+    return f;
+}
+
+void orc_set_learned(const float *params) {
+    int k, n;
+    g_learned = params != NULL;
+    if (!params) return;
+    for (k = 0; k < 5; k++)
+        for (n = 0; n < 32; n++) g_w1[k][n] = bf16r(params[k * 32 + n]);
+    for (k = 0; k < 32; k++)
+        for (n = 0; n < 32; n++) g_w2[k][n] = bf16r(params[544 + k * 32 + n]);
+    for (k = 0; k < 32; k++)
+        for (n = 0; n < 3; n++) g_w3[k][n] = bf16r(params[1600 + k * 16 + n]);
+    for (n = 0; n < 32; n++) {
+        g_b1[n] = params[512 + n];
+        g_b2[n] = params[1568 + n];
+    }
+    for (n = 0; n < 3; n++) g_b3[n] = params[2112 + n];
+}
+
+/* Learned weights of one face from its oriented stencil (face between v[2]
+ * and v[3]) and the plain weights wp in the same orientation.  The features
+ * are part of the model's definition: x_k = (v_k - v_2) * (1/s) with s the
+ * largest neighbour difference, evaluated in FP32 on the FP32 stencil (the
+ * learned form reconstructs in FP32), then rounded to BF16. */
+static void learned_weights(const double *v, const double *wp, double *w) {
+    double x[5], h1[32], h2[32], z[3], zm, u[3], su;
+    float vf[5], sf = 0.0f, inv;
+    int k, n;
+    for (k = 0; k < 5; k++) vf[k] = (float)v[k];
+    for (k = 0; k < 4; k++) sf = fmaxf(sf, fabsf(vf[k + 1] - vf[k]));
+    inv = sf > 0.0f ? 1.0f / sf : 0.0f;
+    for (k = 0; k < 5; k++) x[k] = bf16r((double)((vf[k] - vf[2]) * inv));
+    for (n = 0; n < 32; n++) {
+        double acc = 0.0;
+        for (k = 0; k < 5; k++) acc += x[k] * g_w1[k][n];
+        acc += g_b1[n];
+        h1[n] = bf16r(acc > 0.0 ? acc : 0.0);
+    }
+    for (n = 0; n < 32; n++) {
+        double acc = 0.0;
+        for (k = 0; k < 32; k++) acc += h1[k] * g_w2[k][n];
+        acc += g_b2[n];
+        h2[n] = bf16r(acc > 0.0 ? acc : 0.0);
+    }
+    for (n = 0; n < 3; n++) {
+        double acc = 0.0;
+        for (k = 0; k < 32; k++) acc += h2[k] * g_w3[k][n];
+        z[n] = acc + g_b3[n];
+    }
+    zm = fmax(z[0], fmax(z[1], z[2]));
+    su = 0.0;
+    for (k = 0; k < 3; k++) {
+        u[k] = wp[k] * exp(z[k] - zm);
+        su += u[k];
+    }
+    for (k = 0; k < 3; k++) w[k] = u[k] / su;
+}
+
+static double learned_value(const double *v, const double *p, const double *wp, int kind) {
+    double w[3], lv, pv;
+    int ok;
+    learned_weights(v, wp, w);
+    lv = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
+    pv = wp[0] * p[0] + wp[1] * p[1] + wp[2] * p[2];
+    ok = isfinite(lv);
+    if (kind & 1) ok = ok && !(lv < 0.0 && pv >= 0.0);
+    if (kind & 2) ok = ok && !(lv > 1.0 && pv <= 1.0);
+    if (kind & 4) ok = ok && !(lv <= 0.0 && pv > 0.0);
+    return ok ? lv : pv;
+}
+
+void orc_weno5_learned_cell(double weno_eps, const double *cf, const double *v, int kind, double *vL,
+                            double *vR) {
+    const double vj = v[2];
+    const double d1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];
+    const double b0 = cf[18] * d1 * d1 + cf[19] * d1 * d0 + cf[20] * d0 * d0 + weno_eps;
+    const double b1 = cf[21] * d0 * d0 + cf[22] * d0 * dm1 + cf[23] * dm1 * dm1 + weno_eps;
+    const double b2 = cf[24] * dm1 * dm1 + cf[25] * dm1 * dm2 + cf[26] * dm2 * dm2 + weno_eps;
+    const double bb[3] = {b0 * b0, b1 * b1, b2 * b2};
+    double p[3], a[3], wp[3], po[3], wo[3], vr[5], sum;
+    int k;
+    /* face j+1/2: the stencil is already oriented */
+    p[0] = vj + cf[6] * d1 + cf[7] * d0;
+    p[1] = vj + cf[8] * d0 + cf[9] * dm1;
+    p[2] = vj + cf[10] * dm1 + cf[11] * dm2;
+    for (k = 0; k < 3; k++) a[k] = cf[15 + k] / bb[k];
+    sum = a[0] + a[1] + a[2];
+    for (k = 0; k < 3; k++) wp[k] = a[k] / sum;
+    *vR = learned_value(v, p, wp, kind);
+    /* face j-1/2: reversed stencil, candidate k becomes oriented 2-k */
+    p[0] = vj + cf[0] * d1 + cf[1] * d0;
+    p[1] = vj + cf[2] * d0 + cf[3] * dm1;
+    p[2] = vj + cf[4] * dm1 + cf[5] * dm2;
+    for (k = 0; k < 3; k++) a[k] = cf[12 + k] / bb[k];
+    sum = a[0] + a[1] + a[2];
+    for (k = 0; k < 3; k++) {
+        po[2 - k] = p[k];
+        wo[2 - k] = a[k] / sum;
+    }
+    for (k = 0; k < 5; k++) vr[k] = v[4 - k];
+    *vL = learned_value(vr, po, wo, kind);
+}
+
+/* ------------------------------------------------------------------------ */
 /* HLLC at one face: m_riemann_solvers.fpp:134-325                           */
 
 static inline double fmin_ref(double a, double b) { return (a < b) ? a : b; }
@@ -482,8 +603,14 @@ int orc_compute_rhs(const orc_case *c, const double *const cb[3], const double *
                             POS(jj, I, J, K);
                             st[r + 2] = qp[v * g.ng + gidx(&g, I, J, K)];
                         }
-                        orc_weno_cell(c->weno_order, c->weno_eps, cf, st, &vL[v * ns + (j + 1)],
-                                      &vR[v * ns + (j + 1)]);
+                        if (g_learned && c->weno_order == 5) {
+                            const int kind = (v < g.nf || v >= g.advxb ? 1 : 0) | (v >= g.advxb ? 2 : 0) |
+                                             (v == g.E_idx ? 4 : 0);
+                            orc_weno5_learned_cell(c->weno_eps, cf, st, kind, &vL[v * ns + (j + 1)],
+                                                   &vR[v * ns + (j + 1)]);
+                        } else
+                            orc_weno_cell(c->weno_order, c->weno_eps, cf, st, &vL[v * ns + (j + 1)],
+                                          &vR[v * ns + (j + 1)]);
                     }
                 }
                 /* Riemann-state buffers for bc = -4 (m_riemann_solvers.fpp:480-550) */

@@ -102,6 +102,23 @@ int orc_icfl_max(const orc_case *c, const double *const ds[3], const double *q_p
  * and vR (face j+1/2) (m_weno.fpp:391-540). */
 void orc_weno_cell(int weno_order, double weno_eps, const double *coef27, const double *v,
                    double *vL, double *vR);
+/* Learned WENO5 weights (no reference counterpart; they replace the
+ * nonlinear weights of m_weno.fpp:508-510,526-528).  params: the 2128 FP32
+ * parameters of the 5 -> 32 -> 32 -> 3 MLP in the layout of
+ * include/mfc_rhs.h, or NULL to return to the plain weights.  While set,
+ * orc_compute_rhs (and the RK step) use them for WENO5, in FP64 arithmetic
+ * with BF16 round-to-nearest-even applied where the GPU feeds its matrix
+ * cores: features, weights and both hidden activations.  The features are
+ * evaluated in FP32 on the FP32-rounded stencil, as the model defines them.
+ * Not thread safe. */
+void orc_set_learned(const float *params);
+/* The learned reconstruction of one variable at one WENO5 cell.  kind: bit 0
+ * = the variable is >= 0 where its plain value is (partial densities, volume
+ * fractions), bit 1 = <= 1 where the plain value is (volume fractions), bit 2
+ * = > 0 where the plain value is (pressure).  Writes vL, vR as
+ * orc_weno_cell. */
+void orc_weno5_learned_cell(double weno_eps, const double *coef27, const double *v, int kind, double *vL,
+                            double *vR);
 /* HLLC flux at one face (m_riemann_solvers.fpp:134-325).  qL, qR: primitive
  * states (sys_size each) left and right of the face.  dir: 0, 1, 2.  flux
  * receives sys_size values, *ustar the contact velocity. */

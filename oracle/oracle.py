@@ -60,6 +60,10 @@ def lib():
         L.orc_weno_cell.restype = None
         L.orc_hllc_face.argtypes = [cp, ctypes.c_int, dp, dp, dp, dp]
         L.orc_hllc_face.restype = None
+        L.orc_set_learned.argtypes = [ctypes.POINTER(ctypes.c_float)]
+        L.orc_set_learned.restype = None
+        L.orc_weno5_learned_cell.argtypes = [ctypes.c_double, dp, dp, ctypes.c_int, dp, dp]
+        L.orc_weno5_learned_cell.restype = None
         _LIB = L
     return _LIB
 
@@ -224,6 +228,31 @@ def weno_cell(weno_order, weno_eps, coef27, stencil5):
     cf = np.ascontiguousarray(coef27, dtype=np.float64)
     st = np.ascontiguousarray(stencil5, dtype=np.float64)
     lib().orc_weno_cell(weno_order, weno_eps, _p(cf), _p(st), ctypes.byref(vl), ctypes.byref(vr))
+    return vl.value, vr.value
+
+
+def set_learned(params):
+    """Learned WENO5 weights for compute_rhs / tvd_rk_step: the 2128 MLP
+    parameters (FP32), or None for the plain weights."""
+    global _LEARNED
+    if params is None:
+        _LEARNED = None
+        lib().orc_set_learned(ctypes.cast(None, ctypes.POINTER(ctypes.c_float)))
+        return
+    _LEARNED = np.ascontiguousarray(params, dtype=np.float32).copy()
+    assert _LEARNED.size == 2128
+    lib().orc_set_learned(_LEARNED.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+
+
+_LEARNED = None
+
+
+def weno5_learned_cell(weno_eps, coef27, stencil5, kind=0):
+    """Learned reconstruction of one cell (set_learned first): (vL, vR)."""
+    vl, vr = ctypes.c_double(), ctypes.c_double()
+    cf = np.ascontiguousarray(coef27, dtype=np.float64)
+    st = np.ascontiguousarray(stencil5, dtype=np.float64)
+    lib().orc_weno5_learned_cell(weno_eps, _p(cf), _p(st), kind, ctypes.byref(vl), ctypes.byref(vr))
     return vl.value, vr.value
 
 

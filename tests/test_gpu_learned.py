@@ -1,5 +1,7 @@
-"""The learned-weight form in the sweeps: the untrained model reproduces the
-FP32-bulk plain path, conservation still holds, the per-face fallback keeps
+"""The learned-weight form in the sweeps: with a network far from zero its
+right-hand side matches the C oracle's learned right-hand side (FP64 with
+BF16 at the matrix-core operands) to the FP32-bulk tolerance, the untrained
+model reproduces the FP32-bulk plain path, conservation still holds, the per-face fallback keeps
 volume fractions in range, and -- the gain test -- a network trained on the
 GPU from plain-solver coarse/fine pairs lowers the coarse-grid error against
 a 2x-finer plain run (the judge) on a case it was not trained on."""
@@ -8,7 +10,9 @@ import pytest
 
 from microfc_amd import PREC_FP32_BULK, PREC_FP32_LEARNED, RhsModule, ic, learned
 
-from helpers import perturbed, say, scaled_linf
+from helpers import perturbed, say, scaled_linf, to_oracle
+from oracle import oracle as O
+from test_mlp_oracle import random_params
 
 pytestmark = pytest.mark.gpu
 
@@ -23,6 +27,39 @@ def _rhs(torch, prm, grids, q):
         return rd.cpu().numpy()
     finally:
         mod.finalize()
+
+
+def _cases():
+    return {"bubble": ic.shock_bubble_2d(N=64), "shear": ic.shear_layer_2d(N=64), "sod": ic.sod_1d(m=199),
+            "droplet3d": ic.shock_droplet_3d(N=28)}
+
+
+def test_learned_rhs_matches_oracle(gpu):
+    """Per variable, the GPU learned RHS is within the FP32-bulk tolerance of
+    the oracle's learned RHS, while the network moves that RHS far more than
+    this away from the plain one (so a misplaced stencil, candidate or weight
+    would not pass)."""
+    torch = gpu
+    params = random_params(11, scale=0.6, last=0.8).astype(np.float32)
+    learned.Mlp(params)
+    try:
+        for name, (prm, grids, q) in _cases().items():
+            q = perturbed(prm, q)
+            case = to_oracle(prm)
+            O.set_learned(None)
+            plain = O.compute_rhs(case, grids, q)
+            O.set_learned(params)
+            ref = O.compute_rhs(case, grids, q)
+            prm.precision = PREC_FP32_LEARNED
+            got = _rhs(torch, prm, grids, q)
+            err = scaled_linf(prm, got, ref)
+            eff = scaled_linf(prm, plain, ref)
+            say(f"  {name}: learned GPU vs learned oracle {err.max():.2e}; learned vs plain oracle {eff.max():.2e}")
+            assert np.all(np.isfinite(got))
+            assert err.max() <= 2e-3
+            assert eff.max() >= 20.0 * err.max()
+    finally:
+        O.set_learned(None)
 
 
 def test_untrained_learned_equals_plain_fp32(gpu):
@@ -71,26 +108,37 @@ def test_trained_weights_beat_plain_against_finer_judge(gpu):
     # training pairs from a plain fine run (seed 0), shear layer resolved by
     # about one coarse cell
     fprm, fg, fq = ic.shear_layer_2d(N=2 * Nc, seed=0, delta=0.02)
-    data = learned.training_pairs(fprm, fg, fq, nsnap=6, steps_between=40)
+    data = learned.training_pairs(fprm, fg, fq, nsnap=12, steps_between=20)
     say(f"  training pairs: {data.shape[1]}")
     mlp = learned.Mlp(seed=0)
-    losses = learned.train(mlp, data, epochs=30, batch=8192, lr=2e-3, seed=0)
-    say(f"  training loss {losses[0]:.4e} -> {losses[-1]:.4e} ({len(losses)} Adam steps on the GPU)")
-    assert losses[-1] < losses[0]
-    # evaluation on another seed: coarse plain vs coarse learned vs the 2x
-    # finer plain run as judge
-    t_end = 0.3
-    fprm, fg, fq = ic.shear_layer_2d(N=2 * Nc, seed=5, delta=0.02)
-    judge, nf, sf = learned.run(fprm, fg, fq, t_end)
-    judge_c = learned.restrict_host(judge)
-    cprm, cg, cq = ic.shear_layer_2d(N=Nc, seed=5, delta=0.02)
-    cprm.precision = PREC_FP32_BULK
-    plain, npl, sp = learned.run(cprm, cg, cq, t_end)
-    cprm.precision = PREC_FP32_LEARNED
-    lrn, nl, sl = learned.run(cprm, cg, cq, t_end)
-    ep = learned.l1_error(plain, judge_c)
-    el = learned.l1_error(lrn, judge_c)
-    say(f"  L1(rho) vs 2x judge: plain {ep:.4e}, learned {el:.4e} (gain {ep / el:.3f}x); "
-        f"GPU s: plain {sp:.3f}, learned {sl:.3f}, judge {sf:.3f}")
-    assert np.all(np.isfinite(lrn))
-    assert el < ep
+    _, loss0 = mlp.gradient(data)
+    losses = learned.train(mlp, data, epochs=30, batch=8192, lr=3e-3, seed=0)
+    _, loss1 = mlp.gradient(data)
+    say(f"  loss over all pairs {loss0:.4e} -> {loss1:.4e} ({len(losses)} Adam steps on the GPU)")
+    assert loss1 < 0.7 * loss0
+    # evaluation on two other seeds: coarse plain vs coarse learned, both
+    # against the 2x finer plain FP64 run (the judge), restricted to the
+    # coarse grid
+    names = ("rho", "rho*u", "rho*v", "E")
+    for seed in (5, 7):
+        t_end = 0.3
+        fprm, fg, fq = ic.shear_layer_2d(N=2 * Nc, seed=seed, delta=0.02)
+        judge, nf, sf = learned.run(fprm, fg, fq, t_end)
+        judge_c = learned.restrict_host(judge)
+        cprm, cg, cq = ic.shear_layer_2d(N=Nc, seed=seed, delta=0.02)
+        res = {}
+        for prec in (0, PREC_FP32_BULK, PREC_FP32_LEARNED):
+            cprm.precision = prec
+            out, nsteps, sec = learned.run(cprm, cg, cq, t_end)
+            assert np.all(np.isfinite(out))
+            res[prec] = (np.array([learned.l1_error(out, judge_c, v) for v in range(4)]), sec)
+        e64, e32, el = res[0][0], res[PREC_FP32_BULK][0], res[PREC_FP32_LEARNED][0]
+        say(f"  seed {seed}, t = {t_end}, {nsteps} steps on {Nc}^2; L1 vs 2x judge per variable {names}:")
+        say(f"    plain FP64   {' '.join(f'{x:.3e}' for x in e64)}  GPU s {res[0][1]:.3f}")
+        say(f"    plain FP32   {' '.join(f'{x:.3e}' for x in e32)}  GPU s {res[PREC_FP32_BULK][1]:.3f}")
+        say(f"    learned      {' '.join(f'{x:.3e}' for x in el)}  GPU s {res[PREC_FP32_LEARNED][1]:.3f}")
+        say(f"    gain over plain FP32 {' '.join(f'{x:.3f}x' for x in e32 / el)}; judge GPU s {sf:.3f}")
+        # the shear layer's density, transverse momentum and energy errors
+        # all drop; the streamwise momentum is left to the report
+        for v in (0, 2, 3):
+            assert el[v] < e32[v], (seed, names[v])
