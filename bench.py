@@ -55,25 +55,33 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--case", default="shock_bubble_2d", choices=["shock_bubble_2d", "shock_droplet_3d", "sod_1d"])
+    ap.add_argument("--case", default="shock_bubble_2d",
+                    choices=["shock_bubble_2d", "shock_droplet_3d", "shear_layer_2d", "sod_1d"])
     ap.add_argument("--N", type=int, default=0, help="cells per direction (default: the case's size)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_bulk"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_bulk", "fp32_learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU halo exchange is design-only "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU: the multi-GPU halo exchange is designed but "
+                                   "not built, so --gpus must be 1"}))
         sys.exit(2)
 
     import numpy as np
     import torch
 
-    from microfc_amd import PREC_FP32_BULK, PREC_FP64, RhsModule, ic
+    from microfc_amd import PREC_FP32_BULK, PREC_FP32_LEARNED, PREC_FP64, RhsModule, ic
 
-    prec = PREC_FP64 if args.precision == "fp64" else PREC_FP32_BULK
+    prec = {"fp64": PREC_FP64, "fp32_bulk": PREC_FP32_BULK, "fp32_learned": PREC_FP32_LEARNED}[args.precision]
+    if prec == PREC_FP32_LEARNED:
+        # throughput does not depend on the weights; a network with a non-zero
+        # output layer exercises the softmax and the per-face fallback
+        from microfc_amd import learned
+        p = learned.init_params(0)
+        p[learned.OFF["W3"]:learned.OFF["W3"] + 512] = np.random.default_rng(1).normal(0, 0.1, 512)
+        learned.Mlp(p)
     if args.case == "shock_bubble_2d":
         N = args.N or 2048
         prm, grids, q = ic.shock_bubble_2d(N=N, precision=prec)
@@ -82,6 +90,10 @@ def main():
         N = args.N or 256
         prm, grids, q = ic.shock_droplet_3d(N=N, precision=prec)
         workload = f"3D shock-droplet water/air {N}^3, WENO5+HLLC+RK3"
+    elif args.case == "shear_layer_2d":
+        N = args.N or 4096
+        prm, grids, q = ic.shear_layer_2d(N=N, precision=prec)
+        workload = f"2D shear layer (Kelvin-Helmholtz) {N}x{N}, one gas, WENO5+HLLC+RK3"
     else:
         N = args.N or 1000
         prm, grids, q = ic.sod_1d(m=N - 1)
@@ -142,7 +154,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == PREC_FP64 else "f32 bulk / f64 accumulate",
+        "dtype": {PREC_FP64: "f64", PREC_FP32_BULK: "f32 bulk / f64 accumulate",
+                  PREC_FP32_LEARNED: "f32 bulk + bf16 MFMA weights / f64 accumulate"}[prec],
         "data": "synthetic initial condition of the named case (generated, not read)",
         "config": {"workload": workload, "cells": ncell, "sys_size": S, "cfl": cfl, "dt": dt,
                    "precision": args.precision, "time_stepper": "TVD-RK3"},

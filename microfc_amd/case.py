@@ -25,7 +25,7 @@ from .rhs import Params
 SIMULATION_KEYS = ["case_dir", "run_time_info", "m", "n", "p", "dt", "t_step_start", "t_step_stop",
                    "t_step_save", "num_fluids", "time_stepper", "weno_eps", "weno_flat", "riemann_flat",
                    "cu_mpi", "probe_wrt", "fd_order", "num_probes", "t_step_old", "weno_Re_flux",
-                   "weno_order", "precision", "parallel_io", "bulk_precision"]
+                   "weno_order", "precision", "parallel_io", "bulk_precision", "mlp_weights"]
 for _d in "xyz":
     SIMULATION_KEYS += [f"bc_{_d}%beg", f"bc_{_d}%end"]
 for _f in range(1, 11):

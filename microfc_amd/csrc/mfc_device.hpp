@@ -9,6 +9,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace mfc {
 
 constexpr int kNumFluidsMax = 10;
@@ -85,6 +87,47 @@ struct WenoCell {
 // is the stencil j-polyn..j+polyn, cf the cell's coefficients.  Computes the
 // smoothness indicators once and evaluates the left (face j-1/2) and/or right
 // (face j+1/2) value.
+// FP64 quotient for operands in the normal range: the instruction sequence
+// the compiler emits for an IEEE a / b on gfx950 (v_rcp_f64, two Newton
+// steps, the quotient and one residual correction) without its range-scaling
+// (v_div_scale) and special-case (v_div_fixup) steps.  Those steps change
+// nothing when a, b, 1/b and a/b are normal numbers whose exponents differ
+// by less than 768 and |a| > 2^-970; the result is then the correctly
+// rounded quotient, bit for bit the same as a / b.  weno_cell uses it only
+// behind a range test of its divisors (kNdivLo..kNdivHi).
+__device__ __forceinline__ double ndiv(double a, double b) {
+    const double r = __builtin_amdgcn_rcp(b);
+    const double e0 = __builtin_fma(-b, r, 1.0);
+    const double r1 = __builtin_fma(r, e0, r);
+    const double e1 = __builtin_fma(-b, r1, 1.0);
+    const double r2 = __builtin_fma(r1, e1, r1);
+    const double q = a * r2;
+    const double rem = __builtin_fma(-b, q, a);
+    return __builtin_fma(rem, r2, q);
+}
+// Squared smoothness indicators inside [2^-300, 2^300] keep every WENO
+// quotient (alpha = d / beta^2 with ideal weights d, omega = alpha / sum)
+// normal, so ndiv is exact there.
+constexpr double kNdivLo = 0x1p-300, kNdivHi = 0x1p300;
+
+template <bool FAST>
+__device__ __forceinline__ double wdiv(double a, double b) {
+    if constexpr (FAST)
+        return ndiv(a, b);
+    else
+        return a / b;
+}
+template <bool FAST>
+__device__ __forceinline__ float wdiv(float a, float b) {
+    return bdiv(a, b);
+}
+
+// Reconstruction of one variable at one cell (m_weno.fpp:391-540): v[0..WO-1]
+// is the stencil j-polyn..j+polyn, cf the cell's coefficients.  Computes the
+// smoothness indicators once and evaluates the left (face j-1/2) and/or right
+// (face j+1/2) value.  In FP64 the weight divisions take ndiv when the
+// squared indicators are in its exact range (always, for physical states),
+// else IEEE division: the result is the same either way.
 template <int WO, typename R, bool WANT_L, bool WANT_R>
 __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL, R &vR) {
     if constexpr (WO == 1) {
@@ -95,21 +138,31 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
         const R d0 = v[2] - v[1], dm1 = v[1] - v[0];
         const R b0 = cf[kBeta + 0] * d0 * d0 + eps;
         const R b1 = cf[kBeta + 3] * dm1 * dm1 + eps;
-        if constexpr (WANT_L) {
-            const R p0 = vj + cf[kCbL + 0] * d0;
-            const R p1 = vj + cf[kCbL + 2] * dm1;
-            const R a0 = bdiv(cf[kDL + 0], b0 * b0);
-            const R a1 = bdiv(cf[kDL + 1], b1 * b1);
-            const R s = a0 + a1;
-            vL = bdiv(a0, s) * p0 + bdiv(a1, s) * p1;
-        }
-        if constexpr (WANT_R) {
-            const R p0 = vj + cf[kCbR + 0] * d0;
-            const R p1 = vj + cf[kCbR + 2] * dm1;
-            const R a0 = bdiv(cf[kDR + 0], b0 * b0);
-            const R a1 = bdiv(cf[kDR + 1], b1 * b1);
-            const R s = a0 + a1;
-            vR = bdiv(a0, s) * p0 + bdiv(a1, s) * p1;
+        const R bb0 = b0 * b0, bb1 = b1 * b1;
+        const R pL0 = vj + cf[kCbL + 0] * d0, pL1 = vj + cf[kCbL + 2] * dm1;
+        const R pR0 = vj + cf[kCbR + 0] * d0, pR1 = vj + cf[kCbR + 2] * dm1;
+        auto weights = [&](auto fast) {
+            constexpr bool F = decltype(fast)::value;
+            if constexpr (WANT_L) {
+                const R a0 = wdiv<F>(cf[kDL + 0], bb0);
+                const R a1 = wdiv<F>(cf[kDL + 1], bb1);
+                const R s = a0 + a1;
+                vL = wdiv<F>(a0, s) * pL0 + wdiv<F>(a1, s) * pL1;
+            }
+            if constexpr (WANT_R) {
+                const R a0 = wdiv<F>(cf[kDR + 0], bb0);
+                const R a1 = wdiv<F>(cf[kDR + 1], bb1);
+                const R s = a0 + a1;
+                vR = wdiv<F>(a0, s) * pR0 + wdiv<F>(a1, s) * pR1;
+            }
+        };
+        if constexpr (sizeof(R) == 8) {
+            if (min_ref(bb0, bb1) >= R(kNdivLo) && max_ref(bb0, bb1) <= R(kNdivHi))
+                weights(std::true_type{});
+            else
+                weights(std::false_type{});
+        } else {
+            weights(std::false_type{});
         }
     } else {
         const R vj = v[2];
@@ -118,21 +171,40 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
         const R b1 = cf[kBeta + 3] * d0 * d0 + cf[kBeta + 4] * d0 * dm1 + cf[kBeta + 5] * dm1 * dm1 + eps;
         const R b2 = cf[kBeta + 6] * dm1 * dm1 + cf[kBeta + 7] * dm1 * dm2 + cf[kBeta + 8] * dm2 * dm2 + eps;
         const R bb0 = b0 * b0, bb1 = b1 * b1, bb2 = b2 * b2;
+        R pL[3], pR[3];
         if constexpr (WANT_L) {
-            const R p0 = vj + cf[kCbL + 0] * d1 + cf[kCbL + 1] * d0;
-            const R p1 = vj + cf[kCbL + 2] * d0 + cf[kCbL + 3] * dm1;
-            const R p2 = vj + cf[kCbL + 4] * dm1 + cf[kCbL + 5] * dm2;
-            const R a0 = bdiv(cf[kDL + 0], bb0), a1 = bdiv(cf[kDL + 1], bb1), a2 = bdiv(cf[kDL + 2], bb2);
-            const R s = a0 + a1 + a2;
-            vL = bdiv(a0, s) * p0 + bdiv(a1, s) * p1 + bdiv(a2, s) * p2;
+            pL[0] = vj + cf[kCbL + 0] * d1 + cf[kCbL + 1] * d0;
+            pL[1] = vj + cf[kCbL + 2] * d0 + cf[kCbL + 3] * dm1;
+            pL[2] = vj + cf[kCbL + 4] * dm1 + cf[kCbL + 5] * dm2;
         }
         if constexpr (WANT_R) {
-            const R p0 = vj + cf[kCbR + 0] * d1 + cf[kCbR + 1] * d0;
-            const R p1 = vj + cf[kCbR + 2] * d0 + cf[kCbR + 3] * dm1;
-            const R p2 = vj + cf[kCbR + 4] * dm1 + cf[kCbR + 5] * dm2;
-            const R a0 = bdiv(cf[kDR + 0], bb0), a1 = bdiv(cf[kDR + 1], bb1), a2 = bdiv(cf[kDR + 2], bb2);
-            const R s = a0 + a1 + a2;
-            vR = bdiv(a0, s) * p0 + bdiv(a1, s) * p1 + bdiv(a2, s) * p2;
+            pR[0] = vj + cf[kCbR + 0] * d1 + cf[kCbR + 1] * d0;
+            pR[1] = vj + cf[kCbR + 2] * d0 + cf[kCbR + 3] * dm1;
+            pR[2] = vj + cf[kCbR + 4] * dm1 + cf[kCbR + 5] * dm2;
+        }
+        auto weights = [&](auto fast) {
+            constexpr bool F = decltype(fast)::value;
+            if constexpr (WANT_L) {
+                const R a0 = wdiv<F>(cf[kDL + 0], bb0), a1 = wdiv<F>(cf[kDL + 1], bb1),
+                        a2 = wdiv<F>(cf[kDL + 2], bb2);
+                const R s = a0 + a1 + a2;
+                vL = wdiv<F>(a0, s) * pL[0] + wdiv<F>(a1, s) * pL[1] + wdiv<F>(a2, s) * pL[2];
+            }
+            if constexpr (WANT_R) {
+                const R a0 = wdiv<F>(cf[kDR + 0], bb0), a1 = wdiv<F>(cf[kDR + 1], bb1),
+                        a2 = wdiv<F>(cf[kDR + 2], bb2);
+                const R s = a0 + a1 + a2;
+                vR = wdiv<F>(a0, s) * pR[0] + wdiv<F>(a1, s) * pR[1] + wdiv<F>(a2, s) * pR[2];
+            }
+        };
+        if constexpr (sizeof(R) == 8) {
+            const R lo = min_ref(min_ref(bb0, bb1), bb2), hi = max_ref(max_ref(bb0, bb1), bb2);
+            if (lo >= R(kNdivLo) && hi <= R(kNdivHi))
+                weights(std::true_type{});
+            else
+                weights(std::false_type{});
+        } else {
+            weights(std::false_type{});
         }
     }
 }

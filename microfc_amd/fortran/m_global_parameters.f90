@@ -50,6 +50,7 @@ module m_global_parameters
     integer :: t_step_start, t_step_stop, t_step_save, t_step_old
     integer :: num_fluids, time_stepper, weno_order, fd_order, num_probes, precision
     integer :: bulk_precision
+    character(len=path_len) :: mlp_weights
     real(kind(0d0)) :: weno_eps
     type(int_bounds_info) :: bc_x, bc_y, bc_z
     type(physical_parameters), dimension(num_fluids_max) :: fluid_pp
@@ -75,6 +76,7 @@ contains
         num_fluids = dflt_int; time_stepper = dflt_int; weno_order = dflt_int
         fd_order = dflt_int; num_probes = dflt_int; precision = 2
         bulk_precision = mfc_prec_fp64
+        mlp_weights = ''
         weno_eps = dflt_real
         bc_x%beg = dflt_int; bc_x%end = dflt_int
         bc_y%beg = dflt_int; bc_y%end = dflt_int

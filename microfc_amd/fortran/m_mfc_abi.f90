@@ -10,7 +10,8 @@ module m_mfc_abi
     implicit none
 
     integer, parameter :: mfc_num_fluids_max = 10
-    integer, parameter :: mfc_prec_fp64 = 0, mfc_prec_fp32_bulk = 1
+    integer, parameter :: mfc_prec_fp64 = 0, mfc_prec_fp32_bulk = 1, mfc_prec_fp32_learned = 2
+    integer, parameter :: mfc_mlp_nparam = 2128
 
     type, bind(C) :: mfc_params
         integer(c_int) :: m, n, p
@@ -105,6 +106,15 @@ module m_mfc_abi
             import :: c_ptr
         end function mfc_last_error
 
+        integer(c_int) function mfc_mlp_init(params) bind(C, name='mfc_mlp_init')
+            import :: c_int, c_float
+            real(c_float), intent(in) :: params(*)
+        end function mfc_mlp_init
+
+        type(c_ptr) function mfc_mlp_last_error() bind(C, name='mfc_mlp_last_error')
+            import :: c_ptr
+        end function mfc_mlp_last_error
+
     end interface
 
 contains
@@ -114,11 +124,26 @@ contains
     subroutine s_mfc_check(rc, what)
         integer(c_int), intent(in) :: rc
         character(len=*), intent(in) :: what
+        if (rc >= 0) return
+        call s_mfc_stop(rc, what, mfc_last_error())
+    end subroutine s_mfc_check
+
+    !> The same for the learned-weight entry points (mfc_mlp_*).
+    subroutine s_mfc_mlp_check(rc, what)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: what
+        if (rc >= 0) return
+        call s_mfc_stop(rc, what, mfc_mlp_last_error())
+    end subroutine s_mfc_mlp_check
+
+    subroutine s_mfc_stop(rc, what, cmsg)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: what
+        type(c_ptr), intent(in) :: cmsg
         character(kind=c_char), pointer :: msg(:)
         integer :: i
         character(len=512) :: text
-        if (rc >= 0) return
-        call c_f_pointer(mfc_last_error(), msg, [512])
+        call c_f_pointer(cmsg, msg, [512])
         text = ''
         do i = 1, 512
             if (msg(i) == c_null_char) exit
@@ -126,6 +151,6 @@ contains
         end do
         print '(A,A,A,I0,A,A)', 'Error: ', what, ' failed (', rc, '): ', trim(text)
         stop 1
-    end subroutine s_mfc_check
+    end subroutine s_mfc_stop
 
 end module m_mfc_abi

@@ -20,7 +20,29 @@ contains
         call s_mfc_check(mfc_rhs_set_grid(0_c_int, x_cb, dx), 'mfc_rhs_set_grid(x)')
         if (n > 0) call s_mfc_check(mfc_rhs_set_grid(1_c_int, y_cb, dy), 'mfc_rhs_set_grid(y)')
         if (p > 0) call s_mfc_check(mfc_rhs_set_grid(2_c_int, z_cb, dz), 'mfc_rhs_set_grid(z)')
+        if (bulk_precision == mfc_prec_fp32_learned) call s_load_mlp_weights()
     end subroutine s_initialize_rhs_module
+
+    !> Learned WENO5 weights (no reference counterpart): loads the network
+    !! named by mlp_weights, mfc_mlp_nparam little-endian FP32 values in the
+    !! parameter layout of include/mfc_rhs.h, into the library.
+    subroutine s_load_mlp_weights()
+        real(c_float) :: params(mfc_mlp_nparam)
+        logical :: file_exist
+        integer :: ios
+        inquire (FILE=trim(mlp_weights), EXIST=file_exist)
+        if (len_trim(mlp_weights) == 0 .or. .not. file_exist) then
+            print '(A)', 'Error: bulk_precision = 2 needs mlp_weights, the file of trained MLP parameters'
+            stop 1
+        end if
+        open (3, FILE=trim(mlp_weights), FORM='unformatted', ACCESS='stream', ACTION='read', STATUS='old')
+        read (3, iostat=ios) params; close (3)
+        if (ios /= 0) then
+            print '(A,I0,A)', 'Error: mlp_weights must hold ', mfc_mlp_nparam, ' FP32 values'
+            stop 1
+        end if
+        call s_mfc_mlp_check(mfc_mlp_init(params), 'mfc_mlp_init')
+    end subroutine s_load_mlp_weights
 
     !> s_compute_rhs(q_cons_vf, q_prim_vf, rhs_vf, t_step) (m_rhs.fpp:405-679).
     !! The fields are device-resident; q_prim_vf is written when

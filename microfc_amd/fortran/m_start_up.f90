@@ -25,7 +25,7 @@ contains
             weno_eps, weno_flat, riemann_flat, cu_mpi, &
             bc_x, bc_y, bc_z, fluid_pp, probe_wrt, &
             fd_order, probe, num_probes, t_step_old, &
-            weno_Re_flux, weno_order, precision, parallel_io, bulk_precision
+            weno_Re_flux, weno_order, precision, parallel_io, bulk_precision, mlp_weights
         inquire (FILE=trim(file_path), EXIST=file_exist)
         if (.not. file_exist) then
             print '(A)', trim(file_path)//' is missing. Exiting ...'

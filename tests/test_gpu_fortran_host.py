@@ -18,8 +18,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SIM = os.path.join(ROOT, "microfc_amd", "fortran", "simulation")
 
 
-def _run(tmp_path, prm, grids, q, dt, nsteps, save, run_time_info=False):
+def _run(tmp_path, prm, grids, q, dt, nsteps, save, run_time_info=False, **extra):
     case = C.case_from_params(prm, dt=dt, t_step_stop=nsteps, t_step_save=save, run_time_info=run_time_info)
+    case.update(extra)
     C.write_inp(case, str(tmp_path / "simulation.inp"))
     C.write_serial(str(tmp_path), 0, prm, grids, q)
     say(f"[fortran host] {tmp_path.name} steps={nsteps}")
@@ -70,3 +71,48 @@ def test_fortran_driver_aborts_on_cfl_violation(gpu, tmp_path):
     C.write_serial(str(tmp_path), 0, prm, grids, q)
     r = subprocess.run([SIM], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "ICFL is greater than 1.0" in r.stdout
+
+
+def test_fortran_driver_learned_weights(gpu, tmp_path):
+    """bulk_precision = 2 with mlp_weights naming a file of FP32 parameters:
+    the driver loads the network, and its run equals the Python host's run
+    of the same library bit for bit."""
+    torch = gpu
+    from microfc_amd import PREC_FP32_LEARNED, RhsModule, learned
+    params = learned.init_params(3)
+    params[learned.OFF["W3"]:learned.OFF["W3"] + 512] = np.random.default_rng(4).normal(0, 0.3, 512)
+    params.astype("<f4").tofile(tmp_path / "mlp.bin")
+    prm, grids, q = ic.shock_bubble_2d(N=64)
+    q = perturbed(prm, q)
+    prm.precision = PREC_FP32_LEARNED
+    dt, nsteps = 1e-7, 8
+    _run(tmp_path, prm, grids, q, dt, nsteps, 8, mlp_weights="'mlp.bin'")
+    got = C.read_serial(str(tmp_path), 8, prm)
+    learned.Mlp(params)
+    mod = RhsModule(prm, grids)
+    try:
+        q1 = torch.from_numpy(q.copy()).cuda()
+        q2 = mod.new_field()
+        rhs = mod.new_field(ghosted=False)
+        for t, d in enumerate(C.dt_sequence(dt, 0, nsteps)):
+            mod.s_tvd_rk(t, (q1, q2), None, rhs, d)
+        mod.sync()
+        ref = prm.interior(q1.cpu().numpy())
+    finally:
+        mod.finalize()
+    assert np.array_equal(got, ref)
+    # and the network changed the result
+    prm.precision = 1
+    (tmp_path / "bulk").mkdir()
+    _run(tmp_path / "bulk", prm, grids, q, dt, nsteps, 8)
+    assert not np.array_equal(C.read_serial(str(tmp_path / "bulk"), 8, prm), got)
+
+
+def test_fortran_driver_learned_needs_weights(gpu, tmp_path):
+    prm, grids, q = ic.sod_1d(m=99)
+    prm.precision = 2
+    case = C.case_from_params(prm, dt=1e-4, t_step_stop=2, t_step_save=2)
+    C.write_inp(case, str(tmp_path / "simulation.inp"))
+    C.write_serial(str(tmp_path), 0, prm, grids, q)
+    r = subprocess.run([SIM], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "mlp_weights" in r.stdout
