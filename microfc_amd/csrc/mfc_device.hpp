@@ -87,14 +87,20 @@ struct WenoCell {
 // is the stencil j-polyn..j+polyn, cf the cell's coefficients.  Computes the
 // smoothness indicators once and evaluates the left (face j-1/2) and/or right
 // (face j+1/2) value.
-// FP64 quotient for operands in the normal range: the instruction sequence
-// the compiler emits for an IEEE a / b on gfx950 (v_rcp_f64, two Newton
-// steps, the quotient and one residual correction) without its range-scaling
+// FP64 quotient of the WENO weights: the instruction sequence the compiler
+// emits for an IEEE a / b on gfx950 (v_rcp_f64, two Newton steps, the
+// quotient and one residual correction) without its range-scaling
 // (v_div_scale) and special-case (v_div_fixup) steps.  Those steps change
-// nothing when a, b, 1/b and a/b are normal numbers whose exponents differ
-// by less than 768 and |a| > 2^-970; the result is then the correctly
-// rounded quotient, bit for bit the same as a / b.  weno_cell uses it only
-// behind a range test of its divisors (kNdivLo..kNdivHi).
+// nothing -- the result is the correctly rounded quotient, bit for bit
+// a / b -- when b, 1/b and a/b are normal, |a| >= 2^-969 (or a = 0) and the
+// exponents of a and b differ by less than 768.  For the weights this holds
+// whenever every squared smoothness indicator beta^2 lies in [1e-200, 1e100]:
+// alpha = d/beta^2 with ideal weights d in [1e-3, 1] is then in [1e-103,
+// 1e200] and omega = alpha/sum(alpha) >= 3e-304.  beta >= weno_eps >= 1e-100
+// (checked by mfc_rhs_init) gives the lower end; the upper end, beta <= 1e50,
+// holds for any stencil whose differences are below ~1e24, i.e. for every
+// physical state (a diverging run reaches inf/NaN, which both divisions
+// propagate, and the NaN guard stops it).
 __device__ __forceinline__ double ndiv(double a, double b) {
     const double r = __builtin_amdgcn_rcp(b);
     const double e0 = __builtin_fma(-b, r, 1.0);
@@ -105,10 +111,6 @@ __device__ __forceinline__ double ndiv(double a, double b) {
     const double rem = __builtin_fma(-b, q, a);
     return __builtin_fma(rem, r2, q);
 }
-// Squared smoothness indicators inside [2^-300, 2^300] keep every WENO
-// quotient (alpha = d / beta^2 with ideal weights d, omega = alpha / sum)
-// normal, so ndiv is exact there.
-constexpr double kNdivLo = 0x1p-300, kNdivHi = 0x1p300;
 
 template <bool FAST>
 __device__ __forceinline__ double wdiv(double a, double b) {
@@ -125,9 +127,8 @@ __device__ __forceinline__ float wdiv(float a, float b) {
 // Reconstruction of one variable at one cell (m_weno.fpp:391-540): v[0..WO-1]
 // is the stencil j-polyn..j+polyn, cf the cell's coefficients.  Computes the
 // smoothness indicators once and evaluates the left (face j-1/2) and/or right
-// (face j+1/2) value.  In FP64 the weight divisions take ndiv when the
-// squared indicators are in its exact range (always, for physical states),
-// else IEEE division: the result is the same either way.
+// (face j+1/2) value.  In FP64 the weight divisions are ndiv (bit-identical
+// to IEEE division in the range stated there).
 template <int WO, typename R, bool WANT_L, bool WANT_R>
 __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL, R &vR) {
     if constexpr (WO == 1) {
@@ -156,14 +157,7 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
                 vR = wdiv<F>(a0, s) * pR0 + wdiv<F>(a1, s) * pR1;
             }
         };
-        if constexpr (sizeof(R) == 8) {
-            if (min_ref(bb0, bb1) >= R(kNdivLo) && max_ref(bb0, bb1) <= R(kNdivHi))
-                weights(std::true_type{});
-            else
-                weights(std::false_type{});
-        } else {
-            weights(std::false_type{});
-        }
+        weights(std::integral_constant<bool, sizeof(R) == 8>{});
     } else {
         const R vj = v[2];
         const R d1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];
@@ -197,15 +191,7 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
                 vR = wdiv<F>(a0, s) * pR[0] + wdiv<F>(a1, s) * pR[1] + wdiv<F>(a2, s) * pR[2];
             }
         };
-        if constexpr (sizeof(R) == 8) {
-            const R lo = min_ref(min_ref(bb0, bb1), bb2), hi = max_ref(max_ref(bb0, bb1), bb2);
-            if (lo >= R(kNdivLo) && hi <= R(kNdivHi))
-                weights(std::true_type{});
-            else
-                weights(std::false_type{});
-        } else {
-            weights(std::false_type{});
-        }
+        weights(std::integral_constant<bool, sizeof(R) == 8>{});
     }
 }
 

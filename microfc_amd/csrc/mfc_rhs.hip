@@ -159,6 +159,12 @@ int check_params(const mfc_params &p) {
         return fail(-1, "Unsupported precision");
     if (p.precision == MFC_PREC_FP32_LEARNED && p.weno_order != 5)
         return fail(-1, "learned WENO weights need weno_order = 5");
+    // The FP64 weight divisions (ndiv) equal IEEE division while the squared
+    // smoothness indicators stay within [1e-200, 1e100], which weno_eps >=
+    // 1e-100 guarantees from below.
+    if (p.precision == MFC_PREC_FP64 && p.weno_order > 1 && !(p.weno_eps >= 1e-100))
+        return fail(-1, "weno_eps below 1e-100 is not supported in FP64 (the weight division is exact for "
+                        "weno_eps >= 1e-100)");
     return 0;
 }
 

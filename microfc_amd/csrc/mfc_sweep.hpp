@@ -79,25 +79,55 @@ __device__ __forceinline__ long xcd_logical_block(long bid, long nblocks) {
 template <typename R>
 __device__ __forceinline__ double to_d(R x) { return (double)x; }
 
-// TVD-RK stage combination of the base, the stage input and rhs.
+// Operands of finish_cell that come from HBM: the rhs accumulated by the
+// earlier sweeps, the RK base and the stage input (the volume fractions of
+// the stage input feed the alpha*div(u) source in every sweep).
 template <int S>
-__device__ __forceinline__ void rk_combine(const SweepArgs &a, long go, const double *r, double *o) {
+struct FinishIn {
+    double rhs[S], base[S], in[S];
+};
+
+template <int NF, int D>
+__device__ __forceinline__ void finish_load(const SweepArgs &a, long go, long ro, FinishIn<Sys<NF, D>::S> &f) {
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    if (a.accumulate) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) f.rhs[v] = a.rhs.v[v][ro];
+    }
+    if (a.rk == kRkNone || a.rk == kRkEuler) {
+#pragma unroll
+        for (int i = 0; i < NF; ++i) f.in[Y::ADV + i] = a.qin.v[Y::ADV + i][go];
+    } else {
+#pragma unroll
+        for (int v = 0; v < S; ++v) f.in[v] = a.qin.v[v][go];
+    }
+    if (a.rk != kRkNone) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) f.base[v] = a.qbase.v[v][go];
+    }
+}
+
+// TVD-RK stage combination of the base, the stage input and rhs
+// (m_time_steppers.fpp:129-362).
+template <int S>
+__device__ __forceinline__ void rk_combine(const SweepArgs &a, const FinishIn<S> &f, const double *r, double *o) {
     switch (a.rk) {
     case kRkEuler:
 #pragma unroll
-        for (int v = 0; v < S; ++v) o[v] = a.qbase.v[v][go] + a.dt * r[v];
+        for (int v = 0; v < S; ++v) o[v] = f.base[v] + a.dt * r[v];
         break;
     case kRk2Final:
 #pragma unroll
-        for (int v = 0; v < S; ++v) o[v] = (a.qbase.v[v][go] + a.qin.v[v][go] + a.dt * r[v]) / 2.0;
+        for (int v = 0; v < S; ++v) o[v] = (f.base[v] + f.in[v] + a.dt * r[v]) / 2.0;
         break;
     case kRk3Stage2:
 #pragma unroll
-        for (int v = 0; v < S; ++v) o[v] = (3.0 * a.qbase.v[v][go] + a.qin.v[v][go] + a.dt * r[v]) / 4.0;
+        for (int v = 0; v < S; ++v) o[v] = (3.0 * f.base[v] + f.in[v] + a.dt * r[v]) / 4.0;
         break;
     default:
 #pragma unroll
-        for (int v = 0; v < S; ++v) o[v] = (a.qbase.v[v][go] + 2.0 * a.qin.v[v][go] + 2.0 * a.dt * r[v]) / 3.0;
+        for (int v = 0; v < S; ++v) o[v] = (f.base[v] + 2.0 * f.in[v] + 2.0 * a.dt * r[v]) / 3.0;
         break;
     }
 }
@@ -106,24 +136,25 @@ __device__ __forceinline__ void rk_combine(const SweepArgs &a, long go, const do
 // rhs store or the fused RK stage (m_rhs.fpp:565-652, m_time_steppers.fpp).
 template <int NF, int D, typename R>
 __device__ __forceinline__ void finish_cell(const SweepArgs &a, long go, long ro, double inv,
-                                            const R *Fm, const R *Fp, R usm, R usp) {
+                                            const R *Fm, const R *Fp, R usm, R usp,
+                                            const FinishIn<Sys<NF, D>::S> &f) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     double r[S];
 #pragma unroll
     for (int v = 0; v < S; ++v) {
         const double df = to_d(Fm[v]) - to_d(Fp[v]);
-        r[v] = a.accumulate ? a.rhs.v[v][ro] + inv * df : inv * df;
+        r[v] = a.accumulate ? f.rhs[v] + inv * df : inv * df;
     }
     const double dus = to_d(usp) - to_d(usm);
 #pragma unroll
-    for (int i = 0; i < NF; ++i) r[Y::ADV + i] = r[Y::ADV + i] + inv * a.qin.v[Y::ADV + i][go] * dus;
+    for (int i = 0; i < NF; ++i) r[Y::ADV + i] = r[Y::ADV + i] + inv * f.in[Y::ADV + i] * dus;
     if constexpr (!std::is_same<R, double>::value) {
         // FP32 bulk: the RK stage accumulates in FP64 and clips the volume
         // fractions to [0, 1] (positivity; the reference has no clip).
         if (a.rk != kRkNone) {
             double o[S];
-            rk_combine<S>(a, go, r, o);
+            rk_combine<S>(a, f, r, o);
 #pragma unroll
             for (int i = 0; i < NF; ++i) o[Y::ADV + i] = min(max(o[Y::ADV + i], 0.0), 1.0);
 #pragma unroll
@@ -138,21 +169,19 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, long go, long ro
         break;
     case kRkEuler:
 #pragma unroll
-        for (int v = 0; v < S; ++v) a.qout.v[v][go] = a.qbase.v[v][go] + a.dt * r[v];
+        for (int v = 0; v < S; ++v) a.qout.v[v][go] = f.base[v] + a.dt * r[v];
         break;
     case kRk2Final:
 #pragma unroll
-        for (int v = 0; v < S; ++v) a.qout.v[v][go] = (a.qbase.v[v][go] + a.qin.v[v][go] + a.dt * r[v]) / 2.0;
+        for (int v = 0; v < S; ++v) a.qout.v[v][go] = (f.base[v] + f.in[v] + a.dt * r[v]) / 2.0;
         break;
     case kRk3Stage2:
 #pragma unroll
-        for (int v = 0; v < S; ++v)
-            a.qout.v[v][go] = (3.0 * a.qbase.v[v][go] + a.qin.v[v][go] + a.dt * r[v]) / 4.0;
+        for (int v = 0; v < S; ++v) a.qout.v[v][go] = (3.0 * f.base[v] + f.in[v] + a.dt * r[v]) / 4.0;
         break;
     default:
 #pragma unroll
-        for (int v = 0; v < S; ++v)
-            a.qout.v[v][go] = (a.qbase.v[v][go] + 2.0 * a.qin.v[v][go] + 2.0 * a.dt * r[v]) / 3.0;
+        for (int v = 0; v < S; ++v) a.qout.v[v][go] = (f.base[v] + 2.0 * f.in[v] + 2.0 * a.dt * r[v]) / 3.0;
         break;
     }
 }
@@ -368,8 +397,10 @@ __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
             Fm[v] = sf[v][lane];
             Fp[v] = sf[v][lane + 1];
         }
+        FinishIn<S> fin;
+        finish_load<NF, D>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), fin);
         finish_cell<NF, D, R>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), a.inv_ds[i], Fm, Fp,
-                              sf[S][lane], sf[S][lane + 1]);
+                              sf[S][lane], sf[S][lane + 1], fin);
     }
 }
 
@@ -381,7 +412,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
 constexpr int kTX = 16, kTS = 16;
 
 template <int NF, int D, int DIR, int WO, typename R, bool ML>
-__global__ __launch_bounds__(256) void k_sweep_t(const SweepArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4, 8))) void k_sweep_t(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int PN = (WO - 1) / 2;
@@ -413,6 +444,11 @@ __global__ __launch_bounds__(256) void k_sweep_t(const SweepArgs a) {
     auto rcell = [&](int i, int c) -> long {
         return (DIR == 1) ? interior_off(a, i, c, t) : interior_off(a, i, t, c);
     };
+    FinishIn<S> fin;
+#ifdef MFC_PREFETCH_FINISH
+    if (ts < kTS - 1 && x0 + tx <= a.m && y0 + ts <= s)
+        finish_load<NF, D>(a, gcell(x0 + tx, y0 + ts), rcell(x0 + tx, y0 + ts), fin);
+#endif
 
     // 1. stage primitive rows y0-1-PN .. y0+kTS-1+PN of columns x0..x0+kTX-1
     for (int e = tid; e < NR * kTX; e += 256) {
@@ -500,8 +536,11 @@ __global__ __launch_bounds__(256) void k_sweep_t(const SweepArgs a) {
             Fm[v] = sf[v][ts][tx];
             Fp[v] = sf[v][ts + 1][tx];
         }
+#ifndef MFC_PREFETCH_FINISH
+        finish_load<NF, D>(a, gcell(i, cc), rcell(i, cc), fin);
+#endif
         finish_cell<NF, D, R>(a, gcell(i, cc), rcell(i, cc), a.inv_ds[cc], Fm, Fp, sf[S][ts][tx],
-                              sf[S][ts + 1][tx]);
+                              sf[S][ts + 1][tx], fin);
     }
 }
 

@@ -102,8 +102,8 @@ class Mlp:
         return loss.value
 
     def infer(self, S):
-        ptr, n = self._samples(S)
-        out = self.torch.zeros((4, n), dtype=self.torch.float32, device="cuda")
+        out = self.torch.zeros((4, S.shape[1]), dtype=self.torch.float32, device="cuda")
+        ptr, n = self._samples(S)  # also orders the zero fill of out before the kernel
         _check(_lib().mfc_mlp_infer(ptr, n, ctypes.c_void_p(out.data_ptr())), "mfc_mlp_infer")
         return out
 
@@ -116,7 +116,15 @@ class Mlp:
             return np.asarray(f["params"], dtype=np.float32)
 
 
+def _torch_done():
+    """The library runs on its own streams: work torch queued on the caller's
+    buffers (allocation fills, copies) must be complete before a call."""
+    import torch
+    torch.cuda.current_stream().synchronize()
+
+
 def restrict(m: int, n: int, b: int, fine_var, coarse_var):
+    _torch_done()
     _check(_lib().mfc_ml_restrict(m, n, b, ctypes.c_void_p(fine_var.data_ptr()), ctypes.c_void_p(coarse_var.data_ptr())),
            "mfc_ml_restrict")
 
@@ -126,6 +134,7 @@ def samples(direction: int, m: int, n: int, b: int, coarse_var, fine_var, coef27
     ns = 2 * (m + 1) * (n + 1)
     out = torch.zeros((13, ns), dtype=torch.float32, device="cuda")
     cf = np.ascontiguousarray(coef27, dtype=np.float64)
+    _torch_done()
     _check(_lib().mfc_ml_samples(direction, m, n, b, ctypes.c_void_p(coarse_var.data_ptr()),
                                  ctypes.c_void_p(fine_var.data_ptr()),
                                  cf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), eps,
