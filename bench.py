@@ -22,6 +22,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
+# Measured HBM bytes (read + written) per launch of the dominant kernel, from
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script's
+# default command, scaled by a 2 GiB device copy (FETCH_SIZE x2.0, WRITE_SIZE
+# x1.0), and its VALU-busy fraction from SQ_ACTIVE_INST_VALU / GRBM_GUI_ACTIVE
+# (profiles/hbm_traffic_shock_bubble_2d_2048_fp64.md,
+# profiles/sweep_kernels_shock_bubble_2d_2048_fp64.md).  Other workloads
+# report null.
+PROFILED = {("shock_bubble_2d", 2048, "fp64", "sweep_y"): {"traffic": 739.7e6 + 277.6e6, "valu_busy": 0.75}}
+
 
 def sweep_bytes_per_step(prm):
     """Algorithmic HBM bytes per RK3 step of each sweep kernel, per cell:
@@ -140,6 +149,7 @@ def main():
     dom_bytes_per_launch = per_step[dom] * ncell / 3.0  # three launches per step
     dom_avg_s = dom_ms / 1e3 / dom_n
     achieved = dom_bytes_per_launch / dom_avg_s / 1e9
+    prof = PROFILED.get((args.case, N, args.precision, dom), {})
     S, D = prm.sys_size, prm.num_dims
     step_bytes = 64 * S * ncell
     f_alg = 3 * (D * (85 * S + 180 + 4 * S + 10) + 20 + 4 * S)
@@ -160,7 +170,8 @@ def main():
         "config": {"workload": workload, "cells": ncell, "sys_size": S, "cfl": cfl, "dt": dt,
                    "precision": args.precision, "time_stepper": "TVD-RK3"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("traffic"),
+                     "valu_busy_profiled": prof.get("valu_busy"),
                      "algorithmic_bytes_per_launch": dom_bytes_per_launch,
                      "avg_launch_ms": dom_avg_s * 1e3},
         "step_hbm": {"algorithmic_bytes_per_cell_step": 64 * S, "achieved_GBps": step_bytes * args.steps / sec / 1e9,

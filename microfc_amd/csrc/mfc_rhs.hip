@@ -525,7 +525,10 @@ SweepArgs sweep_args(int d) {
     a.bc_beg = bcs[d][0];
     a.bc_end = bcs[d][1];
     if (d == 0) {
-        a.ntile = (G.prm.m + 1 + 62) / 63;
+        const int tw = G.prm.precision == MFC_PREC_FP32_LEARNED ? x_tile_cells<true, float>()
+                       : G.prm.precision == MFC_PREC_FP32_BULK  ? x_tile_cells<false, float>()
+                                                                : x_tile_cells<false, double>();
+        a.ntile = (G.prm.m + 1 + tw - 1) / tw;
         a.nblocks = (long)a.ntile * (G.prm.n + 1) * (G.prm.p + 1);
     } else {
         const int s = G.dir[d].s;
@@ -571,6 +574,7 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         if (last && rk != kRkNone) {
             a.qbase = *qbase;
             a.qout = *qout;
+            a.base_is_in = qbase->v[0] == qin.v[0];
         }
         if (int rc = sweep(d, a)) return rc;
     }

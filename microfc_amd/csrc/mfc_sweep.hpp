@@ -52,6 +52,7 @@ struct SweepArgs {
     int bc_beg, bc_end;   // of this direction
     int rk;               // RkKind
     int accumulate;       // 0: first direction assigns rhs, 1: adds
+    int base_is_in;       // the RK base is the stage input (first stages)
     int chunk, nchunk;    // cells per tile along the sweep and tile count (k_sweep_t)
     int ntile;            // x tiles (k_sweep_x) or x column groups (k_sweep_t)
     long nblocks;
@@ -105,6 +106,31 @@ __device__ __forceinline__ void finish_load(const SweepArgs &a, long go, long ro
     if (a.rk != kRkNone) {
 #pragma unroll
         for (int v = 0; v < S; ++v) f.base[v] = a.qbase.v[v][go];
+    }
+}
+
+// The same with the stage input of the cell taken from the copy a sweep
+// staged in LDS (element v at cin[v * stride]); the RK base too when it is
+// the stage input.
+template <int NF, int D>
+__device__ __forceinline__ void finish_load_staged(const SweepArgs &a, long go, long ro, const double *cin, int stride,
+                                                   FinishIn<Sys<NF, D>::S> &f) {
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    if (a.accumulate) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) f.rhs[v] = a.rhs.v[v][ro];
+    }
+#pragma unroll
+    for (int v = 0; v < S; ++v) f.in[v] = cin[v * stride];
+    if (a.rk != kRkNone) {
+        if (a.base_is_in) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) f.base[v] = f.in[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < S; ++v) f.base[v] = a.qbase.v[v][go];
+        }
     }
 }
 
@@ -190,6 +216,13 @@ template <int NF, int D>
 __device__ __forceinline__ void load_prim(const SweepArgs &a, long go, double *p) {
     constexpr int S = Sys<NF, D>::S;
     double c[S];
+#pragma unroll
+    for (int v = 0; v < S; ++v) c[v] = a.qin.v[v][go];
+    cons_to_prim<NF, D>(c, p, a.eos);
+}
+template <int NF, int D>
+__device__ __forceinline__ void load_cons_prim(const SweepArgs &a, long go, double *c, double *p) {
+    constexpr int S = Sys<NF, D>::S;
 #pragma unroll
     for (int v = 0; v < S; ++v) c[v] = a.qin.v[v][go];
     cons_to_prim<NF, D>(c, p, a.eos);
@@ -299,13 +332,26 @@ __device__ __forceinline__ void ml_face_states(int var, const float *stR, const 
 }
 
 // ---------------------------------------------------------------------------
-// x sweep
+// x sweep: one wave per tile of a row.  Face-parallel (FP64 and the learned
+// form): lane L reconstructs the two sides of face x0-1+L + 1/2 from the
+// stencils of its two cells; 63 cells per tile are finished.  Cell-parallel
+// (FP32 bulk): lane L reconstructs both faces of cell x0-1+L (the smoothness
+// indicators serve both), the left-face values move one lane down through
+// LDS, and lane L solves face x0-1+L + 1/2; 62 cells per tile.  Measured on
+// MI355X, cell-parallel is 6 % faster in FP32 and no faster in FP64, where
+// its larger register footprint costs a wave of occupancy.
+template <bool ML, typename R>
+constexpr bool x_cell_parallel() { return !ML && sizeof(R) == 4; }
+template <bool ML, typename R>
+constexpr int x_tile_cells() { return x_cell_parallel<ML, R>() ? 62 : 63; }
+
 template <int NF, int D, int WO, typename R, bool ML>
-__global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4, 8))) void k_sweep_x(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int PN = (WO - 1) / 2;
     constexpr int NC = 64 + 2 * PN + 1;
+    constexpr int TW = x_tile_cells<ML, R>();
     __shared__ R sp[S][NC];
     __shared__ R sf[S + 1][64];
 
@@ -315,7 +361,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
     const int j = (int)(row % (a.n + 1));
     const int k = (int)(row / (a.n + 1));
     const int lane = threadIdx.x;
-    const int x0 = tile * 63;
+    const int x0 = tile * TW;
 
     // 1. stage primitive stencil cells x0-1-PN .. x0+63+PN
     for (int c = lane; c < NC; c += 64) {
@@ -332,7 +378,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
     }
     __syncthreads();
 
-    // 2. face x0-1+lane + 1/2
+    // 2. face jf + 1/2, jf = x0-1+lane: qL left of it, qR right of it
     const int jf = x0 - 1 + lane;
     R qL[S], qR[S];
     if constexpr (ML) {
@@ -354,25 +400,47 @@ __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
             }
             ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, Rg, ml[0], lane, qL[v], qR[v]);
         }
-    }
-    if (jf <= a.m) {
-        R f[S], us;
-        if constexpr (!ML) {
-            R cfR[kNCoef], cfL[kNCoef];
-            load_coef<R, WO>(a, jf, cfR);
-            load_coef<R, WO>(a, jf + 1, cfL);
-            const R eps = (R)a.eps;
+    } else if constexpr (x_cell_parallel<ML, R>()) {
+        // both reconstructions of cell jf (coefficients of a cell past the
+        // last face are clamped; those lanes' values are not used), one
+        // variable at a time through LDS to bound the live registers
+        __shared__ R sl[S][64], sr[S][64];
+        R cf[kNCoef];
+        load_coef<R, WO>(a, min(jf, a.m + 1), cf);
+        const R eps = (R)a.eps;
+#pragma unroll 1
+        for (int v = 0; v < S; ++v) {
+            R st[WO], vl, vr;
 #pragma unroll
-            for (int v = 0; v < S; ++v) {
-                R st[WO], dummy;
-#pragma unroll
-                for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + w];
-                weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
-#pragma unroll
-                for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + 1 + w];
-                weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
-            }
+            for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + w];
+            weno_cell<WO, R, true, true>(st, cf, eps, vl, vr);
+            sl[v][lane] = vl;
+            sr[v][lane] = vr;
         }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            qL[v] = sr[v][lane];
+            qR[v] = sl[v][min(lane + 1, 63)];
+        }
+    } else if (jf <= a.m) {
+        R cfR[kNCoef], cfL[kNCoef];
+        load_coef<R, WO>(a, jf, cfR);
+        load_coef<R, WO>(a, jf + 1, cfL);
+        const R eps = (R)a.eps;
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            R st[WO], dummy;
+#pragma unroll
+            for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + w];
+            weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
+#pragma unroll
+            for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + 1 + w];
+            weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
+        }
+    }
+    if (lane < TW + 1 && jf <= a.m) {
+        R f[S], us;
         if (jf == -1 && a.bc_beg == -4) {
 #pragma unroll
             for (int v = 0; v < S; ++v) qL[v] = qR[v];
@@ -390,7 +458,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const SweepArgs a) {
 
     // 3. cell x0+lane
     const int i = x0 + lane;
-    if (lane < 63 && i <= a.m) {
+    if (lane < TW && i <= a.m) {
         R Fm[S], Fp[S];
 #pragma unroll
         for (int v = 0; v < S; ++v) {
@@ -419,7 +487,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4,
     constexpr int NR = kTS + 2 * PN + 1; // staged rows
     constexpr int PRIM_WORDS = S * NR * kTX;
     constexpr int FLUX_WORDS = (S + 1) * kTS * kTX;
+    constexpr int LDS_BYTES = sizeof(R) * (PRIM_WORDS > FLUX_WORDS ? PRIM_WORDS : FLUX_WORDS);
+    // The conservative stage input of the tile's own cells stays in LDS for
+    // the finish (rhs accumulation, RK stage) instead of being read from HBM
+    // a second time, while the block still fits four to a CU.
+    constexpr bool KEEP = LDS_BYTES + 8 * S * (kTS - 1) * kTX <= 40 * 1024;
     __shared__ R lds[PRIM_WORDS > FLUX_WORDS ? PRIM_WORDS : FLUX_WORDS];
+    __shared__ double lcons[KEEP ? S : 1][kTS - 1][kTX];
     R(*sp)[NR][kTX] = reinterpret_cast<R(*)[NR][kTX]>(lds);
     R(*sf)[kTS][kTX] = reinterpret_cast<R(*)[kTS][kTX]>(lds);
 
@@ -444,26 +518,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4,
     auto rcell = [&](int i, int c) -> long {
         return (DIR == 1) ? interior_off(a, i, c, t) : interior_off(a, i, t, c);
     };
-    FinishIn<S> fin;
-#ifdef MFC_PREFETCH_FINISH
-    if (ts < kTS - 1 && x0 + tx <= a.m && y0 + ts <= s)
-        finish_load<NF, D>(a, gcell(x0 + tx, y0 + ts), rcell(x0 + tx, y0 + ts), fin);
-#endif
 
     // 1. stage primitive rows y0-1-PN .. y0+kTS-1+PN of columns x0..x0+kTX-1
     for (int e = tid; e < NR * kTX; e += 256) {
         const int ex = e % kTX, er = e / kTX;
         const int c = y0 - 1 - PN + er;
         const int i = min(x0 + ex, a.m);
-        double pr[S];
+        double pr[S], cv[S];
         if (c <= s + a.b) {
-            load_prim<NF, D>(a, gcell(i, c), pr);
+            load_cons_prim<NF, D>(a, gcell(i, c), cv, pr);
         } else {
 #pragma unroll
-            for (int v = 0; v < S; ++v) pr[v] = 0.0;
+            for (int v = 0; v < S; ++v) pr[v] = cv[v] = 0.0;
         }
 #pragma unroll
         for (int v = 0; v < S; ++v) sp[v][er][ex] = (R)pr[v];
+        if constexpr (KEEP) {
+            if (er > PN && er < PN + kTS) {
+#pragma unroll
+                for (int v = 0; v < S; ++v) lcons[v][er - PN - 1][ex] = cv[v];
+            }
+        }
     }
     __syncthreads();
 
@@ -536,9 +611,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4,
             Fm[v] = sf[v][ts][tx];
             Fp[v] = sf[v][ts + 1][tx];
         }
-#ifndef MFC_PREFETCH_FINISH
-        finish_load<NF, D>(a, gcell(i, cc), rcell(i, cc), fin);
-#endif
+        FinishIn<S> fin;
+        if constexpr (KEEP)
+            finish_load_staged<NF, D>(a, gcell(i, cc), rcell(i, cc), &lcons[0][ts][tx], (kTS - 1) * kTX, fin);
+        else
+            finish_load<NF, D>(a, gcell(i, cc), rcell(i, cc), fin);
         finish_cell<NF, D, R>(a, gcell(i, cc), rcell(i, cc), a.inv_ds[cc], Fm, Fp, sf[S][ts][tx],
                               sf[S][ts + 1][tx], fin);
     }
