@@ -84,6 +84,31 @@ __global__ void k_mlp_pack(const float *P, MlpFrags *F) {
         F->b2[l] = P[kOffB2 + l];
     }
     if (l < 16) F->b3[l] = P[kOffB3 + l];
+    // transposed inference: W3^T rows shifted to 4q..4q+2, bias fragments
+    // (element e of lane l is unit 16n + 4*(l/16) + e)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            bf16x4 w;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int o = j - 4 * q; // output row of this lane's A row
+                w[e] = (short)((o >= 0 && o < kMlpOut) ? f2bf(P[kOffW3 + (16 * c + k0 + e) * 16 + o]) : 0);
+            }
+            F->w3q[q][c][l] = w;
+        }
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+        f32x4 b1, b2;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            b1[e] = P[kOffB1 + 16 * n + k0 + e];
+            b2[e] = P[kOffB2 + 16 * n + k0 + e];
+        }
+        F->c1[n][l] = b1;
+        F->c2[n][l] = b2;
+    }
 }
 
 struct TrainLds {

@@ -57,6 +57,11 @@ struct MlpFrags {
     bf16x4 w3t[2][64];
     bf16x4 w2t[2][2][64];
     float b1[32], b2[32], b3[16];
+    // Transposed inference (activations as units x samples, see
+    // mlp_infer_lane): W3^T with its three output rows moved to rows
+    // 4q..4q+2 for lane group q, and the biases as accumulator fragments.
+    bf16x4 w3q[4][2][64];
+    f32x4 c1[2][64], c2[2][64];
 };
 
 // The fragments one lane needs, held in registers for a whole kernel.
@@ -180,6 +185,75 @@ __device__ __forceinline__ void mlp_forward_tile(const MlpRegs &R, const unsigne
         for (int e = 0; e < 4; ++e) z[i0 + e][j] = d[e] + R.b3;
     }
     wave_lds_sync();
+}
+
+// ---------------------------------------------------------------------------
+// Transposed inference for the sweeps.  With activations stored as units x
+// samples (H^T = W^T X^T), the D fragment of one layer -- unit i of sample j
+// in element i%4 of lane j + 16*(i/4) -- is, after ReLU and BF16 rounding,
+// exactly the B fragment (row k = unit) of the next layer, so the three
+// layers chain in registers.  The weight A fragments are the ones of the
+// sample-major layout (the A and B lane layouts coincide).  For the output
+// layer, lane group q = lane/16 uses W3^T with its three rows moved to rows
+// 4q..4q+2: the logits of the sample owned by lane 16q + j then land in
+// elements 0..2 of that lane's accumulator, and the other groups' passes add
+// exact zeros there, so one accumulator per side collects every lane's own
+// logits over the four groups.
+struct MlpTRegs {
+    bf16x4 w1[2], w2[2][2], w3q[4][2];
+    f32x4 c1[2], c2[2], c3;
+};
+
+__device__ __forceinline__ void mlp_load_tregs(const MlpFrags *F, int lane, MlpTRegs &R) {
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+        R.w1[n] = F->w1[n][lane];
+        R.c1[n] = F->c1[n][lane];
+        R.c2[n] = F->c2[n][lane];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) R.w2[c][n] = F->w2[c][n][lane];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) R.w3q[q][n] = F->w3q[q][n][lane];
+    }
+    R.c3 = f32x4{F->b3[0], F->b3[1], F->b3[2], 0.f};
+}
+
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// ReLU and BF16 (round to nearest even) of a D fragment: v_cvt_pk_bf16_f32,
+// then the ReLU as a signed 16-bit max (a negative BF16 is a negative
+// int16), which commutes with the rounding.
+__device__ __forceinline__ bf16x4 relu_bf16(f32x4 d) {
+    const s16x2 lo = __builtin_elementwise_max(
+        __builtin_bit_cast(s16x2, __builtin_convertvector((f32x2){d[0], d[1]}, bf16x2_t)), (s16x2){0, 0});
+    const s16x2 hi = __builtin_elementwise_max(
+        __builtin_bit_cast(s16x2, __builtin_convertvector((f32x2){d[2], d[3]}, bf16x2_t)), (s16x2){0, 0});
+    return bf16x4{lo[0], lo[1], hi[0], hi[1]};
+}
+
+// Logits of the two samples (sides) of every lane of a wave: xs[side][lane]
+// holds the BF16 features (5, padded to 8) of that lane's sample; returns
+// this lane's logits in z[side][0..2].  32 MFMAs per side.
+__device__ __forceinline__ void mlp_infer_lane(const MlpTRegs &R, const unsigned short (*xs)[64][8], int lane,
+                                               f32x4 *z) {
+    const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        f32x4 acc = R.c3;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            bf16x4 b1 = {0, 0, 0, 0};
+            if (g < 2) b1 = *reinterpret_cast<const bf16x4 *>(&xs[side][16 * q + j][4 * g]);
+            const bf16x4 h0 = relu_bf16(mfma16(R.w1[0], b1, R.c1[0]));
+            const bf16x4 h1 = relu_bf16(mfma16(R.w1[1], b1, R.c1[1]));
+            const bf16x4 g0 = relu_bf16(mfma16(R.w2[1][0], h1, mfma16(R.w2[0][0], h0, R.c2[0])));
+            const bf16x4 g1 = relu_bf16(mfma16(R.w2[1][1], h1, mfma16(R.w2[0][1], h0, R.c2[1])));
+            acc = mfma16(R.w3q[q][1], g1, mfma16(R.w3q[q][0], g0, acc));
+        }
+        z[side] = acc;
+    }
 }
 
 // Learned weights of one face from its logits and plain weights.

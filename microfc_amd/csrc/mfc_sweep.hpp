@@ -242,18 +242,13 @@ __device__ __forceinline__ void load_coef(const SweepArgs &a, int cell, R *cf) {
 // owns two reconstructions: the right face of its left cell and the left
 // face of its right cell.  Both are oriented so the face lies between
 // stencil entries 2 and 3 (the left-face stencil is reversed, its candidates
-// and weights listed in the reversed order), their features go to LDS, the
-// wave evaluates the MLP on 8 tiles of 16 with MFMA, and each lane combines
-// the learned weights with its candidates.  Per face the plain weights are
-// the fallback when the learned value is not finite or breaks positivity
+// and weights listed in the reversed order), their BF16 features go to LDS,
+// the wave evaluates the MLP for its 128 samples with the transposed,
+// register-chained MFMA passes of mlp_infer_lane, and each lane combines its
+// own logits with its candidates.  Per face the plain weights are the
+// fallback when the learned value is not finite or breaks positivity
 // (partial densities and volume fractions >= 0, volume fractions <= 1,
 // pressure > 0 where the plain value is).
-
-struct MlLds {
-    unsigned short x[128][16];
-    MlpActLds act;
-    float z[128][4];
-};
 
 // Candidates and plain weights of one side of a WENO5 cell (FP32 bulk; the
 // same arithmetic as weno_cell), in the oriented order.
@@ -282,14 +277,15 @@ __device__ __forceinline__ void weno5_parts(const float *v, const float *cf, flo
     }
 }
 
-// Features of an oriented stencil into one row of the LDS tile.
+// Features of an oriented stencil into one LDS row: 5 BF16 values and 3
+// zero pads.  Returns the scale s.
 __device__ __forceinline__ float ml_features(const float *v, unsigned short *row) {
     float sc = 0.f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) sc = fmaxf(sc, fabsf(v[k + 1] - v[k]));
     const float inv = sc > 0.f ? 1.f / sc : 0.f;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) row[k] = k < kMlpIn ? f2bf((v[k] - v[2]) * inv) : (unsigned short)0;
+    for (int k = 0; k < 8; ++k) row[k] = k < kMlpIn ? f2bf((v[k] - v[2]) * inv) : (unsigned short)0;
     return sc;
 }
 
@@ -309,26 +305,26 @@ __device__ __forceinline__ float ml_value(int var, const float *z, const float *
 
 // Both face states of one variable for the wave's 64 faces.  stR: stencil of
 // the cell left of the lane's face (cells j-2..j+2), stL: of the cell right
-// of it (cells j-1..j+3); cfR/cfL their coefficients.
+// of it (cells j-1..j+3); cfR/cfL their coefficients; xs the wave's feature
+// rows.
 template <int NF, int D>
 __device__ __forceinline__ void ml_face_states(int var, const float *stR, const float *stL, const float *cfR,
-                                               const float *cfL, float eps, const MlpRegs &Rg, MlLds &L, int lane,
-                                               float &qLv, float &qRv) {
+                                               const float *cfL, float eps, const MlpTRegs &T,
+                                               unsigned short (*xs)[64][8], int lane, float &qLv, float &qRv) {
     float pR[3], wR[3], pL[3], wL[3], vo[5];
     weno5_parts<true>(stR, cfR, eps, pR, wR);
     weno5_parts<false>(stL, cfL, eps, pL, wL);
-    ml_features(stR, L.x[lane]);
+    ml_features(stR, xs[0][lane]);
 #pragma unroll
     for (int k = 0; k < 5; ++k) vo[k] = stL[4 - k];
-    ml_features(vo, L.x[64 + lane]);
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        MlpFwdState st;
-        mlp_forward_tile(Rg, L.x + 16 * t, L.act, L.z + 16 * t, lane, st);
-    }
-    qLv = ml_value<NF, D>(var, L.z[lane], pR, wR);
-    qRv = ml_value<NF, D>(var, L.z[64 + lane], pL, wL);
+    ml_features(vo, xs[1][lane]);
     wave_lds_sync();
+    f32x4 z[2];
+    mlp_infer_lane(T, xs, lane, z);
+    wave_lds_sync(); // the next variable rewrites xs
+    const float zR[3] = {z[0][0], z[0][1], z[0][2]}, zL[3] = {z[1][0], z[1][1], z[1][2]};
+    qLv = ml_value<NF, D>(var, zR, pR, wR);
+    qRv = ml_value<NF, D>(var, zL, pL, wL);
 }
 
 // ---------------------------------------------------------------------------
@@ -346,7 +342,7 @@ template <bool ML, typename R>
 constexpr int x_tile_cells() { return x_cell_parallel<ML, R>() ? 62 : 63; }
 
 template <int NF, int D, int WO, typename R, bool ML>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4, 8))) void k_sweep_x(const SweepArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 8))) void k_sweep_x(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int PN = (WO - 1) / 2;
@@ -382,23 +378,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4, 
     const int jf = x0 - 1 + lane;
     R qL[S], qR[S];
     if constexpr (ML) {
-        // every lane takes part in the wave's MFMA tiles, valid face or not
-        __shared__ MlLds ml[1];
-        MlpRegs Rg;
-        mlp_load_regs(a.mlp, lane, Rg);
+        // every lane takes part in the wave's MFMA passes, valid face or not;
+        // one variable at a time, results through LDS
+        __shared__ unsigned short xs[2][64][8];
+        __shared__ R mres[2][S][64];
+        MlpTRegs T;
+        mlp_load_tregs(a.mlp, lane, T);
         R cfR[kNCoef], cfL[kNCoef];
         load_coef<R, WO>(a, min(jf, a.m), cfR);
         load_coef<R, WO>(a, min(jf, a.m) + 1, cfL);
         const R eps = (R)a.eps;
-#pragma unroll
+#pragma unroll 1
         for (int v = 0; v < S; ++v) {
-            R stR[5], stL[5];
+            R stR[5], stL[5], l, r;
 #pragma unroll
             for (int w = 0; w < 5; ++w) {
                 stR[w] = sp[v][lane + w];
                 stL[w] = sp[v][lane + 1 + w];
             }
-            ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, Rg, ml[0], lane, qL[v], qR[v]);
+            ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, T, xs, lane, l, r);
+            mres[0][v][lane] = l;
+            mres[1][v][lane] = r;
+        }
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            qL[v] = mres[0][v][lane];
+            qR[v] = mres[1][v][lane];
         }
     } else if constexpr (x_cell_parallel<ML, R>()) {
         // both reconstructions of cell jf (coefficients of a cell past the
@@ -480,7 +485,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4, 
 constexpr int kTX = 16, kTS = 16;
 
 template <int NF, int D, int DIR, int WO, typename R, bool ML>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4, 8))) void k_sweep_t(const SweepArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 8))) void k_sweep_t(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int PN = (WO - 1) / 2;
@@ -548,23 +553,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ML ? 1 : 4,
     const bool face_ok = jf <= s;
     R qL[S], qR[S];
     if constexpr (ML) {
-        __shared__ MlLds ml[4];
+        __shared__ unsigned short xs[4][2][64][8];
+        __shared__ R mres[2][S][256];
         const int lane = tid & 63, wv = tid >> 6;
-        MlpRegs Rg;
-        mlp_load_regs(a.mlp, lane, Rg);
+        MlpTRegs T;
+        mlp_load_tregs(a.mlp, lane, T);
         R cfR[kNCoef], cfL[kNCoef];
         load_coef<R, WO>(a, min(jf, s), cfR);
         load_coef<R, WO>(a, min(jf, s) + 1, cfL);
         const R eps = (R)a.eps;
-#pragma unroll
+#pragma unroll 1
         for (int v = 0; v < S; ++v) {
-            R stR[5], stL[5];
+            R stR[5], stL[5], l, r;
 #pragma unroll
             for (int w = 0; w < 5; ++w) {
                 stR[w] = sp[v][ts + w][tx];
                 stL[w] = sp[v][ts + 1 + w][tx];
             }
-            ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, Rg, ml[wv], lane, qL[v], qR[v]);
+            ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, T, xs[wv], lane, l, r);
+            mres[0][v][tid] = l;
+            mres[1][v][tid] = r;
+        }
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            qL[v] = mres[0][v][tid];
+            qR[v] = mres[1][v][tid];
         }
     }
     if (face_ok) {
