@@ -107,12 +107,15 @@ def test_trained_weights_beat_plain_against_finer_judge(gpu):
     Nc = 128
     # training pairs from a plain fine run (seed 0), shear layer resolved by
     # about one coarse cell
-    fprm, fg, fq = ic.shear_layer_2d(N=2 * Nc, seed=0, delta=0.02)
-    data = learned.training_pairs(fprm, fg, fq, nsnap=12, steps_between=20)
-    say(f"  training pairs: {data.shape[1]}")
+    parts = []
+    for seed in (0, 1, 2):
+        fprm, fg, fq = ic.shear_layer_2d(N=2 * Nc, seed=seed, delta=0.02)
+        parts.append(learned.training_pairs(fprm, fg, fq, nsnap=12, steps_between=20))
+    data = torch.cat(parts, dim=1).contiguous()
+    say(f"  training pairs: {data.shape[1]} (three 256^2 plain runs)")
     mlp = learned.Mlp(seed=0)
     _, loss0 = mlp.gradient(data)
-    losses = learned.train(mlp, data, epochs=30, batch=8192, lr=3e-3, seed=0)
+    losses = learned.train(mlp, data, epochs=90, batch=8192, lr=3e-3, seed=0)
     _, loss1 = mlp.gradient(data)
     say(f"  loss over all pairs {loss0:.4e} -> {loss1:.4e} ({len(losses)} Adam steps on the GPU)")
     assert loss1 < 0.7 * loss0
@@ -142,3 +145,21 @@ def test_trained_weights_beat_plain_against_finer_judge(gpu):
         # all drop; the streamwise momentum is left to the report
         for v in (0, 2, 3):
             assert el[v] < e32[v], (seed, names[v])
+    # matched-error view: everything against a 4x finer plain FP64 run,
+    # compared on the coarse grid
+    seed, t_end = 5, 0.3
+    jprm, jg, jq = ic.shear_layer_2d(N=4 * Nc, seed=seed, delta=0.02)
+    judge4, _, s4 = learned.run(jprm, jg, jq, t_end)
+    ref = learned.restrict_host(learned.restrict_host(judge4))
+    fprm, fg, fq = ic.shear_layer_2d(N=2 * Nc, seed=seed, delta=0.02)
+    fine, _, s2 = learned.run(fprm, fg, fq, t_end)
+    cprm, cg, cq = ic.shear_layer_2d(N=Nc, seed=seed, delta=0.02)
+    cprm.precision = PREC_FP32_LEARNED
+    lrn, _, sl = learned.run(cprm, cg, cq, t_end)
+    cprm.precision = PREC_FP32_BULK
+    pl, _, sp = learned.run(cprm, cg, cq, t_end)
+    rows = [("plain FP32 128^2", pl, sp), ("learned 128^2", lrn, sl),
+            ("plain FP64 256^2", learned.restrict_host(fine), s2)]
+    say(f"  against the 512^2 plain FP64 run (GPU s {s4:.3f}), L1 on the 128^2 grid {names}:")
+    for nm, x, sec in rows:
+        say(f"    {nm:18s} {' '.join(f'{learned.l1_error(x, ref, v):.3e}' for v in range(4))}  GPU s {sec:.3f}")
