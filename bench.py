@@ -190,23 +190,37 @@ def main():
 
 
 def cpu_baseline(args, prm, grids, q, np):
-    """The FP64 oracle on one host core: one TVD-RK3 step of the same grid
-    (bounded sample of the workload), same dt rule."""
+    """The FP64 oracle on the host: one TVD-RK3 step of the same grid (a
+    bounded sample of the workload), same dt rule, on every host thread the
+    run may use (OpenMP over pencils, OMP_NUM_THREADS) and on one core."""
+    import ctypes
+
     from oracle import oracle as O
     oc = O.Case(m=prm.m, n=prm.n, p=prm.p, num_fluids=prm.num_fluids, weno_order=prm.weno_order,
                 time_stepper=prm.time_stepper, weno_eps=prm.weno_eps, gammas=list(prm.gammas),
                 pi_infs=list(prm.pi_infs), bc_beg=prm.bc_beg, bc_end=prm.bc_end)
     ncell = (prm.m + 1) * (prm.n + 1) * (prm.p + 1)
     if ncell > 20_000_000:
-        return {"value": None, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+        return {"value": None, "unit": "cell-updates/s", "cores": None, "kind": "port",
                 "sample": "skipped: grid too large for a bounded CPU sample"}
-    q1 = q.copy()
-    q2 = np.zeros_like(q1)
-    t0 = time.perf_counter()
-    O.tvd_rk_step(oc, grids, 1e-9, q1, q2, t_step=0)
-    sec = time.perf_counter() - t0
-    return {"value": ncell / sec, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"one TVD-RK3 step of the same {ncell}-cell grid ({sec:.1f} s) on one host core"}
+    L = O.lib()
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+
+    def one_step(n):
+        L.orc_set_threads(n)
+        q1 = q.copy()
+        q2 = np.zeros_like(q1)
+        t0 = time.perf_counter()
+        O.tvd_rk_step(oc, grids, 1e-9, q1, q2, t_step=0)
+        return time.perf_counter() - t0
+
+    sec_all = one_step(threads)
+    sec_one = one_step(1)
+    return {"value": ncell / sec_all, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"one TVD-RK3 step of the same {ncell}-cell grid on {threads} host threads "
+                      f"({sec_all:.2f} s) and on one core ({sec_one:.1f} s)",
+            "single_core": {"value": ncell / sec_one, "cores": 1}}
 
 
 if __name__ == "__main__":

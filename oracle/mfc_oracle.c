@@ -13,6 +13,26 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads for the pencil and update loops (OpenMP builds); the result does
+ * not depend on the count. */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 #define SGM_EPS 1e-16 /* m_global_parameters.fpp:39 */
 
@@ -546,11 +566,94 @@ int orc_bc_fill(const orc_case *c, double *q) {
 /* ------------------------------------------------------------------------ */
 /* s_compute_rhs: m_rhs.fpp:405-679                                          */
 
+/* One pencil of one direction of s_compute_rhs: WENO (m_weno.fpp:470-536)
+ * for cells -1..s+1, the Riemann-state buffers for bc = -4
+ * (m_riemann_solvers.fpp:480-550), HLLC at faces -1/2..s+1/2 with the right
+ * face value of cell j as the solver's left state (m_rhs.fpp:545-556), and
+ * the flux divergence and alpha*div(u) source (m_rhs.fpp:565-652).  The
+ * reference also reconstructs further ghost cells and ghost rows whose values
+ * never reach the interior fluxes. */
+static void pencil(const orc_case *c, const geom *gp, int dir, int s, int k1, int k2, const double *coef,
+                   const double *qc, const double *qp, const double *dsd, double *rhs) {
+    const geom g = *gp;
+    const long nc = s + 2 * g.b + 1;
+    const long ns = s + 3; /* cells -1..s+1 */
+    double *vL = (double *)malloc(sizeof(double) * g.S * ns);
+    double *vR = (double *)malloc(sizeof(double) * g.S * ns);
+    double *flux = (double *)malloc(sizeof(double) * g.S * (s + 2));
+    double *ustar = (double *)malloc(sizeof(double) * (s + 2));
+    int j, q, v;
+#define POS(jj, I, J, K)                                                   \
+    do {                                                                   \
+        if (dir == 0) { I = (jj); J = k1; K = k2; }                        \
+        else if (dir == 1) { I = k1; J = (jj); K = k2; }                   \
+        else { I = k1; J = k2; K = (jj); }                                 \
+    } while (0)
+    for (j = -1; j <= s + 1; j++) {
+        double cf[27];
+        for (q = 0; q < 27; q++) cf[q] = coef[q * nc + j + g.b];
+        for (v = 0; v < g.S; v++) {
+            double st[5];
+            int r;
+            for (r = -2; r <= 2; r++) {
+                int I, J, K, jj = j + r;
+                if (c->weno_order == 1 && r != 0) { st[r + 2] = 0.0; continue; }
+                if (c->weno_order == 3 && (r < -1 || r > 1)) { st[r + 2] = 0.0; continue; }
+                POS(jj, I, J, K);
+                st[r + 2] = qp[v * g.ng + gidx(&g, I, J, K)];
+            }
+            if (g_learned && c->weno_order == 5) {
+                const int kind = (v < g.nf || v >= g.advxb ? 1 : 0) | (v >= g.advxb ? 2 : 0) |
+                                 (v == g.E_idx ? 4 : 0);
+                orc_weno5_learned_cell(c->weno_eps, cf, st, kind, &vL[v * ns + (j + 1)], &vR[v * ns + (j + 1)]);
+            } else
+                orc_weno_cell(c->weno_order, c->weno_eps, cf, st, &vL[v * ns + (j + 1)], &vR[v * ns + (j + 1)]);
+        }
+    }
+    if (c->bc_beg[dir] == -4)
+        for (v = 0; v < g.S; v++) vR[v * ns + 0] = vL[v * ns + 1];
+    if (c->bc_end[dir] == -4)
+        for (v = 0; v < g.S; v++) vL[v * ns + (s + 2)] = vR[v * ns + (s + 1)];
+    for (j = -1; j <= s; j++) {
+        double ql[3 * ORC_NUM_FLUIDS_MAX + 4], qr[3 * ORC_NUM_FLUIDS_MAX + 4], f[3 * ORC_NUM_FLUIDS_MAX + 4];
+        for (v = 0; v < g.S; v++) {
+            ql[v] = vR[v * ns + (j + 1)];
+            qr[v] = vL[v * ns + (j + 2)];
+        }
+        orc_hllc_face(c, dir, ql, qr, f, &ustar[j + 1]);
+        for (v = 0; v < g.S; v++) flux[v * (s + 2) + (j + 1)] = f[v];
+    }
+    for (v = 0; v < g.S; v++)
+        for (j = 0; j <= s; j++) {
+            int I, J, K;
+            long r;
+            POS(j, I, J, K);
+            r = (long)v * (g.m + 1) * (g.n + 1) * (g.p + 1) + ridx(&g, I, J, K);
+            if (dir == 0)
+                rhs[r] = 1.0 / dsd[j + g.b] * (flux[v * (s + 2) + j] - flux[v * (s + 2) + j + 1]);
+            else
+                rhs[r] = rhs[r] + 1.0 / dsd[j + g.b] * (flux[v * (s + 2) + j] - flux[v * (s + 2) + j + 1]);
+        }
+    for (v = g.advxb; v < g.advxb + g.nf; v++)
+        for (j = 0; j <= s; j++) {
+            int I, J, K;
+            long r;
+            POS(j, I, J, K);
+            r = (long)v * (g.m + 1) * (g.n + 1) * (g.p + 1) + ridx(&g, I, J, K);
+            rhs[r] = rhs[r] + 1.0 / dsd[j + g.b] * qc[v * g.ng + gidx(&g, I, J, K)] * (ustar[j + 1] - ustar[j]);
+        }
+#undef POS
+    free(vL);
+    free(vR);
+    free(flux);
+    free(ustar);
+}
+
 int orc_compute_rhs(const orc_case *c, const double *const cb[3], const double *const ds[3],
                     const double *q_cons, double *q_prim, double *rhs, int t_step) {
     geom g;
-    double *qc, *qp, *coef, *vL, *vR, *flux, *ustar;
-    int dir, v;
+    double *qc, *qp, *coef;
+    int dir;
     mkgeom(c, &g);
     if (c->weno_order != 1 && c->weno_order != 3 && c->weno_order != 5) return -1;
     qc = (double *)malloc(sizeof(double) * g.S * g.ng);
@@ -568,92 +671,16 @@ int orc_compute_rhs(const orc_case *c, const double *const cb[3], const double *
         const int t1 = dir == 0 ? g.n : g.m;                  /* first transverse  */
         const int t2 = dir == 2 ? g.n : g.p;                  /* second transverse */
         const long nc = s + 2 * g.b + 1;
-        const long ns = s + 3;                                /* cells -1..s+1 */
-        const int a = 0;
-        int j, k1, k2, q;
+        const long npencil = (long)(t1 + 1) * (t2 + 1);
+        long pc;
         coef = (double *)malloc(sizeof(double) * 27 * nc);
-        vL = (double *)malloc(sizeof(double) * g.S * ns);
-        vR = (double *)malloc(sizeof(double) * g.S * ns);
-        flux = (double *)malloc(sizeof(double) * g.S * (s + 2));
-        ustar = (double *)malloc(sizeof(double) * (s + 2));
         orc_weno_coefficients(s, g.b, c->weno_order, cb[dir], coef);
-        (void)a;
-        for (k2 = 0; k2 <= t2; k2++)
-            for (k1 = 0; k1 <= t1; k1++) {
-                /* physical (i,j,k) of sweep position jj */
-#define POS(jj, I, J, K)                                                   \
-    do {                                                                   \
-        if (dir == 0) { I = (jj); J = k1; K = k2; }                        \
-        else if (dir == 1) { I = k1; J = (jj); K = k2; }                   \
-        else { I = k1; J = k2; K = (jj); }                                 \
-    } while (0)
-                /* WENO (m_weno.fpp:470-536) for cells -1..s+1; the reference
-                 * also reconstructs further ghost cells and ghost rows whose
-                 * values never reach the interior fluxes. */
-                for (j = -1; j <= s + 1; j++) {
-                    double cf[27];
-                    for (q = 0; q < 27; q++) cf[q] = coef[q * nc + j + g.b];
-                    for (v = 0; v < g.S; v++) {
-                        double st[5];
-                        int r;
-                        for (r = -2; r <= 2; r++) {
-                            int I, J, K, jj = j + r;
-                            if (c->weno_order == 1 && r != 0) { st[r + 2] = 0.0; continue; }
-                            if (c->weno_order == 3 && (r < -1 || r > 1)) { st[r + 2] = 0.0; continue; }
-                            POS(jj, I, J, K);
-                            st[r + 2] = qp[v * g.ng + gidx(&g, I, J, K)];
-                        }
-                        if (g_learned && c->weno_order == 5) {
-                            const int kind = (v < g.nf || v >= g.advxb ? 1 : 0) | (v >= g.advxb ? 2 : 0) |
-                                             (v == g.E_idx ? 4 : 0);
-                            orc_weno5_learned_cell(c->weno_eps, cf, st, kind, &vL[v * ns + (j + 1)],
-                                                   &vR[v * ns + (j + 1)]);
-                        } else
-                            orc_weno_cell(c->weno_order, c->weno_eps, cf, st, &vL[v * ns + (j + 1)],
-                                          &vR[v * ns + (j + 1)]);
-                    }
-                }
-                /* Riemann-state buffers for bc = -4 (m_riemann_solvers.fpp:480-550) */
-                if (c->bc_beg[dir] == -4)
-                    for (v = 0; v < g.S; v++) vR[v * ns + 0] = vL[v * ns + 1];
-                if (c->bc_end[dir] == -4)
-                    for (v = 0; v < g.S; v++) vL[v * ns + (s + 2)] = vR[v * ns + (s + 1)];
-                /* HLLC at faces j+1/2, j in [-1, s]; the caller passes the
-                 * right-face value of cell j as the solver's left state
-                 * (m_rhs.fpp:545-556). */
-                for (j = -1; j <= s; j++) {
-                    double ql[3 * ORC_NUM_FLUIDS_MAX + 4], qr[3 * ORC_NUM_FLUIDS_MAX + 4], f[3 * ORC_NUM_FLUIDS_MAX + 4];
-                    for (v = 0; v < g.S; v++) {
-                        ql[v] = vR[v * ns + (j + 1)];
-                        qr[v] = vL[v * ns + (j + 2)];
-                    }
-                    orc_hllc_face(c, dir, ql, qr, f, &ustar[j + 1]);
-                    for (v = 0; v < g.S; v++) flux[v * (s + 2) + (j + 1)] = f[v];
-                }
-                /* flux divergence and alpha*div(u) (m_rhs.fpp:565-652) */
-                for (v = 0; v < g.S; v++)
-                    for (j = 0; j <= s; j++) {
-                        int I, J, K;
-                        long r;
-                        POS(j, I, J, K);
-                        r = (long)v * (g.m + 1) * (g.n + 1) * (g.p + 1) + ridx(&g, I, J, K);
-                        if (dir == 0)
-                            rhs[r] = 1.0 / ds[dir][j + g.b] * (flux[v * (s + 2) + j] - flux[v * (s + 2) + j + 1]);
-                        else
-                            rhs[r] = rhs[r] + 1.0 / ds[dir][j + g.b] * (flux[v * (s + 2) + j] - flux[v * (s + 2) + j + 1]);
-                    }
-                for (v = g.advxb; v < g.advxb + g.nf; v++)
-                    for (j = 0; j <= s; j++) {
-                        int I, J, K;
-                        long r;
-                        POS(j, I, J, K);
-                        r = (long)v * (g.m + 1) * (g.n + 1) * (g.p + 1) + ridx(&g, I, J, K);
-                        rhs[r] = rhs[r] + 1.0 / ds[dir][j + g.b] * qc[v * g.ng + gidx(&g, I, J, K)] *
-                                              (ustar[j + 1] - ustar[j]);
-                    }
-#undef POS
-            }
-        free(coef); free(vL); free(vR); free(flux); free(ustar);
+        /* Pencils are independent (each writes its own rhs cells), so they
+         * may run on several threads; the arithmetic of each is unchanged. */
+#pragma omp parallel for schedule(dynamic, 4)
+        for (pc = 0; pc < npencil; pc++)
+            pencil(c, &g, dir, s, (int)(pc % (t1 + 1)), (int)(pc / (t1 + 1)), coef, qc, qp, ds[dir], rhs);
+        free(coef);
     }
     if (q_prim) memcpy(q_prim, qp, sizeof(double) * g.S * g.ng); /* :659-675 */
     free(qc);
@@ -670,7 +697,7 @@ int orc_tvd_rk_step(const orc_case *c, const double *const cb[3], const double *
     int v, i, j, k, rc;
     mkgeom(c, &g);
     const long nr = (long)(g.m + 1) * (g.n + 1) * (g.p + 1);
-#define LOOP for (v = 0; v < g.S; v++) for (k = 0; k <= g.p; k++) for (j = 0; j <= g.n; j++) for (i = 0; i <= g.m; i++)
+#define LOOP _Pragma("omp parallel for private(k, j, i)") for (v = 0; v < g.S; v++) for (k = 0; k <= g.p; k++) for (j = 0; j <= g.n; j++) for (i = 0; i <= g.m; i++)
 #define Q(a) a[v * g.ng + gidx(&g, i, j, k)]
 #define R rhs[v * nr + ridx(&g, i, j, k)]
     rc = orc_compute_rhs(c, cb, ds, q1, q_prim, rhs, t_step);

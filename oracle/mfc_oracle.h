@@ -44,6 +44,11 @@ typedef struct {
     int bc_beg[3], bc_end[3];    /* bc_x%beg, bc_x%end, bc_y..., bc_z... */
 } orc_case;
 
+/* Host threads of the pencil and update loops (OpenMP builds; the results do
+ * not depend on the count) and the current count. */
+void orc_set_threads(int n);
+int orc_max_threads(void);
+
 /* Derived sizes (m_global_parameters.fpp:285-381). */
 int orc_num_dims(const orc_case *c);
 int orc_sys_size(const orc_case *c);
