@@ -165,6 +165,9 @@ int check_params(const mfc_params &p) {
     if (p.precision == MFC_PREC_FP64 && p.weno_order > 1 && !(p.weno_eps >= 1e-100))
         return fail(-1, "weno_eps below 1e-100 is not supported in FP64 (the weight division is exact for "
                         "weno_eps >= 1e-100)");
+    // In FP32, beta^2 >= weno_eps^2 must stay a normal float and d/beta^2 finite.
+    if (p.precision != MFC_PREC_FP64 && p.weno_order > 1 && !(p.weno_eps >= 1e-18))
+        return fail(-1, "weno_eps below 1e-18 is not supported in the FP32 forms (its square underflows)");
     return 0;
 }
 

@@ -54,6 +54,10 @@ def _init(**kw):
     (dict(pi_infs=[-1.0]), "pi_inf"),
     (dict(time_stepper=4), "time_stepper"),
     (dict(n=0, p=5), "p > 0"),
+    (dict(weno_eps=1e-120), "weno_eps below 1e-100"),
+    (dict(precision=2, weno_order=3), "weno_order = 5"),
+    (dict(precision=7), "precision"),
+    (dict(precision=1, weno_eps=1e-20), "weno_eps below 1e-18"),
 ])
 def test_invalid_cases_are_rejected(kw, msg):
     rc, err = _init(**kw)
