@@ -536,7 +536,7 @@ SweepArgs sweep_args(int d) {
     } else {
         const int s = G.dir[d].s;
         a.ntile = (G.prm.m + 1 + kTX - 1) / kTX;
-        a.chunk = kTS - 1;
+        a.chunk = (G.prm.precision == MFC_PREC_FP32_LEARNED ? t_faces<true>() : t_faces<false>()) - 1;
         a.nchunk = (s + 1 + a.chunk - 1) / a.chunk;
         const int other = d == 1 ? G.prm.p + 1 : G.prm.n + 1;
         a.nblocks = (long)a.ntile * a.nchunk * other;
