@@ -87,6 +87,40 @@ def perturbed(prm, q, seed=1, amp=1e-3):
     return q
 
 
+def stretched_grids(prm: Params, amps=(0.5, 0.3, 0.4)):
+    """Non-uniform grids: s_cb(xi) = xi + a sin(2 pi xi) / (2 pi) on [0, 1]
+    (cell widths varying by a factor (1+a)/(1-a)), ghosted by the case's BCs."""
+    from microfc_amd.rhs import populate_grid_buffers, uniform_cell_boundaries
+    grids = []
+    for d, s in enumerate(prm.extents()):
+        xi = uniform_cell_boundaries(s, 0.0, 1.0)
+        cb = xi + amps[d] * np.sin(2.0 * np.pi * xi) / (2.0 * np.pi)
+        grids.append(populate_grid_buffers(cb, prm.buff_size, prm.bc_beg[d], prm.bc_end[d]))
+    return grids
+
+
+def multi_fluid_2d(nf: int, N: int = 40, seed: int = 3):
+    """nf stiffened-gas fluids in smooth random layers with a pressure pulse:
+    every partial density, volume fraction and velocity component varies."""
+    rng = np.random.default_rng(seed)
+    gam = rng.uniform(1.3, 4.4, nf)
+    pinf = np.where(np.arange(nf) % 2 == 0, 0.0, rng.uniform(1e3, 1e5, nf))
+    prm = Params(m=N - 1, n=N - 1, num_fluids=nf, weno_order=5, gammas=list(1.0 / (gam - 1.0)),
+                 pi_infs=list(gam * pinf / (gam - 1.0)), bc_beg=(-3, -1, -3), bc_end=(-3, -1, -3))
+    grids = ic.uniform_grids(prm)
+    X, Y, _ = ic.mesh(prm, grids)
+    w = np.stack([1.0 + 0.5 * np.sin(2 * np.pi * (k + 1) * X + rng.uniform(0, 6)) *
+                  np.cos(2 * np.pi * Y + rng.uniform(0, 6)) for k in range(nf)])
+    alpha = w / w.sum(axis=0)
+    prim = np.zeros(prm.interior_shape)
+    prim[:nf] = alpha * rng.uniform(1.0, 1000.0, nf)[:, None, None, None]
+    prim[nf] = 10.0 * np.sin(2 * np.pi * Y)
+    prim[nf + 1] = 5.0 * np.cos(2 * np.pi * X)
+    prim[nf + 2] = 1e5 * (1.0 + 0.5 * np.exp(-((X - 0.4) ** 2 + (Y - 0.6) ** 2) / 0.01))
+    prim[nf + 3:] = alpha
+    return prm, grids, ic.ghosted(prm, ic.prim_to_cons(prm, prim))
+
+
 def small_cases():
     """(name, params, grids, q) for the parity matrix; sizes keep the oracle
     to well under a second per RHS."""
@@ -115,6 +149,13 @@ def small_cases():
     out.append(("shear2d_64_periodic", prm, g, perturbed(prm, q)))
     prm, g, q = ic.shock_droplet_3d(N=28)
     out.append(("droplet3d_28", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shock_bubble_2d(N=48)
+    out.append(("bubble2d_stretched", prm, stretched_grids(prm), perturbed(prm, q)))
+    prm, g, q = ic.shock_droplet_3d(N=28)
+    out.append(("droplet3d_stretched", prm, stretched_grids(prm), perturbed(prm, q)))
+    for nf in (3, 4):
+        prm, g, q = multi_fluid_2d(nf)
+        out.append((f"fluids{nf}_2d", prm, g, q))
     return out
 
 
