@@ -17,12 +17,14 @@ import pytest
 from microfc_amd import PREC_FP32_BULK, RhsModule, ic
 from oracle import oracle as O
 
-from helpers import perturbed, say, scaled_linf, to_oracle
+from helpers import perturbed, say, scaled_linf, small_cases, to_oracle
 
 pytestmark = pytest.mark.gpu
 
 
 def _cases():
+    """The benchmark-like cases, plus the FP64 parity matrix (WENO1/3,
+    RK1/2, every BC code, stretched grids, three and four fluids)."""
     out = []
     prm, g, q = ic.shock_bubble_2d(N=64)
     out.append(("bubble2d_64", prm, g, perturbed(prm, q)))
@@ -32,6 +34,8 @@ def _cases():
     out.append(("sod1d_200", prm, g, perturbed(prm, q)))
     prm, g, q = ic.shock_droplet_3d(N=32)
     out.append(("droplet3d_32", prm, g, perturbed(prm, q)))
+    names = {c[0] for c in out}
+    out += [c for c in small_cases() if c[0] not in names]
     return out
 
 
