@@ -386,7 +386,7 @@ __global__ void k_ml_samples(int dir, int m, int n, int b, const double *coarse,
     const int i = (int)(c % (m + 1)), jj = (int)(c / (m + 1));
     const int bj = n > 0 ? b : 0;
     const long cnx = m + 2 * b + 1;
-    const int fm = 2 * m + 1, fn = n > 0 ? 2 * n + 1 : 0;
+    const int fm = 2 * m + 1;
     const long fnx = fm + 2 * b + 1;
     auto C = [&](int x, int y) { return coarse[(long)(x + b) + cnx * (long)(y + bj)]; };
     auto Fv = [&](int x, int y) { return fine[(long)(x + b) + fnx * (long)(y + bj)]; };

@@ -99,13 +99,13 @@ struct TimedScope {
     int id;
     hipEvent_t a = nullptr;
     explicit TimedScope(int k) : id(k) {
-        if (T.on && (a = T.get())) hipEventRecord(a, G.stream);
+        if (T.on && (a = T.get())) (void)hipEventRecord(a, G.stream);
     }
     ~TimedScope() {
         if (!a) return;
         hipEvent_t b = T.get();
         if (!b) return;
-        hipEventRecord(b, G.stream);
+        (void)hipEventRecord(b, G.stream);
         T.pending.push_back({id, a, b});
     }
 };
@@ -276,7 +276,7 @@ __global__ void k_bc_fill(Fields q, int S, int momxb, Box x, int bxb, int bxe, i
     const int slab = blockIdx.y;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
-    const int gx = 2 * x.b, gy = 2 * x.bj, gz = 2 * x.bk;
+    const int gx = 2 * x.b, gy = 2 * x.bj;
     const int NY = x.n + 1 + gy, NX = x.m + 1;
     if (slab == 0) {
         if (t >= n0) return;
@@ -834,19 +834,20 @@ int mfc_set_stream(void *stream) {
 
 int mfc_rhs_finalize(void) {
     if (!G.init) return 0;
-    hipStreamSynchronize(G.stream);
+    // best-effort teardown: errors here leave nothing to recover
+    (void)hipStreamSynchronize(G.stream);
     timing_flush();
-    for (auto e : T.pool) hipEventDestroy(e);
+    for (auto e : T.pool) (void)hipEventDestroy(e);
     T = Timing();
     for (int d = 0; d < 3; ++d) {
-        if (G.dir[d].coef) hipFree(G.dir[d].coef);
-        if (G.dir[d].inv_ds) hipFree(G.dir[d].inv_ds);
-        if (G.dir[d].ds_dev) hipFree(G.dir[d].ds_dev);
+        if (G.dir[d].coef) (void)hipFree(G.dir[d].coef);
+        if (G.dir[d].inv_ds) (void)hipFree(G.dir[d].inv_ds);
+        if (G.dir[d].ds_dev) (void)hipFree(G.dir[d].ds_dev);
         G.dir[d] = Dir();
     }
-    if (G.partials) hipFree(G.partials);
-    if (G.scratch_state) hipFree(G.scratch_state);
-    if (G.own) hipStreamDestroy(G.own);
+    if (G.partials) (void)hipFree(G.partials);
+    if (G.scratch_state) (void)hipFree(G.scratch_state);
+    if (G.own) (void)hipStreamDestroy(G.own);
     G = State();
     return 0;
 }
