@@ -73,3 +73,15 @@ def test_padded_parameters_get_zero_gradient():
     g, _ = Q.gradient(P, S, bf16_points=True)
     W1, b1, W2, b2, W3, b3 = Q.unpack(g)
     assert np.all(W1[5:] == 0) and np.all(W3[:, 3:] == 0) and np.all(b3[3:] == 0)
+
+
+def test_keep_selects_valid_in_hull_targets():
+    """Training pairs are kept when the stencil's variation is real (flag in
+    row 12) and the fine-grid target lies inside the candidates' hull."""
+    import torch
+    from microfc_amd import learned
+    S = torch.zeros((13, 5))
+    S[5:8] = torch.tensor([[-1.0] * 5, [0.0] * 5, [1.0] * 5])
+    S[11] = torch.tensor([0.5, -1.0, 1.5, 0.0, 0.2])
+    S[12] = torch.tensor([1.0, 1.0, 1.0, 1.0, 0.0])
+    assert learned.keep(S).tolist() == [True, True, False, True, False]
