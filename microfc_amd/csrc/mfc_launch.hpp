@@ -1,5 +1,5 @@
 // mfc_launch.hpp -- dispatch of the sweep kernels over (num_dims, weno_order,
-// precision) for one number of fluids.  Each mfc_sweep_nf<N>.hip instantiates
+// precision, uniform-grid coefficients) for one number of fluids.  Each mfc_sweep_nf<N>.hip instantiates
 // launch_sweep<N> so the kernel variants compile in parallel.
 #pragma once
 
@@ -18,23 +18,25 @@ template <int NF>
 int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
 
 #define MFC_SWEEP_LAUNCH_IMPL(NF)                                                                       \
-    template <int D, int WO, typename R, bool ML>                                                       \
+    template <int D, int WO, typename R, bool ML, bool UNI>                                             \
     static int launch_nf##NF##_dwrm(int dir, const SweepArgs &a, hipStream_t st) {                      \
         const dim3 grid((unsigned)a.nblocks);                                                           \
         if (dir == 0) {                                                                                 \
-            hipLaunchKernelGGL((k_sweep_x<NF, D, WO, R, ML>), grid, dim3(64), 0, st, a);                \
+            hipLaunchKernelGGL((k_sweep_x<NF, D, WO, R, ML, UNI>), grid, dim3(64), 0, st, a);                \
         } else if (dir == 1) {                                                                          \
-            if constexpr (D >= 2) hipLaunchKernelGGL((k_sweep_t<NF, D, 1, WO, R, ML>), grid, dim3(t_threads<ML>()), 0, st, a); \
+            if constexpr (D >= 2) hipLaunchKernelGGL((k_sweep_t<NF, D, 1, WO, R, ML, UNI>), grid, dim3(t_threads<ML>()), 0, st, a); \
             else return -1;                                                                             \
         } else {                                                                                        \
-            if constexpr (D >= 3) hipLaunchKernelGGL((k_sweep_t<NF, D, 2, WO, R, ML>), grid, dim3(t_threads<ML>()), 0, st, a); \
+            if constexpr (D >= 3) hipLaunchKernelGGL((k_sweep_t<NF, D, 2, WO, R, ML, UNI>), grid, dim3(t_threads<ML>()), 0, st, a); \
             else return -1;                                                                             \
         }                                                                                               \
         return hipGetLastError() == hipSuccess ? 0 : -2;                                                \
     }                                                                                                   \
     template <int D, int WO, typename R>                                                                \
     static int launch_nf##NF##_dwr(int dir, const SweepArgs &a, hipStream_t st) {                       \
-        return launch_nf##NF##_dwrm<D, WO, R, false>(dir, a, st);                                       \
+        if constexpr (WO == 5)                                                                          \
+            if (a.uniform) return launch_nf##NF##_dwrm<D, 5, R, false, true>(dir, a, st);               \
+        return launch_nf##NF##_dwrm<D, WO, R, false, false>(dir, a, st);                                \
     }                                                                                                   \
     template <int D, typename R>                                                                        \
     static int launch_nf##NF##_dr(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {           \
@@ -59,9 +61,9 @@ int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
         if (L.precision == 2) {                                                                         \
             if (L.weno_order != 5) return -1;                                                           \
             switch (L.num_dims) {                                                                       \
-            case 1: return launch_nf##NF##_dwrm<1, 5, float, true>(L.dir, a, st);                       \
-            case 2: return launch_nf##NF##_dwrm<2, 5, float, true>(L.dir, a, st);                       \
-            case 3: return launch_nf##NF##_dwrm<3, 5, float, true>(L.dir, a, st);                       \
+            case 1: return launch_nf##NF##_dwrm<1, 5, float, true, false>(L.dir, a, st);                       \
+            case 2: return launch_nf##NF##_dwrm<2, 5, float, true, false>(L.dir, a, st);                       \
+            case 3: return launch_nf##NF##_dwrm<3, 5, float, true, false>(L.dir, a, st);                       \
             default: return -1;                                                                         \
             }                                                                                           \
         }                                                                                               \

@@ -54,6 +54,8 @@ struct Dir {
     double *ds_dev = nullptr; // device, ds(0..s)
     std::vector<double> coef_h, ds_h;
     bool ready = false;
+    bool uniform = false;  // every reconstructed cell has the same coefficients
+    double ucf[kNCoef] = {};
 };
 
 struct State {
@@ -529,10 +531,18 @@ SweepArgs sweep_args(int d) {
                            {G.prm.bc_z_beg, G.prm.bc_z_end}};
     a.bc_beg = bcs[d][0];
     a.bc_end = bcs[d][1];
+    a.uniform = G.dir[d].uniform ? 1 : 0;
+    for (int q = 0; q < kNCoef; ++q) {
+        a.ucf[q] = G.dir[d].ucf[q];
+        a.ucf32[q] = (float)G.dir[d].ucf[q];
+    }
     if (d == 0) {
-        const int tw = G.prm.precision == MFC_PREC_FP32_LEARNED ? x_tile_cells<true, float>()
-                       : G.prm.precision == MFC_PREC_FP32_BULK  ? x_tile_cells<false, float>()
-                                                                : x_tile_cells<false, double>();
+        // the launcher picks the uniform-coefficient kernels for WENO5 on uniform grids
+        const bool uni = G.prm.weno_order == 5 && G.dir[0].uniform;
+        const int tw = G.prm.precision == MFC_PREC_FP32_LEARNED ? x_tile_cells<true, float, false>()
+                       : G.prm.precision == MFC_PREC_FP32_BULK
+                           ? (uni ? x_tile_cells<false, float, true>() : x_tile_cells<false, float, false>())
+                           : (uni ? x_tile_cells<false, double, true>() : x_tile_cells<false, double, false>());
         a.ntile = (G.prm.m + 1 + tw - 1) / tw;
         a.nblocks = (long)a.ntile * (G.prm.n + 1) * (G.prm.p + 1);
     } else {
@@ -628,6 +638,17 @@ int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds) {
     for (int i = 0; i <= s; ++i)
         if (!(ds[i + b] > 0.0)) return fail(-1, "non-positive cell width at index %d of direction %d", i, dir);
     weno_tables(s, b, G.prm.weno_order, s_cb, D.coef_h);
+    // Uniform coefficients over the cells the sweeps reconstruct (-1..s+1)?
+    {
+        const long nc = s + 2 * b + 1;
+        D.uniform = true;
+        for (int q = 0; q < kNCoef && D.uniform; ++q) {
+            const double v0 = D.coef_h[(long)q * nc + (-1 + b)];
+            D.ucf[q] = v0;
+            for (int j = 0; j <= s + 1 && D.uniform; ++j)
+                D.uniform = std::memcmp(&v0, &D.coef_h[(long)q * nc + (j + b)], sizeof(double)) == 0;
+        }
+    }
     std::vector<double> inv(s + 1);
     for (int i = 0; i <= s; ++i) inv[i] = 1.0 / ds[i + b];
     D.ds_h.assign(ds, ds + s + 2 * b + 1);

@@ -57,6 +57,12 @@ struct SweepArgs {
     int ntile;            // x tiles (k_sweep_x) or x column groups (k_sweep_t)
     long nblocks;
     const MlpFrags *mlp;  // learned WENO5 weights (MFC_PREC_FP32_LEARNED)
+    // Uniform grids: when every cell the sweep reconstructs has the same 27
+    // coefficients (bit for bit), they travel here as kernel arguments (read
+    // into scalar registers) instead of being loaded per cell.
+    int uniform;
+    double ucf[kNCoef];
+    float ucf32[kNCoef];
 };
 
 __device__ __forceinline__ long ghost_off(const SweepArgs &a, int i, int j, int k) {
@@ -228,10 +234,13 @@ __device__ __forceinline__ void load_cons_prim(const SweepArgs &a, long go, doub
     cons_to_prim<NF, D>(c, p, a.eos);
 }
 
-template <typename R, int WO>
+template <typename R, int WO, bool UNI>
 __device__ __forceinline__ void load_coef(const SweepArgs &a, int cell, R *cf) {
-    const double *t = a.coef + (long)(cell + a.b);
-    if constexpr (WO > 1) {
+    if constexpr (WO > 1 && UNI) {
+#pragma unroll
+        for (int q = 0; q < kNCoef; ++q) cf[q] = sizeof(R) == 8 ? (R)a.ucf[q] : (R)a.ucf32[q];
+    } else if constexpr (WO > 1) {
+        const double *t = a.coef + (long)(cell + a.b);
 #pragma unroll
         for (int q = 0; q < kNCoef; ++q) cf[q] = (R)t[(long)q * a.ncell];
     }
@@ -328,26 +337,29 @@ __device__ __forceinline__ void ml_face_states(int var, const float *stR, const 
 }
 
 // ---------------------------------------------------------------------------
-// x sweep: one wave per tile of a row.  Face-parallel (FP64 and the learned
-// form): lane L reconstructs the two sides of face x0-1+L + 1/2 from the
-// stencils of its two cells; 63 cells per tile are finished.  Cell-parallel
-// (FP32 bulk): lane L reconstructs both faces of cell x0-1+L (the smoothness
-// indicators serve both), the left-face values move one lane down through
-// LDS, and lane L solves face x0-1+L + 1/2; 62 cells per tile.  Measured on
-// MI355X, cell-parallel is 6 % faster in FP32 and no faster in FP64, where
-// its larger register footprint costs a wave of occupancy.
-template <bool ML, typename R>
-constexpr bool x_cell_parallel() { return !ML && sizeof(R) == 4; }
-template <bool ML, typename R>
-constexpr int x_tile_cells() { return x_cell_parallel<ML, R>() ? 62 : 63; }
+// x sweep: one wave per tile of a row.  Face-parallel (the learned form, and
+// FP64 on non-uniform grids): lane L reconstructs the two sides of face
+// x0-1+L + 1/2 from the stencils of its two cells; 63 cells per tile are
+// finished.  Cell-parallel (FP32 bulk; FP64 with the uniform-grid
+// coefficients in scalar registers): lane L reconstructs both faces of cell
+// x0-1+L (the smoothness indicators and the reciprocals of their squares
+// serve both), the left-face values move one lane down through LDS, and
+// lane L solves face x0-1+L + 1/2; 62 cells per tile.  Measured on MI355X:
+// cell-parallel is 6 % faster in FP32 and with uniform FP64 coefficients,
+// and no faster for FP64 with per-cell coefficient loads, whose registers
+// then cost a wave of occupancy.
+template <bool ML, typename R, bool UNI>
+constexpr bool x_cell_parallel() { return !ML && (sizeof(R) == 4 || UNI); }
+template <bool ML, typename R, bool UNI>
+constexpr int x_tile_cells() { return x_cell_parallel<ML, R, UNI>() ? 62 : 63; }
 
-template <int NF, int D, int WO, typename R, bool ML>
+template <int NF, int D, int WO, typename R, bool ML, bool UNI>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 8))) void k_sweep_x(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int PN = (WO - 1) / 2;
     constexpr int NC = 64 + 2 * PN + 1;
-    constexpr int TW = x_tile_cells<ML, R>();
+    constexpr int TW = x_tile_cells<ML, R, UNI>();
     __shared__ R sp[S][NC];
     __shared__ R sf[S + 1][64];
 
@@ -411,8 +423,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
         MlpTRegs T;
         mlp_load_tregs(a.mlp, lane, T);
         R cfR[kNCoef], cfL[kNCoef];
-        load_coef<R, WO>(a, min(jf, a.m), cfR);
-        load_coef<R, WO>(a, min(jf, a.m) + 1, cfL);
+        load_coef<R, WO, UNI>(a, min(jf, a.m), cfR);
+        load_coef<R, WO, UNI>(a, min(jf, a.m) + 1, cfL);
         const R eps = (R)a.eps;
 #pragma unroll 1
         for (int v = 0; v < S; ++v) {
@@ -431,13 +443,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
             qL[v] = mres[0][v][lane];
             qR[v] = mres[1][v][lane];
         }
-    } else if constexpr (x_cell_parallel<ML, R>()) {
+    } else if constexpr (x_cell_parallel<ML, R, UNI>()) {
         // both reconstructions of cell jf (coefficients of a cell past the
         // last face are clamped; those lanes' values are not used), one
         // variable at a time through LDS to bound the live registers
         __shared__ R sl[S][64], sr[S][64];
         R cf[kNCoef];
-        load_coef<R, WO>(a, min(jf, a.m + 1), cf);
+        load_coef<R, WO, UNI>(a, min(jf, a.m + 1), cf);
         const R eps = (R)a.eps;
 #pragma unroll 1
         for (int v = 0; v < S; ++v) {
@@ -456,8 +468,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
         }
     } else if (jf <= a.m) {
         R cfR[kNCoef], cfL[kNCoef];
-        load_coef<R, WO>(a, jf, cfR);
-        load_coef<R, WO>(a, jf + 1, cfL);
+        load_coef<R, WO, UNI>(a, jf, cfR);
+        load_coef<R, WO, UNI>(a, jf + 1, cfL);
         const R eps = (R)a.eps;
 #pragma unroll
         for (int v = 0; v < S; ++v) {
@@ -517,7 +529,7 @@ constexpr int t_faces() { return 16; }
 template <bool ML>
 constexpr int t_threads() { return kTX * t_faces<ML>(); }
 
-template <int NF, int D, int DIR, int WO, typename R, bool ML>
+template <int NF, int D, int DIR, int WO, typename R, bool ML, bool UNI>
 __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 8))) void k_sweep_t(
     const SweepArgs a) {
     using Y = Sys<NF, D>;
@@ -629,8 +641,8 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
         MlpTRegs T;
         mlp_load_tregs(a.mlp, lane, T);
         R cfR[kNCoef], cfL[kNCoef];
-        load_coef<R, WO>(a, min(jf, s), cfR);
-        load_coef<R, WO>(a, min(jf, s) + 1, cfL);
+        load_coef<R, WO, UNI>(a, min(jf, s), cfR);
+        load_coef<R, WO, UNI>(a, min(jf, s) + 1, cfL);
         const R eps = (R)a.eps;
 #pragma unroll 1
         for (int v = 0; v < S; ++v) {
@@ -653,8 +665,8 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
     if (face_ok) {
         if constexpr (!ML) {
             R cfR[kNCoef], cfL[kNCoef];
-            load_coef<R, WO>(a, jf, cfR);
-            load_coef<R, WO>(a, jf + 1, cfL);
+            load_coef<R, WO, UNI>(a, jf, cfR);
+            load_coef<R, WO, UNI>(a, jf + 1, cfL);
             const R eps = (R)a.eps;
 #pragma unroll
             for (int v = 0; v < S; ++v) {
