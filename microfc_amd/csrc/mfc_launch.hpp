@@ -61,9 +61,9 @@ int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
         if (L.precision == 2) {                                                                         \
             if (L.weno_order != 5) return -1;                                                           \
             switch (L.num_dims) {                                                                       \
-            case 1: return launch_nf##NF##_dwrm<1, 5, float, true, false>(L.dir, a, st);                       \
-            case 2: return launch_nf##NF##_dwrm<2, 5, float, true, false>(L.dir, a, st);                       \
-            case 3: return launch_nf##NF##_dwrm<3, 5, float, true, false>(L.dir, a, st);                       \
+            case 1: return a.uniform ? launch_nf##NF##_dwrm<1, 5, float, true, true>(L.dir, a, st) : launch_nf##NF##_dwrm<1, 5, float, true, false>(L.dir, a, st);                       \
+            case 2: return a.uniform ? launch_nf##NF##_dwrm<2, 5, float, true, true>(L.dir, a, st) : launch_nf##NF##_dwrm<2, 5, float, true, false>(L.dir, a, st);                       \
+            case 3: return a.uniform ? launch_nf##NF##_dwrm<3, 5, float, true, true>(L.dir, a, st) : launch_nf##NF##_dwrm<3, 5, float, true, false>(L.dir, a, st);                       \
             default: return -1;                                                                         \
             }                                                                                           \
         }                                                                                               \

@@ -539,7 +539,7 @@ SweepArgs sweep_args(int d) {
     if (d == 0) {
         // the launcher picks the uniform-coefficient kernels for WENO5 on uniform grids
         const bool uni = G.prm.weno_order == 5 && G.dir[0].uniform;
-        const int tw = G.prm.precision == MFC_PREC_FP32_LEARNED ? x_tile_cells<true, float, false>()
+        const int tw = G.prm.precision == MFC_PREC_FP32_LEARNED ? x_tile_cells<true, float, true>()
                        : G.prm.precision == MFC_PREC_FP32_BULK
                            ? (uni ? x_tile_cells<false, float, true>() : x_tile_cells<false, float, false>())
                            : (uni ? x_tile_cells<false, double, true>() : x_tile_cells<false, double, false>());

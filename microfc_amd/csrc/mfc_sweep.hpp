@@ -530,7 +530,7 @@ template <bool ML>
 constexpr int t_threads() { return kTX * t_faces<ML>(); }
 
 template <int NF, int D, int DIR, int WO, typename R, bool ML, bool UNI>
-__global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 8))) void k_sweep_t(
+__global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_t(
     const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
@@ -543,8 +543,9 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
     constexpr int LDS_BYTES = sizeof(R) * (PRIM_WORDS > FLUX_WORDS ? PRIM_WORDS : FLUX_WORDS);
     // The conservative stage input of the tile's own cells stays in LDS for
     // the finish (rhs accumulation, RK stage) instead of being read from HBM
-    // a second time, while the block still fits four to a CU.
-    constexpr bool KEEP = LDS_BYTES + 8 * S * (kTS - 1) * kTX <= 40 * 1024 * (NT / 256);
+    // a second time, while the block still fits four to a CU (not in the
+    // learned form, whose MLP staging takes that LDS).
+    constexpr bool KEEP = !ML && LDS_BYTES + 8 * S * (kTS - 1) * kTX <= 40 * 1024 * (NT / 256);
     __shared__ R lds[PRIM_WORDS > FLUX_WORDS ? PRIM_WORDS : FLUX_WORDS];
     __shared__ double lcons[KEEP ? S : 1][kTS - 1][kTX];
     R(*sp)[NR][kTX] = reinterpret_cast<R(*)[NR][kTX]>(lds);
