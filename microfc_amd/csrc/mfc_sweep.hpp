@@ -371,11 +371,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
     const int lane = threadIdx.x;
     const int x0 = tile * TW;
 
-    // 1. stage primitive stencil cells x0-1-PN .. x0+63+PN.  FP32 forms
-    // issue every load before the first conversion waits on one; FP64, with
-    // no registers to spare at 4 waves per SIMD, converts as loads arrive
-    // (measured: hoisting is 17 % faster in FP32, 3 % slower in FP64).
-    if constexpr (sizeof(R) == 8) {
+    // 1. stage primitive stencil cells x0-1-PN .. x0+63+PN.  FP32 forms and
+    // uniform-grid FP64 issue every load before the first conversion waits
+    // on one; FP64 with per-cell coefficient loads, with no registers to
+    // spare at 4 waves per SIMD, converts as loads arrive (measured: hoisting
+    // is 17 % faster in FP32, 2-3 % faster in uniform FP64, 3 % slower in
+    // FP64 with per-cell coefficients).
+    if constexpr (sizeof(R) == 8 && !UNI) {
         for (int c = lane; c < NC; c += 64) {
             const int x = x0 - 1 - PN + c;
             double pr[S];
@@ -574,9 +576,9 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
     };
 
     // 1. stage primitive rows y0-1-PN .. y0+kTS-1+PN of columns x0..x0+kTX-1
-    // (FP32 forms issue every load before the first conversion; FP64 converts
-    // as loads arrive, as in the x sweep)
-    if constexpr (sizeof(R) == 8) {
+    // (as in the x sweep: every load issued before the first conversion,
+    // except FP64 with per-cell coefficient loads)
+    if constexpr (sizeof(R) == 8 && !UNI) {
         for (int e = tid; e < NR * kTX; e += NT) {
             const int ex = e % kTX, er = e / kTX;
             const int c = y0 - 1 - PN + er;
