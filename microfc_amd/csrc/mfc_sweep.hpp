@@ -419,9 +419,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
     R qL[S], qR[S];
     if constexpr (ML) {
         // every lane takes part in the wave's MFMA passes, valid face or not;
-        // one variable at a time, results through LDS
+        // one variable at a time, results through LDS: the left states over
+        // the variable's own staged row (the block is one wave, and each
+        // value is written after the reads it depends on), the right states
+        // over the flux rows, which each lane reads back before writing
         __shared__ unsigned short xs[2][64][8];
-        __shared__ R mres[2][S][64];
         MlpTRegs T;
         mlp_load_tregs(a.mlp, lane, T);
         R cfR[kNCoef], cfL[kNCoef];
@@ -437,19 +439,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
                 stL[w] = sp[v][lane + 1 + w];
             }
             ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, T, xs, lane, l, r);
-            mres[0][v][lane] = l;
-            mres[1][v][lane] = r;
+            sp[v][lane] = l;
+            sf[v][lane] = r;
         }
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            qL[v] = mres[0][v][lane];
-            qR[v] = mres[1][v][lane];
+            qL[v] = sp[v][lane];
+            qR[v] = sf[v][lane];
         }
     } else if constexpr (x_cell_parallel<ML, R, UNI>()) {
         // both reconstructions of cell jf (coefficients of a cell past the
         // last face are clamped; those lanes' values are not used), one
-        // variable at a time through LDS to bound the live registers
-        __shared__ R sl[S][64], sr[S][64];
+        // variable at a time through LDS to bound the live registers, placed
+        // as in the learned path above: no LDS beyond the staging and flux
+        // rows, so LDS does not cap the occupancy (measured: 15 KB per wave
+        // held the uniform FP64 x sweep to 3 waves per SIMD).
+        R(*const sl)[NC] = sp;
+        R(*const sr)[64] = sf;
         R cf[kNCoef];
         load_coef<R, WO, UNI>(a, min(jf, a.m + 1), cf);
         const R eps = (R)a.eps;
