@@ -65,7 +65,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--case", default="shock_bubble_2d",
-                    choices=["shock_bubble_2d", "shock_droplet_3d", "shear_layer_2d", "sod_1d"])
+                    choices=["shock_bubble_2d", "shock_droplet_3d", "shear_layer_2d", "bubble_cloud_3d", "sod_1d"])
     ap.add_argument("--N", type=int, default=0, help="cells per direction (default: the case's size)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_bulk", "fp32_learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -101,6 +101,10 @@ def main():
         N = args.N or 256
         prm, grids, q = ic.shock_droplet_3d(N=N, precision=prec)
         workload = f"3D shock-droplet water/air {N}^3, WENO5+HLLC+RK3"
+    elif args.case == "bubble_cloud_3d":
+        N = args.N or 768
+        prm, grids, q = ic.bubble_cloud_3d(N=N, precision=prec)
+        workload = f"3D shock-bubble cloud water/air {N}^3 (64 air bubbles), WENO5+HLLC+RK3"
     elif args.case == "shear_layer_2d":
         N = args.N or 4096
         prm, grids, q = ic.shear_layer_2d(N=N, precision=prec)

@@ -269,9 +269,14 @@ def run(prm: Params, grids, q, t_end: float, cfl: float = 0.4, dt: float | None 
 
 
 def restrict_host(fine_interior):
-    """2x cell averages of an interior field (S, 1, 2n, 2m) -> (S, 1, n, m)."""
+    """2x cell averages of an interior field: (S, 1, 2n, 2m) -> (S, 1, n, m)
+    in 2D, (S, 2p, 2n, 2m) -> (S, p, n, m) in 3D."""
     f = np.asarray(fine_interior)
-    return 0.25 * (f[:, :, 0::2, 0::2] + f[:, :, 0::2, 1::2] + f[:, :, 1::2, 0::2] + f[:, :, 1::2, 1::2])
+    if f.shape[1] == 1:
+        return 0.25 * (f[:, :, 0::2, 0::2] + f[:, :, 0::2, 1::2] + f[:, :, 1::2, 0::2] + f[:, :, 1::2, 1::2])
+    return 0.125 * (f[:, 0::2, 0::2, 0::2] + f[:, 0::2, 0::2, 1::2] + f[:, 0::2, 1::2, 0::2]
+                     + f[:, 0::2, 1::2, 1::2] + f[:, 1::2, 0::2, 0::2] + f[:, 1::2, 0::2, 1::2]
+                     + f[:, 1::2, 1::2, 0::2] + f[:, 1::2, 1::2, 1::2])
 
 
 def l1_error(a, b, var: int = 0):

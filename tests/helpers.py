@@ -153,6 +153,8 @@ def small_cases():
     out.append(("bubble2d_stretched", prm, stretched_grids(prm), perturbed(prm, q)))
     prm, g, q = ic.shock_droplet_3d(N=28)
     out.append(("droplet3d_stretched", prm, stretched_grids(prm), perturbed(prm, q)))
+    prm, g, q = ic.bubble_cloud_3d(N=32, nbubbles=16)
+    out.append(("cloud3d_32", prm, g, perturbed(prm, q)))
     for nf in (3, 4):
         prm, g, q = multi_fluid_2d(nf)
         out.append((f"fluids{nf}_2d", prm, g, q))

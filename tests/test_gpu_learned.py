@@ -31,7 +31,7 @@ def _rhs(torch, prm, grids, q):
 
 def _cases():
     return {"bubble": ic.shock_bubble_2d(N=64), "shear": ic.shear_layer_2d(N=64), "sod": ic.sod_1d(m=199),
-            "droplet3d": ic.shock_droplet_3d(N=28)}
+            "droplet3d": ic.shock_droplet_3d(N=28), "cloud3d": ic.bubble_cloud_3d(N=32, nbubbles=16)}
 
 
 def test_learned_rhs_matches_oracle(gpu):

@@ -1,0 +1,45 @@
+"""The 3D bubble-cloud initial condition: bubble placement and the slab-wise
+generator against a whole-domain evaluation of the same definition."""
+import numpy as np
+
+from microfc_amd import ic
+
+
+def test_cloud_centres_are_seeded_and_disjoint():
+    c, r = ic.bubble_cloud_centres(64, seed=0)
+    c2, r2 = ic.bubble_cloud_centres(64, seed=0)
+    assert np.array_equal(c, c2) and np.array_equal(r, r2)
+    assert len(r) == 64 and r.min() >= 0.02 and r.max() <= 0.05
+    assert np.all(c[:, 0] - r >= 0.2) and np.all(c + r[:, None] <= 0.95) and np.all(c[:, 1:] - r[:, None] >= 0.05)
+    d = np.sqrt(((c[:, None, :] - c[None, :, :]) ** 2).sum(-1))
+    gap = d - r[:, None] - r[None, :]
+    np.fill_diagonal(gap, np.inf)
+    assert gap.min() >= 0.01
+    assert not np.array_equal(ic.bubble_cloud_centres(64, seed=1)[0], c)
+
+
+def test_cloud_slabs_equal_whole_domain_evaluation():
+    N, nb = 36, 64
+    prm, grids, q = ic.bubble_cloud_3d(N=N, nbubbles=nb, slab=5)
+    xc, yc, zc = ic.cell_centres(prm, grids)
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    eta = np.zeros((N, N, N))
+    for c, R in zip(*ic.bubble_cloud_centres(nb)):
+        r = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2)
+        eta = np.maximum(eta, ic._smooth_indicator(r, R, 1.0 / N))
+    a_air = 1e-6 + (1.0 - 2e-6) * eta
+    b = prm.buff_size
+    inner = q[:, b:b + N, b:b + N, b:b + N]
+    assert np.array_equal(inner[7], a_air)
+    assert np.array_equal(inner[6], 1.0 - a_air)
+    # ghost layers are left for the boundary fill
+    assert not q[:, :b].any() and not q[:, :, :, :b].any()
+    # pressure recovered from the conservative state (stiffened-gas mixture)
+    rho = inner[0] + inner[1]
+    ke = 0.5 * (inner[2] ** 2 + inner[3] ** 2 + inner[4] ** 2) / rho
+    gam = inner[6] * prm.gammas[0] + inner[7] * prm.gammas[1]
+    pinf = inner[6] * prm.pi_infs[0] + inner[7] * prm.pi_infs[1]
+    p = (inner[5] - ke - pinf) / gam
+    shocked = (xc < 0.15)[None, None, :]
+    expect = np.broadcast_to(np.where(shocked, 1e9, 1e5), p.shape)
+    assert np.allclose(p, expect, rtol=1e-6, atol=0)
