@@ -547,8 +547,12 @@ SweepArgs sweep_args(int d) {
         a.nblocks = (long)a.ntile * (G.prm.n + 1) * (G.prm.p + 1);
     } else {
         const int s = G.dir[d].s;
+        const bool uni = G.prm.weno_order == 5 && G.dir[d].uniform;
         a.ntile = (G.prm.m + 1 + kTX - 1) / kTX;
-        a.chunk = (G.prm.precision == MFC_PREC_FP32_LEARNED ? t_faces<true>() : t_faces<false>()) - 1;
+        a.chunk = G.prm.precision == MFC_PREC_FP32_LEARNED ? t_tile_cells<true, float, true>()
+                  : G.prm.precision == MFC_PREC_FP32_BULK
+                      ? (uni ? t_tile_cells<false, float, true>() : t_tile_cells<false, float, false>())
+                      : (uni ? t_tile_cells<false, double, true>() : t_tile_cells<false, double, false>());
         a.nchunk = (s + 1 + a.chunk - 1) / a.chunk;
         const int other = d == 1 ? G.prm.p + 1 : G.prm.n + 1;
         a.nblocks = (long)a.ntile * a.nchunk * other;
