@@ -1,6 +1,7 @@
 """The 3D shock-bubble cloud in its three forms at reduced size: the FP32
 bulk and the learned weights against FP64 on the same grid, and all three
-against a 2x finer plain FP64 run (the judge) restricted to the coarse grid.
+against a 2x finer plain FP64 run (the reference solution) restricted to
+the coarse grid.
 
 The network is trained on pairs from a 2D water/air shock-bubble run: it
 sees one normalised 5-cell stencil per face and variable, so it carries
@@ -32,7 +33,7 @@ def _rel_l1(a, b):
                      for v in range(a.shape[0])])
 
 
-def test_cloud_forms_against_fp64_and_finer_judge(gpu):
+def test_cloud_forms_against_fp64_and_finer_run(gpu):
     torch = gpu
     # network: 2D two-fluid pairs (a 256^2 plain FP64 shock-bubble run)
     fprm, fg, fq = ic.shock_bubble_2d(N=256)
@@ -49,9 +50,9 @@ def test_cloud_forms_against_fp64_and_finer_judge(gpu):
 
     Nc, t_end, cfl = 48, 6e-5, 0.3
     jprm, jg, jq = ic.bubble_cloud_3d(N=2 * Nc)
-    judge, jsteps, jsec = learned.run(jprm, jg, jq, t_end, cfl=cfl)
-    assert np.all(np.isfinite(judge))
-    judge_c = learned.restrict_host(judge)
+    ref_f, jsteps, jsec = learned.run(jprm, jg, jq, t_end, cfl=cfl)
+    assert np.all(np.isfinite(ref_f))
+    ref_c = learned.restrict_host(ref_f)
     cprm, cg, cq = ic.bubble_cloud_3d(N=Nc)
     res = {}
     for prec in (0, PREC_FP32_BULK, PREC_FP32_LEARNED):
@@ -63,12 +64,12 @@ def test_cloud_forms_against_fp64_and_finer_judge(gpu):
             assert a.min() >= 0.0 and a.max() <= 1.0, prec
         res[prec] = (out, sec)
     e64 = res[0][0]
-    say(f"  {Nc}^3, {nsteps} RK3 steps to t = {t_end}; judge {2 * Nc}^3 plain FP64 ({jsteps} steps, "
+    say(f"  {Nc}^3, {nsteps} RK3 steps to t = {t_end}; finer run {2 * Nc}^3 plain FP64 ({jsteps} steps, "
         f"GPU s {jsec:.3f}); per variable {NAMES}")
     for prec, name in ((0, "plain FP64"), (PREC_FP32_BULK, "FP32 bulk"), (PREC_FP32_LEARNED, "learned")):
         out, sec = res[prec]
         same = _rel_l1(out, e64)
-        vs_judge = _rel_l1(out, judge_c)
+        vs_fine = _rel_l1(out, ref_c)
         say(f"    {name:11s} GPU s {sec:.3f}; rel L1 vs FP64 {Nc}^3: {np.array2string(same, precision=2)}")
-        say(f"    {'':11s} rel L1 vs judge: {np.array2string(vs_judge, precision=3)}")
+        say(f"    {'':11s} rel L1 vs finer run: {np.array2string(vs_fine, precision=3)}")
     assert _rel_l1(res[PREC_FP32_BULK][0], e64).max() <= 1e-4

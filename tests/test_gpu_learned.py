@@ -4,7 +4,7 @@ BF16 at the matrix-core operands) to the FP32-bulk tolerance, the untrained
 model reproduces the FP32-bulk plain path, conservation still holds, the per-face fallback keeps
 volume fractions in range, and -- the gain test -- a network trained on the
 GPU from plain-solver coarse/fine pairs lowers the coarse-grid error against
-a 2x-finer plain run (the judge) on a case it was not trained on."""
+a 2x-finer plain run (the reference solution) on a case it was not trained on."""
 import numpy as np
 import pytest
 
@@ -102,7 +102,7 @@ def test_learned_conserves_and_stays_bounded(gpu):
     assert np.all(np.isfinite(out)) and a.min() >= 0.0 and a.max() <= 1.0
 
 
-def test_trained_weights_beat_plain_against_finer_judge(gpu):
+def test_trained_weights_beat_plain_against_finer_run(gpu):
     torch = gpu
     Nc = 128
     # training pairs from a plain fine run (seed 0), shear layer resolved by
@@ -120,27 +120,27 @@ def test_trained_weights_beat_plain_against_finer_judge(gpu):
     say(f"  loss over all pairs {loss0:.4e} -> {loss1:.4e} ({len(losses)} Adam steps on the GPU)")
     assert loss1 < 0.7 * loss0
     # evaluation on two other seeds: coarse plain vs coarse learned, both
-    # against the 2x finer plain FP64 run (the judge), restricted to the
-    # coarse grid
+    # against the 2x finer plain FP64 run (the reference solution),
+    # restricted to the coarse grid
     names = ("rho", "rho*u", "rho*v", "E")
     for seed in (5, 7):
         t_end = 0.3
         fprm, fg, fq = ic.shear_layer_2d(N=2 * Nc, seed=seed, delta=0.02)
-        judge, nf, sf = learned.run(fprm, fg, fq, t_end)
-        judge_c = learned.restrict_host(judge)
+        ref_f, nf, sf = learned.run(fprm, fg, fq, t_end)
+        ref_c = learned.restrict_host(ref_f)
         cprm, cg, cq = ic.shear_layer_2d(N=Nc, seed=seed, delta=0.02)
         res = {}
         for prec in (0, PREC_FP32_BULK, PREC_FP32_LEARNED):
             cprm.precision = prec
             out, nsteps, sec = learned.run(cprm, cg, cq, t_end)
             assert np.all(np.isfinite(out))
-            res[prec] = (np.array([learned.l1_error(out, judge_c, v) for v in range(4)]), sec)
+            res[prec] = (np.array([learned.l1_error(out, ref_c, v) for v in range(4)]), sec)
         e64, e32, el = res[0][0], res[PREC_FP32_BULK][0], res[PREC_FP32_LEARNED][0]
-        say(f"  seed {seed}, t = {t_end}, {nsteps} steps on {Nc}^2; L1 vs 2x judge per variable {names}:")
+        say(f"  seed {seed}, t = {t_end}, {nsteps} steps on {Nc}^2; L1 vs the 2x finer run per variable {names}:")
         say(f"    plain FP64   {' '.join(f'{x:.3e}' for x in e64)}  GPU s {res[0][1]:.3f}")
         say(f"    plain FP32   {' '.join(f'{x:.3e}' for x in e32)}  GPU s {res[PREC_FP32_BULK][1]:.3f}")
         say(f"    learned      {' '.join(f'{x:.3e}' for x in el)}  GPU s {res[PREC_FP32_LEARNED][1]:.3f}")
-        say(f"    gain over plain FP32 {' '.join(f'{x:.3f}x' for x in e32 / el)}; judge GPU s {sf:.3f}")
+        say(f"    gain over plain FP32 {' '.join(f'{x:.3f}x' for x in e32 / el)}; finer run GPU s {sf:.3f}")
         # the shear layer's density, transverse momentum and energy errors
         # all drop; the streamwise momentum is left to the report
         for v in (0, 2, 3):
@@ -149,8 +149,8 @@ def test_trained_weights_beat_plain_against_finer_judge(gpu):
     # compared on the coarse grid
     seed, t_end = 5, 0.3
     jprm, jg, jq = ic.shear_layer_2d(N=4 * Nc, seed=seed, delta=0.02)
-    judge4, _, s4 = learned.run(jprm, jg, jq, t_end)
-    ref = learned.restrict_host(learned.restrict_host(judge4))
+    ref4, _, s4 = learned.run(jprm, jg, jq, t_end)
+    ref = learned.restrict_host(learned.restrict_host(ref4))
     fprm, fg, fq = ic.shear_layer_2d(N=2 * Nc, seed=seed, delta=0.02)
     fine, _, s2 = learned.run(fprm, fg, fq, t_end)
     cprm, cg, cq = ic.shear_layer_2d(N=Nc, seed=seed, delta=0.02)
