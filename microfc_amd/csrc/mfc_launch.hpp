@@ -14,10 +14,12 @@ struct SweepLaunch {
     int precision; // MFC_PREC_*
 };
 
-template <int NF>
+// F32: the FP32-bulk and learned forms (their translation units are built
+// with FMA contraction), else the plain FP64 form (built without).
+template <int NF, bool F32>
 int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
 
-#define MFC_SWEEP_LAUNCH_IMPL(NF)                                                                       \
+#define MFC_SWEEP_HELPERS(NF)                                                                           \
     template <int D, int WO, typename R, bool ML, bool UNI>                                             \
     static int launch_nf##NF##_dwrm(int dir, const SweepArgs &a, hipStream_t st) {                      \
         const dim3 grid((unsigned)a.nblocks);                                                           \
@@ -57,9 +59,19 @@ int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
         case 3: return launch_nf##NF##_dr<3, R>(L, a, st);                                              \
         default: return -1;                                                                             \
         }                                                                                               \
-    }                                                                                                   \
+    }
+
+#define MFC_SWEEP_LAUNCH_F64(NF)                                                                        \
+    MFC_SWEEP_HELPERS(NF)                                                                               \
     template <>                                                                                         \
-    int launch_sweep<NF>(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {                    \
+    int launch_sweep<NF, false>(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {             \
+        return L.precision == 0 ? launch_nf##NF##_r<double>(L, a, st) : -1;                             \
+    }
+
+#define MFC_SWEEP_LAUNCH_F32(NF)                                                                        \
+    MFC_SWEEP_HELPERS(NF)                                                                               \
+    template <>                                                                                         \
+    int launch_sweep<NF, true>(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {              \
         if (L.precision == 2) {                                                                         \
             if (L.weno_order != 5) return -1;                                                           \
             switch (L.num_dims) {                                                                       \
@@ -69,7 +81,7 @@ int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
             default: return -1;                                                                         \
             }                                                                                           \
         }                                                                                               \
-        return L.precision == 1 ? launch_nf##NF##_r<float>(L, a, st) : launch_nf##NF##_r<double>(L, a, st); \
+        return L.precision == 1 ? launch_nf##NF##_r<float>(L, a, st) : -1;                              \
     }
 
 } // namespace mfc

@@ -16,10 +16,14 @@ extern "C" {
 
 namespace mfc {
 const MlpFrags *mlp_frags();
-template <> int launch_sweep<1>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<2>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<3>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<4>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<1, false>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<2, false>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<3, false>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<4, false>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<1, true>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<2, true>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<3, true>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+template <> int launch_sweep<4, true>(const SweepLaunch &, const SweepArgs &, hipStream_t);
 } // namespace mfc
 
 using namespace mfc;
@@ -568,11 +572,12 @@ int sweep(int d, SweepArgs &a) {
     }
     TimedScope ts(kKSweepX + d);
     int rc;
+    const bool f32 = G.prm.precision != MFC_PREC_FP64;
     switch (G.prm.num_fluids) {
-    case 1: rc = launch_sweep<1>(L, a, G.stream); break;
-    case 2: rc = launch_sweep<2>(L, a, G.stream); break;
-    case 3: rc = launch_sweep<3>(L, a, G.stream); break;
-    case 4: rc = launch_sweep<4>(L, a, G.stream); break;
+    case 1: rc = f32 ? launch_sweep<1, true>(L, a, G.stream) : launch_sweep<1, false>(L, a, G.stream); break;
+    case 2: rc = f32 ? launch_sweep<2, true>(L, a, G.stream) : launch_sweep<2, false>(L, a, G.stream); break;
+    case 3: rc = f32 ? launch_sweep<3, true>(L, a, G.stream) : launch_sweep<3, false>(L, a, G.stream); break;
+    case 4: rc = f32 ? launch_sweep<4, true>(L, a, G.stream) : launch_sweep<4, false>(L, a, G.stream); break;
     default: rc = -1;
     }
     if (rc != 0) return fail(-3, "sweep launch failed (dir %d): %s", d, hipGetErrorString(hipGetLastError()));

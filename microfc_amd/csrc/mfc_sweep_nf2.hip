@@ -1,6 +1,6 @@
-// Sweep kernels for num_fluids = 2 (see mfc_sweep.hpp).
+// Sweep kernels for num_fluids = 2, plain FP64 form (see mfc_sweep.hpp).
 #include "mfc_launch.hpp"
 
 namespace mfc {
-MFC_SWEEP_LAUNCH_IMPL(2)
+MFC_SWEEP_LAUNCH_F64(2)
 } // namespace mfc

@@ -1,6 +1,6 @@
-// Sweep kernels for num_fluids = 4 (see mfc_sweep.hpp).
+// Sweep kernels for num_fluids = 4, plain FP64 form (see mfc_sweep.hpp).
 #include "mfc_launch.hpp"
 
 namespace mfc {
-MFC_SWEEP_LAUNCH_IMPL(4)
+MFC_SWEEP_LAUNCH_F64(4)
 } // namespace mfc

@@ -1,0 +1,7 @@
+// Sweep kernels for num_fluids = 4, FP32-bulk and learned forms (see
+// mfc_sweep.hpp); built with FMA contraction.
+#include "mfc_launch.hpp"
+
+namespace mfc {
+MFC_SWEEP_LAUNCH_F32(4)
+} // namespace mfc
