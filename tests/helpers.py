@@ -155,6 +155,22 @@ def small_cases():
     out.append(("droplet3d_stretched", prm, stretched_grids(prm), perturbed(prm, q)))
     prm, g, q = ic.bubble_cloud_3d(N=32, nbubbles=16)
     out.append(("cloud3d_32", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shock_bubble_2d(N=40, weno_order=3)
+    prm.time_stepper = 2
+    out.append(("bubble2d_w3_rk2", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shock_bubble_2d(N=40, weno_order=1)
+    prm.bc_beg = (-1, -2, -3)
+    prm.bc_end = (-1, -4, -3)
+    g = ic.uniform_grids(prm)
+    out.append(("bubble2d_w1_periodic_reflect", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shock_droplet_3d(N=20, weno_order=3)
+    prm.bc_beg = (-3, -1, -2)
+    prm.bc_end = (-4, -1, -2)
+    g = ic.uniform_grids(prm)
+    out.append(("droplet3d_w3_mixed_bc", prm, g, perturbed(prm, q)))
+    prm, g, q = ic.shock_droplet_3d(N=20, weno_order=1)
+    prm.time_stepper = 1
+    out.append(("droplet3d_w1_rk1", prm, g, perturbed(prm, q)))
     for nf in (3, 4):
         prm, g, q = multi_fluid_2d(nf)
         out.append((f"fluids{nf}_2d", prm, g, q))
