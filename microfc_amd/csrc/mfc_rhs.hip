@@ -553,10 +553,10 @@ SweepArgs sweep_args(int d) {
         const int s = G.dir[d].s;
         const bool uni = G.prm.weno_order == 5 && G.dir[d].uniform;
         a.ntile = (G.prm.m + 1 + kTX - 1) / kTX;
-        a.chunk = G.prm.precision == MFC_PREC_FP32_LEARNED ? t_tile_cells<true, float, true>()
+        a.chunk = G.prm.precision == MFC_PREC_FP32_LEARNED ? t_tile_cells<true, float, true>(d)
                   : G.prm.precision == MFC_PREC_FP32_BULK
-                      ? (uni ? t_tile_cells<false, float, true>() : t_tile_cells<false, float, false>())
-                      : (uni ? t_tile_cells<false, double, true>() : t_tile_cells<false, double, false>());
+                      ? (uni ? t_tile_cells<false, float, true>(d) : t_tile_cells<false, float, false>(d))
+                      : (uni ? t_tile_cells<false, double, true>(d) : t_tile_cells<false, double, false>(d));
         a.nchunk = (s + 1 + a.chunk - 1) / a.chunk;
         const int other = d == 1 ? G.prm.p + 1 : G.prm.n + 1;
         a.nblocks = (long)a.ntile * a.nchunk * other;

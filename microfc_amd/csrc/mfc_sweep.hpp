@@ -18,10 +18,11 @@
 //    tile of 16 x-columns by 16 faces along the sweep (15 cells).  Stencil
 //    rows of 16 consecutive x are staged in LDS, thread (x, f) solves face f
 //    of column x, and the fluxes go back through LDS for the divergence.
-//  * k_sweep_tc (y or z, cell-parallel): the same tiles, but each 16-lane row
-//    of a wave holds one column, thread (x, c) reconstructs both faces of
-//    cell c, the neighbouring face value moves one lane within the row, and
-//    the fluxes go through LDS to the x-fastest layout (14 cells per tile).
+//  * k_sweep_tc (y or z, cell-parallel): tiles of 16 x-columns by K threads
+//    along the sweep (K = 32 for the FP64 y sweep, else 16); K lanes of a
+//    wave hold one column, thread (x, c) reconstructs both faces of cell c,
+//    the neighbouring face value moves one lane within the column, and the
+//    fluxes go through LDS to the x-fastest layout (K - 2 cells per column).
 #pragma once
 
 #include "mfc_device.hpp"
@@ -544,8 +545,19 @@ constexpr int t_threads() { return kTX * t_faces<ML>(); }
 // The y/z sweep is cell-parallel (k_sweep_tc) in the forms whose x sweep is.
 template <bool ML, typename R, bool UNI>
 constexpr bool t_cell_parallel() { return x_cell_parallel<ML, R, UNI>(); }
+// Threads per column of the cell-parallel tile (a power of two <= 64; the
+// column's lanes are consecutive within a wave).  Measured on MI355X: 32
+// (512-thread blocks, 30 cells per column) makes the FP64 y sweep 2-3 %
+// faster than 16 (14 cells) through less halo staging; the z sweep, whose
+// stencil rows are planes apart, gains at 256^3 but loses 6 % at 512^3
+// (36 planes per block), and in FP32, where the 3D sweeps near the HBM
+// rate, 32 is up to 11 % slower.
+template <typename R>
+constexpr int tc_rows(int dir) { return sizeof(R) == 8 && dir == 1 ? 32 : 16; }
 template <bool ML, typename R, bool UNI>
-constexpr int t_tile_cells() { return t_faces<ML>() - (t_cell_parallel<ML, R, UNI>() ? 2 : 1); }
+constexpr int t_tile_cells(int dir) {
+    return t_cell_parallel<ML, R, UNI>() ? tc_rows<R>(dir) - 2 : t_faces<ML>() - 1;
+}
 
 template <int NF, int D, int DIR, int WO, typename R, bool ML, bool UNI>
 __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_t(
@@ -736,27 +748,29 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
 }
 
 // ---------------------------------------------------------------------------
-// y / z sweep, cell-parallel (the forms with a cell-parallel x sweep).  Lane
-// l of wave w works on column x0 + 4w + l/16; its position ts = l%16 in the
-// 16-lane row is cell y0-1+ts, of which it reconstructs both faces (one set
-// of smoothness indicators and reciprocals for the two).  The upper-face
-// value is the left state of face y0-1+ts + 1/2, and the right state is the
-// lower-face value of cell y0+ts, held by the next lane of the row, from
-// which it moves by a shuffle (no LDS, no barrier).  15 faces per column,
-// cells y0 .. y0+13 finished.  Staged columns are stored with an odd pitch
-// so the 16 lanes of a row read 16 consecutive values of one column.
-template <typename R>
+// y / z sweep, cell-parallel (the forms with a cell-parallel x sweep), K =
+// tc_rows<R>(DIR) threads per column.  Lane l of wave w works on column
+// x0 + (64/K)w + l/K; its position ts = l%K among the column's K lanes is
+// cell y0-1+ts, of which it reconstructs both faces (one set of smoothness
+// indicators and reciprocals for the two).  The upper-face value is the
+// left state of face y0-1+ts + 1/2, and the right state is the lower-face
+// value of cell y0+ts, held by the next lane of the column, from which it
+// moves by a shuffle (no LDS, no barrier).  K-1 faces per column, cells
+// y0 .. y0+K-3 finished.  Staged columns are stored with an odd pitch so
+// the K lanes of a column read K consecutive values.
+template <int W, typename R>
 __device__ __forceinline__ R row_next(R x) {
-    return __shfl_down(x, 1u, 16);
+    return __shfl_down(x, 1u, W);
 }
 
 template <int NF, int D, int DIR, int WO, typename R, bool UNI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_tc(const SweepArgs a) {
+__global__ __launch_bounds__(kTX * tc_rows<R>(DIR)) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_tc(
+    const SweepArgs a) {
     static_assert(t_cell_parallel<false, R, UNI>(), "cell-parallel forms only");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
-    constexpr int NT = 256, kTS = 16;
-    constexpr int TC = t_tile_cells<false, R, UNI>();
+    constexpr int kTS = tc_rows<R>(DIR), NT = kTX * kTS;
+    constexpr int TC = t_tile_cells<false, R, UNI>(DIR);
     constexpr int PN = (WO - 1) / 2;
     constexpr int NR = kTS + 2 * PN; // staged cells y0-1-PN .. y0+kTS-2+PN
     constexpr int P = NR | 1;        // odd column pitches
@@ -765,8 +779,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     constexpr int SP_WORDS = S * kTX * P, FL_WORDS = (S + 1) * kTS * DP;
     constexpr int WORDS = SP_WORDS > FL_WORDS ? SP_WORDS : FL_WORDS;
     // the stage input of the finished cells stays in LDS for the finish
-    // while the block still fits four to a CU
-    constexpr bool KEEP = sizeof(R) * WORDS + 8 * S * kTX * KP <= 40 * 1024;
+    // while four 256-thread blocks (or the same threads in fewer blocks)
+    // still fit a CU
+    constexpr bool KEEP = sizeof(R) * WORDS + 8 * S * kTX * KP <= 40 * 1024 * (NT / 256);
     __shared__ R lds[WORDS];
     __shared__ double lcons[KEEP ? S : 1][kTX][KEEP ? KP : 1];
     R(*sp)[kTX][P] = reinterpret_cast<R(*)[kTX][P]>(lds);
@@ -781,7 +796,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int x0 = xt * kTX;
     const int y0 = chunk * TC;
     const int tid = threadIdx.x;
-    const int tx = 4 * (tid >> 6) + ((tid & 63) >> 4), ts = tid & 15;
+    const int tx = (64 / kTS) * (tid >> 6) + ((tid & 63) / kTS), ts = tid & (kTS - 1);
 
     auto gcell = [&](int i, int c) -> long {
         return (DIR == 1) ? ghost_off(a, i, c, t) : ghost_off(a, i, t, c);
@@ -790,7 +805,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         return (DIR == 1) ? interior_off(a, i, c, t) : interior_off(a, i, t, c);
     };
 
-    // 1. stage primitive rows y0-1-PN .. y0+kTS-2+PN of columns x0..x0+15
+    // 1. stage primitive rows y0-1-PN .. y0+kTS-2+PN of columns x0..x0+kTX-1
     // (loads along x, every load issued before the first conversion)
     {
         constexpr int NIT = (NR * kTX + NT - 1) / NT;
@@ -838,7 +853,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 #pragma unroll
             for (int w = 0; w < WO; ++w) st[w] = sp[v][tx][ts + w];
             weno_cell<WO, R, true, true>(st, cf, eps, vl, qL[v]);
-            qR[v] = row_next(vl);
+            qR[v] = row_next<kTS>(vl);
         }
     }
     // every lane solves its face, valid or not (lane 15 of a row and lanes
@@ -858,7 +873,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     // 3. the fluxes go through the dead stencil columns to the x-fastest
     // thread layout (16 consecutive x per 16 lanes), which finishes cell
     // y0+fs of column x0+fx: the finish's HBM reads and writes then cover
-    // 128-byte rows instead of 32-byte pieces (measured: 9 % faster z sweep)
+    // 128-byte rows instead of 64/K-column pieces (measured with K = 16:
+    // 9 % faster z sweep)
     __syncthreads();
 #pragma unroll
     for (int v = 0; v < S; ++v) fl[v][ts][tx] = f[v];
