@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *S, long n, const
                     wp[k] = S[(long)(8 + k) * n + idx];
                 }
                 const float tt = S[11L * n + idx];
-                mlp_weights(z, wp, wgt);
+                const float g = mlp_weights(z, wp, wgt);
                 const float y = wgt[0] * p[0] + wgt[1] * p[1] + wgt[2] * p[2];
                 const float r = y - tt;
                 lsum += r * r;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *S, long n, const
                 const float sdw = wgt[0] * dy * p[0] + wgt[1] * dy * p[1] + wgt[2] * dy * p[2];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    dzv[k] = wgt[k] * (dy * p[k] - sdw);
+                    dzv[k] = g * wgt[k] * (dy * p[k] - sdw);
                     db3[k] += dzv[k];
                 }
             }

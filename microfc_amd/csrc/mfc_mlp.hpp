@@ -6,9 +6,10 @@
 // the face is j+1/2) is normalised as x_k = (v_k - v_2)/s with s the largest
 // neighbour difference.  The network maps x to logits z (3); the weights are
 // the classical WENO5 weights modulated by them,
-//     w_k = w_plain_k exp(z_k) / sum_m w_plain_m exp(z_m),
-// so a zero last layer reproduces plain WENO5 and the plain weights are the
-// per-face fallback.  The face value is v_2 + s * sum_k w_k p_k with p_k the
+//     w_k = w_plain_k exp(g z_k) / sum_m w_plain_m exp(g z_m),
+// with g in [0, 1] a gate on how far the plain weights are from ideal
+// (mlp_gate), so a zero last layer, or smooth resolved flow, reproduces
+// plain WENO5, and the plain weights are the per-face fallback.  The face value is v_2 + s * sum_k w_k p_k with p_k the
 // normalised candidate polynomials.  Training minimises the mean of
 // (sum_k w_k p_k - t)^2 with t the normalised fine-grid face value.
 //
@@ -256,18 +257,29 @@ __device__ __forceinline__ void mlp_infer_lane(const MlpTRegs &R, const unsigned
     }
 }
 
-// Learned weights of one face from its logits and plain weights.
-__device__ __forceinline__ void mlp_weights(const float *z, const float *wp, float *w) {
-    const float zm = fmaxf(z[0], fmaxf(z[1], z[2]));
-    float u[3], s = 0.f;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        u[k] = wp[k] * __expf(z[k] - zm);
-        s += u[k];
-    }
-    const float inv = 1.f / s;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) w[k] = u[k] * inv;
+// Gate of the logits: g = min(1, sum_k |w_plain_k - d_k| / kGateDev) with d
+// the ideal weights of a uniform grid for the oriented face (0.3, 0.6, 0.1).
+// Where the plain weights are near ideal (smooth, resolved flow, in which
+// WENO5 is already close to its optimal linear scheme) the logits fade out;
+// near discontinuities and under-resolved features they act in full.
+constexpr float kGateDev = 0.03f;
+__device__ __forceinline__ float mlp_gate(const float *wp) {
+    const float dev = fabsf(wp[0] - 0.3f) + fabsf(wp[1] - 0.6f) + fabsf(wp[2] - 0.1f);
+    return fminf(1.f, dev / kGateDev);
+}
+
+// Learned weights of one face from its logits and plain weights:
+// w_k = w_plain_k exp(g z_k) / sum_m w_plain_m exp(g z_m).  Returns g.
+__device__ __forceinline__ float mlp_weights(const float *z, const float *wp, float *w) {
+    const float g = mlp_gate(wp);
+    const float z0 = g * z[0], z1 = g * z[1], z2 = g * z[2];
+    const float zm = fmaxf(z0, fmaxf(z1, z2));
+    const float u0 = wp[0] * __expf(z0 - zm), u1 = wp[1] * __expf(z1 - zm), u2 = wp[2] * __expf(z2 - zm);
+    const float inv = 1.f / (u0 + u1 + u2);
+    w[0] = u0 * inv;
+    w[1] = u1 * inv;
+    w[2] = u2 * inv;
+    return g;
 }
 
 } // namespace mfc

@@ -320,7 +320,7 @@ void orc_set_learned(const float *params) {
  * largest neighbour difference, evaluated in FP32 on the FP32 stencil (the
  * learned form reconstructs in FP32), then rounded to BF16. */
 static void learned_weights(const double *v, const double *wp, double *w) {
-    double x[5], h1[32], h2[32], z[3], zm, u[3], su;
+    double x[5], h1[32], h2[32], z[3], zm, u[3], su, g;
     float vf[5], sf = 0.0f, inv;
     int k, n;
     for (k = 0; k < 5; k++) vf[k] = (float)v[k];
@@ -344,6 +344,10 @@ static void learned_weights(const double *v, const double *wp, double *w) {
         for (k = 0; k < 32; k++) acc += h2[k] * g_w3[k][n];
         z[n] = acc + g_b3[n];
     }
+    /* gate: the logits fade out as the plain weights approach the ideal
+       weights of a uniform grid for the oriented face */
+    g = fmin(1.0, (fabs(wp[0] - 0.3) + fabs(wp[1] - 0.6) + fabs(wp[2] - 0.1)) / 0.03);
+    for (k = 0; k < 3; k++) z[k] *= g;
     zm = fmax(z[0], fmax(z[1], z[2]));
     su = 0.0;
     for (k = 0; k < 3; k++) {

@@ -50,6 +50,17 @@ def _r(a, on):
     return bf16(a) if on else np.asarray(a, dtype=np.float64)
 
 
+GATE_IDEAL = np.array([0.3, 0.6, 0.1])
+GATE_DEV = 0.03
+
+
+def gate(wp):
+    """Logit gate: min(1, sum_k |w_plain_k - d_k| / 0.03), d the uniform-grid
+    ideal weights of the oriented face; the logits act in full away from
+    smooth, resolved flow and fade out as the plain weights approach d."""
+    return np.minimum(1.0, np.abs(np.asarray(wp) - GATE_IDEAL).sum(axis=1) / GATE_DEV)
+
+
 def forward(params, S, bf16_points=True):
     """S: (12, n) samples.  Returns a dict with the intermediate values."""
     S = np.asarray(S, dtype=np.float64)
@@ -67,11 +78,13 @@ def forward(params, S, bf16_points=True):
     wp = S[8:11].T
     p = S[5:8].T
     t = S[11]
-    u = wp * np.exp(z - z.max(axis=1, keepdims=True))
+    g = gate(wp)
+    zg = g[:, None] * z
+    u = wp * np.exp(zg - zg.max(axis=1, keepdims=True))
     w = u / u.sum(axis=1, keepdims=True)
     y = (w * p).sum(axis=1)
     loss = np.mean((y - t) ** 2)
-    return dict(x=xr, a1=a1, h1=h1, a2=a2, h2=h2, z=z, w=w, y=y, p=p, t=t, loss=loss,
+    return dict(x=xr, a1=a1, h1=h1, a2=a2, h2=h2, z=z, g=g, w=w, y=y, p=p, t=t, loss=loss,
                 W1r=W1r, W2r=W2r, W3r=W3r, n=n)
 
 
@@ -81,7 +94,7 @@ def gradient(params, S, bf16_points=True):
     dy = 2.0 * (f["y"] - f["t"]) / n
     dw = dy[:, None] * f["p"]
     sdw = (f["w"] * dw).sum(axis=1, keepdims=True)
-    dz3 = f["w"] * (dw - sdw)
+    dz3 = f["g"][:, None] * f["w"] * (dw - sdw)
     db3 = np.zeros(16)
     db3[:3] = dz3.sum(axis=0)
     dz = np.zeros((n, 16))
