@@ -4,7 +4,8 @@
 * logits forced by the last bias select single candidates, which must be the
   textbook third-order polynomials of each face (this fixes the stencil
   orientation of both faces, independently of the code under test);
-* equal logits give the plain WENO5 value;
+* equal logits give the plain WENO5 value, and so do any logits where the
+  plain weights are ideal (the gate on their distance from ideal closes);
 * the per-face fallback keeps the plain value where the learned one breaks a
   bound the plain one keeps;
 * the C forward pass agrees with the independent numpy MLP oracle;
@@ -54,6 +55,24 @@ def test_forced_logits_pick_textbook_candidates(learned):
         vl, vr = O.weno5_learned_cell(1e-16, cf, v)
         assert vr == pytest.approx(right[k], rel=1e-12), k
         assert vl == pytest.approx(left[k], rel=1e-12), k
+
+
+def test_gate_closes_at_ideal_weights(learned):
+    """A huge weno_eps swamps the smoothness indicators, so the plain weights
+    are the ideal ones to ~1e-10 and the gate (their distance from ideal over
+    0.03) is ~1e-8: logits forced to one candidate then leave the plain
+    WENO5 value, the linear fifth-order one, although the candidates differ.
+    With the gate open (rough data, weno_eps 1e-16) the same logits select
+    single candidates (test_forced_logits_pick_textbook_candidates)."""
+    cf = _coef()
+    v = np.array([0.3, -0.2, 0.7, 1.1, 0.4])
+    linear_r = (2 * v[0] - 13 * v[1] + 47 * v[2] + 27 * v[3] - 3 * v[4]) / 60
+    for b3 in ([80.0, 0.0, -80.0], [0.0, 0.0, 80.0]):
+        learned(_bias_only(b3))
+        pl, pr = O.weno_cell(5, 1e10, cf, v)
+        ll, lr = O.weno5_learned_cell(1e10, cf, v)
+        assert pr == pytest.approx(linear_r, rel=1e-8)
+        assert lr == pytest.approx(pr, rel=1e-6) and ll == pytest.approx(pl, rel=1e-6)
 
 
 def test_equal_logits_give_plain_weno5(learned):
