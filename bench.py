@@ -29,7 +29,7 @@ FP64_PEAK_TFLOPS = 78.6
 # (profiles/hbm_traffic_shock_bubble_2d_2048_fp64.md,
 # profiles/sweep_kernels_shock_bubble_2d_2048_fp64.md).  Other workloads
 # report null.
-PROFILED = {("shock_bubble_2d", 2048, "fp64", "sweep_y"): {"traffic": 715.0e6 + 236.8e6, "valu_busy": 0.85}}
+PROFILED = {("shock_bubble_2d", 2048, "fp64", "sweep_y"): {"traffic": 678.5e6 + 236.6e6, "valu_busy": 0.79}}
 
 
 def sweep_bytes_per_step(prm):
