@@ -552,7 +552,12 @@ SweepArgs sweep_args(int d) {
     } else {
         const int s = G.dir[d].s;
         const bool uni = G.prm.weno_order == 5 && G.dir[d].uniform;
-        a.ntile = (G.prm.m + 1 + kTX - 1) / kTX;
+        const bool cp = G.prm.precision == MFC_PREC_FP32_LEARNED ? t_cell_parallel<true, float, true>()
+                        : G.prm.precision == MFC_PREC_FP32_BULK
+                            ? (uni ? t_cell_parallel<false, float, true>() : t_cell_parallel<false, float, false>())
+                            : (uni ? t_cell_parallel<false, double, true>() : t_cell_parallel<false, double, false>());
+        const int xc = !cp ? kTX : G.prm.precision == MFC_PREC_FP64 ? tc_cols<double>(d) : tc_cols<float>(d);
+        a.ntile = (G.prm.m + 1 + xc - 1) / xc;
         a.chunk = G.prm.precision == MFC_PREC_FP32_LEARNED ? t_tile_cells<true, float, true>(d)
                   : G.prm.precision == MFC_PREC_FP32_BULK
                       ? (uni ? t_tile_cells<false, float, true>(d) : t_tile_cells<false, float, false>(d))

@@ -18,8 +18,9 @@
 //    tile of 16 x-columns by 16 faces along the sweep (15 cells).  Stencil
 //    rows of 16 consecutive x are staged in LDS, thread (x, f) solves face f
 //    of column x, and the fluxes go back through LDS for the divergence.
-//  * k_sweep_tc (y or z, cell-parallel): tiles of 16 x-columns by K threads
-//    along the sweep (K = 32 for the FP64 y sweep, else 16); K lanes of a
+//  * k_sweep_tc (y or z, cell-parallel): tiles of C x-columns by K threads
+//    along the sweep (K = 32 for the FP64 y sweep, else 16; C = 32 for the
+//    FP32 z sweep, else 16); K lanes of a
 //    wave hold one column, thread (x, c) reconstructs both faces of cell c,
 //    the neighbouring face value moves one lane within the column, and the
 //    fluxes go through LDS to the x-fastest layout (K - 2 cells per column).
@@ -554,6 +555,11 @@ constexpr bool t_cell_parallel() { return x_cell_parallel<ML, R, UNI>(); }
 // rate, 32 is up to 11 % slower.
 template <typename R>
 constexpr int tc_rows(int dir) { return sizeof(R) == 8 && dir == 1 ? 32 : 16; }
+// x columns per cell-parallel tile: 32 for the FP32 z sweep, whose stencil
+// rows are planes apart (twice the cells per page touched; measured 7.5 %
+// faster at 512^3, 5 % at 256^3; 64 columns and FP64 gain nothing), else 16.
+template <typename R>
+constexpr int tc_cols(int dir) { return sizeof(R) == 4 && dir == 2 ? 32 : kTX; }
 template <bool ML, typename R, bool UNI>
 constexpr int t_tile_cells(int dir) {
     return t_cell_parallel<ML, R, UNI>() ? tc_rows<R>(dir) - 2 : t_faces<ML>() - 1;
@@ -749,7 +755,8 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
 
 // ---------------------------------------------------------------------------
 // y / z sweep, cell-parallel (the forms with a cell-parallel x sweep), K =
-// tc_rows<R>(DIR) threads per column.  Lane l of wave w works on column
+// tc_rows<R>(DIR) threads per column, C = tc_cols<R>(DIR) columns, C * K
+// threads per block.  Lane l of wave w works on column
 // x0 + (64/K)w + l/K; its position ts = l%K among the column's K lanes is
 // cell y0-1+ts, of which it reconstructs both faces (one set of smoothness
 // indicators and reciprocals for the two).  The upper-face value is the
@@ -764,27 +771,27 @@ __device__ __forceinline__ R row_next(R x) {
 }
 
 template <int NF, int D, int DIR, int WO, typename R, bool UNI>
-__global__ __launch_bounds__(kTX * tc_rows<R>(DIR)) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_tc(
+__global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR)) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_tc(
     const SweepArgs a) {
     static_assert(t_cell_parallel<false, R, UNI>(), "cell-parallel forms only");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
-    constexpr int kTS = tc_rows<R>(DIR), NT = kTX * kTS;
+    constexpr int XC = tc_cols<R>(DIR), kTS = tc_rows<R>(DIR), NT = XC * kTS;
     constexpr int TC = t_tile_cells<false, R, UNI>(DIR);
     constexpr int PN = (WO - 1) / 2;
     constexpr int NR = kTS + 2 * PN; // staged cells y0-1-PN .. y0+kTS-2+PN
     constexpr int P = NR | 1;        // odd column pitches
     constexpr int KP = TC | 1;
-    constexpr int DP = kTX + 1;      // flux rows
-    constexpr int SP_WORDS = S * kTX * P, FL_WORDS = (S + 1) * kTS * DP;
+    constexpr int DP = XC + 1;      // flux rows
+    constexpr int SP_WORDS = S * XC * P, FL_WORDS = (S + 1) * kTS * DP;
     constexpr int WORDS = SP_WORDS > FL_WORDS ? SP_WORDS : FL_WORDS;
     // the stage input of the finished cells stays in LDS for the finish
     // while four 256-thread blocks (or the same threads in fewer blocks)
     // still fit a CU
-    constexpr bool KEEP = sizeof(R) * WORDS + 8 * S * kTX * KP <= 40 * 1024 * (NT / 256);
+    constexpr bool KEEP = sizeof(R) * WORDS + 8 * S * XC * KP <= 40 * 1024 * (NT / 256);
     __shared__ R lds[WORDS];
-    __shared__ double lcons[KEEP ? S : 1][kTX][KEEP ? KP : 1];
-    R(*sp)[kTX][P] = reinterpret_cast<R(*)[kTX][P]>(lds);
+    __shared__ double lcons[KEEP ? S : 1][XC][KEEP ? KP : 1];
+    R(*sp)[XC][P] = reinterpret_cast<R(*)[XC][P]>(lds);
     R(*fl)[kTS][DP] = reinterpret_cast<R(*)[kTS][DP]>(lds);
 
     const long bid = xcd_logical_block(blockIdx.x, a.nblocks);
@@ -793,7 +800,7 @@ __global__ __launch_bounds__(kTX * tc_rows<R>(DIR)) __attribute__((amdgpu_waves_
     const int chunk = (int)(rest % a.nchunk);
     const int t = (int)(rest / a.nchunk);
     const int s = (DIR == 1) ? a.n : a.p;
-    const int x0 = xt * kTX;
+    const int x0 = xt * XC;
     const int y0 = chunk * TC;
     const int tid = threadIdx.x;
     const int tx = (64 / kTS) * (tid >> 6) + ((tid & 63) / kTS), ts = tid & (kTS - 1);
@@ -805,17 +812,17 @@ __global__ __launch_bounds__(kTX * tc_rows<R>(DIR)) __attribute__((amdgpu_waves_
         return (DIR == 1) ? interior_off(a, i, c, t) : interior_off(a, i, t, c);
     };
 
-    // 1. stage primitive rows y0-1-PN .. y0+kTS-2+PN of columns x0..x0+kTX-1
+    // 1. stage primitive rows y0-1-PN .. y0+kTS-2+PN of columns x0..x0+XC-1
     // (loads along x, every load issued before the first conversion)
     {
-        constexpr int NIT = (NR * kTX + NT - 1) / NT;
+        constexpr int NIT = (NR * XC + NT - 1) / NT;
         double cv[NIT][S];
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int e = tid + NT * it;
-            const int ex = e % kTX, er = e / kTX;
+            const int ex = e % XC, er = e / XC;
             const int c = y0 - 1 - PN + er;
-            const bool ok = e < NR * kTX && c <= s + a.b;
+            const bool ok = e < NR * XC && c <= s + a.b;
             const long go = gcell(min(x0 + ex, a.m), ok ? c : y0);
 #pragma unroll
             for (int v = 0; v < S; ++v) cv[it][v] = ok ? a.qin.v[v][go] : 0.0;
@@ -823,8 +830,8 @@ __global__ __launch_bounds__(kTX * tc_rows<R>(DIR)) __attribute__((amdgpu_waves_
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int e = tid + NT * it;
-            if (e < NR * kTX) {
-                const int ex = e % kTX, er = e / kTX;
+            if (e < NR * XC) {
+                const int ex = e % XC, er = e / XC;
                 double pr[S];
                 cons_to_prim<NF, D>(cv[it], pr, a.eos);
 #pragma unroll
@@ -871,16 +878,16 @@ __global__ __launch_bounds__(kTX * tc_rows<R>(DIR)) __attribute__((amdgpu_waves_
     }
     hllc_face<NF, D, DIR, R>(qL, qR, a.eos, f, us);
     // 3. the fluxes go through the dead stencil columns to the x-fastest
-    // thread layout (16 consecutive x per 16 lanes), which finishes cell
+    // thread layout (C consecutive x per C threads), which finishes cell
     // y0+fs of column x0+fx: the finish's HBM reads and writes then cover
-    // 128-byte rows instead of 64/K-column pieces (measured with K = 16:
-    // 9 % faster z sweep)
+    // rows of C doubles instead of 64/K-column pieces (measured with K = 16,
+    // C = 16: 9 % faster z sweep)
     __syncthreads();
 #pragma unroll
     for (int v = 0; v < S; ++v) fl[v][ts][tx] = f[v];
     fl[S][ts][tx] = us;
     __syncthreads();
-    const int fx = tid % kTX, fs = tid / kTX;
+    const int fx = tid % XC, fs = tid / XC;
     const int i = x0 + fx, cc = y0 + fs;
     if (fs < TC && i <= a.m && cc <= s) {
         R Fm[S], Fp[S];
@@ -891,7 +898,7 @@ __global__ __launch_bounds__(kTX * tc_rows<R>(DIR)) __attribute__((amdgpu_waves_
         }
         FinishIn<S> fin;
         if constexpr (KEEP)
-            finish_load_staged<NF, D>(a, gcell(i, cc), rcell(i, cc), &lcons[0][fx][fs], kTX * KP, fin);
+            finish_load_staged<NF, D>(a, gcell(i, cc), rcell(i, cc), &lcons[0][fx][fs], XC * KP, fin);
         else
             finish_load<NF, D>(a, gcell(i, cc), rcell(i, cc), fin);
         finish_cell<NF, D, R>(a, gcell(i, cc), rcell(i, cc), a.inv_ds[cc], Fm, Fp, fl[S][fs][fx], fl[S][fs + 1][fx],
