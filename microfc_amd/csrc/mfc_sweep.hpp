@@ -782,7 +782,7 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR)) __attribute__((a
     constexpr int NR = kTS + 2 * PN; // staged cells y0-1-PN .. y0+kTS-2+PN
     constexpr int P = NR | 1;        // odd column pitches
     constexpr int KP = TC | 1;
-    constexpr int DP = XC + 1;      // flux rows
+    constexpr int DP = XC + 1;       // flux rows
     constexpr int SP_WORDS = S * XC * P, FL_WORDS = (S + 1) * kTS * DP;
     constexpr int WORDS = SP_WORDS > FL_WORDS ? SP_WORDS : FL_WORDS;
     // the stage input of the finished cells stays in LDS for the finish
