@@ -32,8 +32,9 @@ extern "C" {
 
 /* Learned WENO5 weights: a 5 -> 32 -> 32 -> 3 MLP (ReLU) whose logits z
  * modulate the plain weights, w_k ~ w_plain_k exp(g z_k), with the gate
- * g = min(1, sum_k |w_plain_k - d_k| / 0.03) and d = (0.3, 0.6, 0.1) the
- * uniform-grid ideal weights of the oriented face.  Parameters are one
+ * g = min(1, max(0, sum_k |w_plain_k - d_k| - 0.005) / 0.03) and
+ * d = (0.3, 0.6, 0.1) the uniform-grid ideal weights of the oriented face
+ * (w = w_plain where g = 0).  Parameters are one
  * FP32 block of MFC_MLP_NPARAM floats: W1[16][32] (input rows >= 5 zero),
  * b1[32], W2[32][32], b2[32], W3[32][16] (output columns >= 3 zero), b3[16].
  * A training sample is 12 floats (structure of arrays, 12 planes of n): the

@@ -257,21 +257,30 @@ __device__ __forceinline__ void mlp_infer_lane(const MlpTRegs &R, const unsigned
     }
 }
 
-// Gate of the logits: g = min(1, sum_k |w_plain_k - d_k| / kGateDev) with d
-// the ideal weights of a uniform grid for the oriented face (0.3, 0.6, 0.1).
-// Where the plain weights are near ideal (smooth, resolved flow, in which
-// WENO5 is already close to its optimal linear scheme) the logits fade out;
-// near discontinuities and under-resolved features they act in full.
-constexpr float kGateDev = 0.03f;
+// Gate of the logits: with dev = sum_k |w_plain_k - d_k| and d the ideal
+// weights of a uniform grid for the oriented face (0.3, 0.6, 0.1),
+// g = min(1, max(0, dev - kGateOff) / kGateDev).  Where the plain weights are
+// near ideal (smooth, resolved flow, in which WENO5 is already close to its
+// optimal linear scheme) g is zero and the face keeps its plain weights
+// exactly; near discontinuities and under-resolved features the logits act
+// in full.  A wave whose faces are all closed skips the network.
+constexpr float kGateOff = 0.005f, kGateDev = 0.03f;
 __device__ __forceinline__ float mlp_gate(const float *wp) {
     const float dev = fabsf(wp[0] - 0.3f) + fabsf(wp[1] - 0.6f) + fabsf(wp[2] - 0.1f);
-    return fminf(1.f, dev / kGateDev);
+    return fminf(1.f, fmaxf(0.f, dev - kGateOff) / kGateDev);
 }
 
 // Learned weights of one face from its logits and plain weights:
-// w_k = w_plain_k exp(g z_k) / sum_m w_plain_m exp(g z_m).  Returns g.
+// w_k = w_plain_k exp(g z_k) / sum_m w_plain_m exp(g z_m), or w = w_plain
+// when g = 0.  Returns g.
 __device__ __forceinline__ float mlp_weights(const float *z, const float *wp, float *w) {
     const float g = mlp_gate(wp);
+    if (g == 0.f) {
+        w[0] = wp[0];
+        w[1] = wp[1];
+        w[2] = wp[2];
+        return 0.f;
+    }
     const float z0 = g * z[0], z1 = g * z[1], z2 = g * z[2];
     const float zm = fmaxf(z0, fmaxf(z1, z2));
     const float u0 = wp[0] * __expf(z0 - zm), u1 = wp[1] * __expf(z1 - zm), u2 = wp[2] * __expf(z2 - zm);

@@ -305,9 +305,22 @@ __device__ __forceinline__ float ml_features(const float *v, unsigned short *row
     return sc;
 }
 
+// Whether the network may act on a stencil: its variation must stand above
+// rounding noise (largest neighbour difference > 1e-6 of the largest
+// magnitude, the rule that admits training pairs) and the gate must be open.
+__device__ __forceinline__ bool ml_open(const float *v, const float *wp) {
+    float sc = 0.f, vmax = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sc = fmaxf(sc, fabsf(v[k + 1] - v[k]));
+#pragma unroll
+    for (int k = 0; k < 5; ++k) vmax = fmaxf(vmax, fabsf(v[k]));
+    return sc > 1e-6f * vmax && mlp_gate(wp) > 0.f;
+}
+
 template <int NF, int D>
-__device__ __forceinline__ float ml_value(int var, const float *z, const float *p, const float *wp) {
+__device__ __forceinline__ float ml_value(int var, bool open, const float *z, const float *p, const float *wp) {
     using Y = Sys<NF, D>;
+    if (!open) return wp[0] * p[0] + wp[1] * p[1] + wp[2] * p[2];
     float w[3];
     mlp_weights(z, wp, w);
     const float learned = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
@@ -330,17 +343,25 @@ __device__ __forceinline__ void ml_face_states(int var, const float *stR, const 
     float pR[3], wR[3], pL[3], wL[3], vo[5];
     weno5_parts<true>(stR, cfR, eps, pR, wR);
     weno5_parts<false>(stL, cfL, eps, pL, wL);
-    ml_features(stR, xs[0][lane]);
+    // the network runs for the whole wave (its MFMA passes need every lane)
+    // unless it may act on neither side of any of the 64 faces
 #pragma unroll
     for (int k = 0; k < 5; ++k) vo[k] = stL[4 - k];
-    ml_features(vo, xs[1][lane]);
-    wave_lds_sync();
-    f32x4 z[2];
-    mlp_infer_lane(T, xs, lane, z);
-    wave_lds_sync(); // the next variable rewrites xs
-    const float zR[3] = {z[0][0], z[0][1], z[0][2]}, zL[3] = {z[1][0], z[1][1], z[1][2]};
-    qLv = ml_value<NF, D>(var, zR, pR, wR);
-    qRv = ml_value<NF, D>(var, zL, pL, wL);
+    const bool openR = ml_open(stR, wR), openL = ml_open(vo, wL);
+    if (__any(openR || openL)) {
+        ml_features(stR, xs[0][lane]);
+        ml_features(vo, xs[1][lane]);
+        wave_lds_sync();
+        f32x4 z[2];
+        mlp_infer_lane(T, xs, lane, z);
+        wave_lds_sync(); // the next variable rewrites xs
+        const float zR[3] = {z[0][0], z[0][1], z[0][2]}, zL[3] = {z[1][0], z[1][1], z[1][2]};
+        qLv = ml_value<NF, D>(var, openR, zR, pR, wR);
+        qRv = ml_value<NF, D>(var, openL, zL, pL, wL);
+    } else {
+        qLv = wR[0] * pR[0] + wR[1] * pR[1] + wR[2] * pR[2];
+        qRv = wL[0] * pL[0] + wL[1] * pL[1] + wL[2] * pL[2];
+    }
 }
 
 // ---------------------------------------------------------------------------

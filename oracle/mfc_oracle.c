@@ -321,10 +321,17 @@ void orc_set_learned(const float *params) {
  * learned form reconstructs in FP32), then rounded to BF16. */
 static void learned_weights(const double *v, const double *wp, double *w) {
     double x[5], h1[32], h2[32], z[3], zm, u[3], su, g;
-    float vf[5], sf = 0.0f, inv;
+    float vf[5], sf = 0.0f, vmf = 0.0f, inv;
     int k, n;
     for (k = 0; k < 5; k++) vf[k] = (float)v[k];
     for (k = 0; k < 4; k++) sf = fmaxf(sf, fabsf(vf[k + 1] - vf[k]));
+    for (k = 0; k < 5; k++) vmf = fmaxf(vmf, fabsf(vf[k]));
+    /* a stencil whose variation is rounding noise keeps the plain weights
+       (the rule that admits training pairs, on the FP32 stencil) */
+    if (!(sf > 1e-6f * vmf)) {
+        for (k = 0; k < 3; k++) w[k] = wp[k];
+        return;
+    }
     inv = sf > 0.0f ? 1.0f / sf : 0.0f;
     for (k = 0; k < 5; k++) x[k] = bf16r((double)((vf[k] - vf[2]) * inv));
     for (n = 0; n < 32; n++) {
@@ -345,8 +352,13 @@ static void learned_weights(const double *v, const double *wp, double *w) {
         z[n] = acc + g_b3[n];
     }
     /* gate: the logits fade out as the plain weights approach the ideal
-       weights of a uniform grid for the oriented face */
-    g = fmin(1.0, (fabs(wp[0] - 0.3) + fabs(wp[1] - 0.6) + fabs(wp[2] - 0.1)) / 0.03);
+       weights of a uniform grid for the oriented face, and the plain weights
+       are kept exactly within 0.005 of them */
+    g = fmin(1.0, fmax(0.0, fabs(wp[0] - 0.3) + fabs(wp[1] - 0.6) + fabs(wp[2] - 0.1) - 0.005) / 0.03);
+    if (g == 0.0) {
+        for (k = 0; k < 3; k++) w[k] = wp[k];
+        return;
+    }
     for (k = 0; k < 3; k++) z[k] *= g;
     zm = fmax(z[0], fmax(z[1], z[2]));
     su = 0.0;

@@ -51,14 +51,16 @@ def _r(a, on):
 
 
 GATE_IDEAL = np.array([0.3, 0.6, 0.1])
-GATE_DEV = 0.03
+GATE_OFF, GATE_DEV = 0.005, 0.03
 
 
 def gate(wp):
-    """Logit gate: min(1, sum_k |w_plain_k - d_k| / 0.03), d the uniform-grid
-    ideal weights of the oriented face; the logits act in full away from
-    smooth, resolved flow and fade out as the plain weights approach d."""
-    return np.minimum(1.0, np.abs(np.asarray(wp) - GATE_IDEAL).sum(axis=1) / GATE_DEV)
+    """Logit gate: min(1, max(0, sum_k |w_plain_k - d_k| - 0.005) / 0.03), d
+    the uniform-grid ideal weights of the oriented face; the logits act in
+    full away from smooth, resolved flow, fade out as the plain weights
+    approach d, and where g = 0 the plain weights are kept exactly."""
+    dev = np.abs(np.asarray(wp) - GATE_IDEAL).sum(axis=1)
+    return np.minimum(1.0, np.maximum(0.0, dev - GATE_OFF) / GATE_DEV)
 
 
 def forward(params, S, bf16_points=True):
@@ -81,7 +83,7 @@ def forward(params, S, bf16_points=True):
     g = gate(wp)
     zg = g[:, None] * z
     u = wp * np.exp(zg - zg.max(axis=1, keepdims=True))
-    w = u / u.sum(axis=1, keepdims=True)
+    w = np.where((g == 0.0)[:, None], wp, u / u.sum(axis=1, keepdims=True))
     y = (w * p).sum(axis=1)
     loss = np.mean((y - t) ** 2)
     return dict(x=xr, a1=a1, h1=h1, a2=a2, h2=h2, z=z, g=g, w=w, y=y, p=p, t=t, loss=loss,
