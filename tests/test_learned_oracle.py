@@ -5,7 +5,8 @@
   textbook third-order polynomials of each face (this fixes the stencil
   orientation of both faces, independently of the code under test);
 * equal logits give the plain WENO5 value, and so do any logits where the
-  plain weights are ideal (the gate on their distance from ideal closes);
+  plain weights are ideal (the gate on their distance from ideal closes) or
+  where the stencil's variation is rounding noise;
 * the per-face fallback keeps the plain value where the learned one breaks a
   bound the plain one keeps;
 * the C forward pass agrees with the independent numpy MLP oracle;
@@ -73,6 +74,21 @@ def test_gate_closes_at_ideal_weights(learned):
         ll, lr = O.weno5_learned_cell(1e10, cf, v)
         assert pr == pytest.approx(linear_r, rel=1e-8)
         assert lr == pytest.approx(pr, rel=1e-6) and ll == pytest.approx(pl, rel=1e-6)
+
+
+def test_rounding_noise_keeps_plain_weno5(learned):
+    """A stencil whose variation is below 1e-6 of its magnitude (the rule
+    that admits training pairs) keeps the plain weights whatever the logits,
+    while the same shape at a resolvable amplitude does not."""
+    cf = _coef()
+    learned(_bias_only([80.0, 0.0, -80.0]))
+    shape = np.array([0.3, -0.2, 0.7, 1.1, 0.4])
+    for amp, plain_expected in ((1e-8, True), (1e-2, False)):
+        v = 1.0 + amp * shape
+        pl, pr = O.weno_cell(5, 1e-30, cf, v)
+        ll, lr = O.weno5_learned_cell(1e-30, cf, v)
+        same = lr == pytest.approx(pr, rel=1e-15, abs=0) and ll == pytest.approx(pl, rel=1e-15, abs=0)
+        assert same == plain_expected, amp
 
 
 def test_equal_logits_give_plain_weno5(learned):
