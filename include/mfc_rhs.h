@@ -147,6 +147,14 @@ int mfc_ml_restrict(int m, int n, int b, const double *fine, double *coarse);
  * WENO5 coefficients of the uniform coarse grid (mfc_weno_coefficients layout). */
 int mfc_ml_samples(int dir, int m, int n, int b, const double *coarse, const double *fine, const double *coef27,
                    double eps, float *out);
+/* The same for 1D, 2D and 3D fields (p = 0 below 3D): the 3D restriction
+ * averages 2x2x2 fine cells, and a 3D sample's target averages the 4 fine
+ * faces that tile the coarse face; dir 2 is z.  out has 13 planes of
+ * 2(m+1)(n+1)(p+1) floats.  mfc_ml_restrict and mfc_ml_samples are these
+ * with p = 0. */
+int mfc_ml_restrict3(int m, int n, int p, int b, const double *fine, double *coarse);
+int mfc_ml_samples3(int dir, int m, int n, int p, int b, const double *coarse, const double *fine,
+                    const double *coef27, double eps, float *out);
 const char *mfc_mlp_last_error(void);
 
 /* Per-kernel timing with HIP events recorded on the library's stream around

@@ -370,53 +370,62 @@ __device__ __forceinline__ double weno_r_value(const double *v, const Coef27 &cf
 }
 
 // One thread per (coarse interior cell, side).  coarse: one variable of the
-// coarse ghosted field ((m+1) x (n+1) interior, b ghosts, BC-filled); fine:
-// the same variable on the 2x finer grid ((2m+2) x (2n+2) interior when
-// n > 0).  The face value target is the mean over the fine cells along the
-// face of the fine WENO5 face values from both sides.  Output: 13 planes
-// (x[5], p[3], w_plain[3], t, valid) of 2 (m+1)(n+1) samples; sample 2c is
-// the right face of coarse cell c, 2c+1 its left face (stencil reversed).
-__global__ void k_ml_samples(int dir, int m, int n, int b, const double *coarse, const double *fine, Coef27 cf,
-                             double eps, float *out) {
-    const long ncell = (long)(m + 1) * (n + 1);
+// coarse ghosted field ((m+1) x (n+1) x (p+1) interior, b ghosts in every
+// direction present, BC-filled); fine: the same variable on the 2x finer
+// grid.  The face value target is the mean over the 1, 2 or 4 fine faces
+// that tile the coarse face of the fine WENO5 face values from both sides.
+// Output: 13 planes (x[5], p[3], w_plain[3], t, valid) of
+// 2 (m+1)(n+1)(p+1) samples; sample 2c is the right face of coarse cell c,
+// 2c+1 its left face (stencil reversed).
+__global__ void k_ml_samples(int dir, int m, int n, int p, int b, const double *coarse, const double *fine,
+                             Coef27 cf, double eps, float *out) {
+    const long ncell = (long)(m + 1) * (n + 1) * (p + 1);
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * ncell) return;
     const long c = t >> 1;
     const int side = (int)(t & 1);
-    const int i = (int)(c % (m + 1)), jj = (int)(c / (m + 1));
-    const int bj = n > 0 ? b : 0;
-    const long cnx = m + 2 * b + 1;
-    const int fm = 2 * m + 1;
-    const long fnx = fm + 2 * b + 1;
-    auto C = [&](int x, int y) { return coarse[(long)(x + b) + cnx * (long)(y + bj)]; };
-    auto Fv = [&](int x, int y) { return fine[(long)(x + b) + fnx * (long)(y + bj)]; };
+    const int i = (int)(c % (m + 1)), jj = (int)((c / (m + 1)) % (n + 1)), kk = (int)(c / ((long)(m + 1) * (n + 1)));
+    const int bj = n > 0 ? b : 0, bk = p > 0 ? b : 0;
+    const long cnx = m + 2 * b + 1, cny = n + 2 * bj + 1;
+    const long fnx = 2 * m + 1 + 2 * b + 1, fny = n > 0 ? 2 * n + 1 + 2 * b + 1 : 1;
+    auto C = [&](int x, int y, int z) {
+        return coarse[(long)(x + b) + cnx * ((long)(y + bj) + cny * (long)(z + bk))];
+    };
+    auto Fv = [&](int x, int y, int z) {
+        return fine[(long)(x + b) + fnx * ((long)(y + bj) + fny * (long)(z + bk))];
+    };
     // coarse stencil along dir, oriented so the face is between v[2] and v[3]
     double v[5];
     for (int k = 0; k < 5; ++k) {
         const int o = side == 0 ? k - 2 : 2 - k;
-        v[k] = dir == 0 ? C(i + o, jj) : C(i, jj + o);
+        v[k] = dir == 0 ? C(i + o, jj, kk) : dir == 1 ? C(i, jj + o, kk) : C(i, jj, kk + o);
     }
     // fine face: right face of coarse cell i is between fine cells 2i+1 and
-    // 2i+2; left face between 2i and 2i-1 (oriented the same way)
-    const int along = dir == 0 ? i : jj;
+    // 2i+2; left face between 2i and 2i-1 (oriented the same way).  Fine
+    // cell at index `x` along dir and transverse offset r (bit 0: the first
+    // transverse direction present, bit 1: the second).
+    const int along = dir == 0 ? i : dir == 1 ? jj : kk;
     const int fa = side == 0 ? 2 * along + 1 : 2 * along; // fine cell on the v[2] side
     const int sgn = side == 0 ? 1 : -1;
-    const int nrow = (dir == 0 && n == 0) ? 1 : 2; // fine cells along the face
-    const int other = dir == 0 ? jj : i;
+    const int nrow = 1 << ((n > 0) + (p > 0));              // fine faces on the coarse face
+    auto F = [&](int x, int r) {
+        const int t1 = r & 1, t2 = r >> 1;
+        if (dir == 0) return Fv(x, n > 0 ? 2 * jj + t1 : 0, p > 0 ? 2 * kk + t2 : 0);
+        if (dir == 1) return Fv(2 * i + t1, x, p > 0 ? 2 * kk + t2 : 0);
+        return Fv(2 * i + t1, 2 * jj + t2, x);
+    };
     double tsum = 0.0;
     for (int r = 0; r < nrow; ++r) {
-        const int fo = (dir == 0 && n == 0) ? 0 : 2 * other + r;
         double a[5], bb[5];
         for (int k = 0; k < 5; ++k) {
-            const int xa = fa + sgn * (k - 2), xb = fa + sgn * (1 + 2 - k);
-            a[k] = dir == 0 ? Fv(xa, fo) : Fv(fo, xa);
-            bb[k] = dir == 0 ? Fv(xb, fo) : Fv(fo, xb);
+            a[k] = F(fa + sgn * (k - 2), r);
+            bb[k] = F(fa + sgn * (1 + 2 - k), r);
         }
         tsum += 0.5 * (weno_r_value(a, cf, eps) + weno_r_value(bb, cf, eps));
     }
     const double tface = tsum / nrow;
-    double p[3], w[3];
-    weno_r_parts(v, cf, eps, p, w);
+    double pc[3], w[3];
+    weno_r_parts(v, cf, eps, pc, w);
     double s = 0.0, vmax = 0.0;
     for (int k = 0; k < 4; ++k) s = fmax(s, fabs(v[k + 1] - v[k]));
     for (int k = 0; k < 5; ++k) vmax = fmax(vmax, fabs(v[k]));
@@ -425,7 +434,7 @@ __global__ void k_ml_samples(int dir, int m, int n, int b, const double *coarse,
     const double inv = valid ? 1.0 / s : 0.0;
     for (int k = 0; k < 5; ++k) out[(long)k * ns + t] = (float)((v[k] - v[2]) * inv);
     for (int k = 0; k < 3; ++k) {
-        out[(long)(5 + k) * ns + t] = (float)((p[k] - v[2]) * inv);
+        out[(long)(5 + k) * ns + t] = (float)((pc[k] - v[2]) * inv);
         out[(long)(8 + k) * ns + t] = (float)w[k];
     }
     out[11L * ns + t] = (float)((tface - v[2]) * inv);
@@ -433,21 +442,30 @@ __global__ void k_ml_samples(int dir, int m, int n, int b, const double *coarse,
 }
 
 // Coarse cell averages of one variable: each coarse interior cell is the
-// mean of the 2 (1D) or 2x2 (2D) fine cells it covers.
-__global__ void k_ml_restrict(int m, int n, int b, const double *fine, double *coarse) {
-    const long ncell = (long)(m + 1) * (n + 1);
+// mean of the 2 (1D), 2x2 (2D) or 2x2x2 (3D) fine cells it covers.
+__global__ void k_ml_restrict(int m, int n, int p, int b, const double *fine, double *coarse) {
+    const long ncell = (long)(m + 1) * (n + 1) * (p + 1);
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    const int i = (int)(c % (m + 1)), jj = (int)(c / (m + 1));
-    const int bj = n > 0 ? b : 0;
-    const long cnx = m + 2 * b + 1, fnx = 2 * m + 1 + 2 * b + 1;
-    auto Fv = [&](int x, int y) { return fine[(long)(x + b) + fnx * (long)(y + bj)]; };
+    const int i = (int)(c % (m + 1)), jj = (int)((c / (m + 1)) % (n + 1)), kk = (int)(c / ((long)(m + 1) * (n + 1)));
+    const int bj = n > 0 ? b : 0, bk = p > 0 ? b : 0;
+    const long cnx = m + 2 * b + 1, cny = n + 2 * bj + 1;
+    const long fnx = 2 * m + 1 + 2 * b + 1, fny = n > 0 ? 2 * n + 1 + 2 * b + 1 : 1;
+    auto Fv = [&](int x, int y, int z) {
+        return fine[(long)(x + b) + fnx * ((long)(y + bj) + fny * (long)(z + bk))];
+    };
     double v;
-    if (n > 0)
-        v = 0.25 * ((Fv(2 * i, 2 * jj) + Fv(2 * i + 1, 2 * jj)) + (Fv(2 * i, 2 * jj + 1) + Fv(2 * i + 1, 2 * jj + 1)));
-    else
-        v = 0.5 * (Fv(2 * i, 0) + Fv(2 * i + 1, 0));
-    coarse[(long)(i + b) + cnx * (long)(jj + bj)] = v;
+    if (p > 0) {
+        const int x = 2 * i, y = 2 * jj, z = 2 * kk;
+        v = 0.125 * (((Fv(x, y, z) + Fv(x + 1, y, z)) + (Fv(x, y + 1, z) + Fv(x + 1, y + 1, z))) +
+                     ((Fv(x, y, z + 1) + Fv(x + 1, y, z + 1)) + (Fv(x, y + 1, z + 1) + Fv(x + 1, y + 1, z + 1))));
+    } else if (n > 0) {
+        v = 0.25 * ((Fv(2 * i, 2 * jj, 0) + Fv(2 * i + 1, 2 * jj, 0)) +
+                    (Fv(2 * i, 2 * jj + 1, 0) + Fv(2 * i + 1, 2 * jj + 1, 0)));
+    } else {
+        v = 0.5 * (Fv(2 * i, 0, 0) + Fv(2 * i + 1, 0, 0));
+    }
+    coarse[(long)(i + b) + cnx * ((long)(jj + bj) + cny * (long)(kk + bk))] = v;
 }
 
 int mlp_require() {
@@ -551,29 +569,41 @@ int mfc_mlp_infer(const float *samples, long n, float *out) {
     return 0;
 }
 
-int mfc_ml_restrict(int m, int n, int b, const double *fine, double *coarse) {
-    if (m < 0 || n < 0 || b < 0 || !fine || !coarse) return mfail(-1, "bad arguments");
+int mfc_ml_restrict3(int m, int n, int p, int b, const double *fine, double *coarse) {
+    if (m < 0 || n < 0 || p < 0 || (p > 0 && n == 0) || b < 0 || !fine || !coarse) return mfail(-1, "bad arguments");
     hipStream_t st = M.init ? M.stream : nullptr;
-    const long ncell = (long)(m + 1) * (n + 1);
-    hipLaunchKernelGGL(k_ml_restrict, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, m, n, b, fine, coarse);
+    const long ncell = (long)(m + 1) * (n + 1) * (p + 1);
+    hipLaunchKernelGGL(k_ml_restrict, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, m, n, p, b, fine,
+                       coarse);
     MHIP(hipGetLastError());
     MHIP(hipStreamSynchronize(st));
     return 0;
 }
 
-int mfc_ml_samples(int dir, int m, int n, int b, const double *coarse, const double *fine,
-                   const double *coef27, double eps, float *out) {
-    if (dir < 0 || dir > 1 || (dir == 1 && n == 0) || b < 4 || !coarse || !fine || !coef27 || !out)
+int mfc_ml_restrict(int m, int n, int b, const double *fine, double *coarse) {
+    return mfc_ml_restrict3(m, n, 0, b, fine, coarse);
+}
+
+int mfc_ml_samples3(int dir, int m, int n, int p, int b, const double *coarse, const double *fine,
+                    const double *coef27, double eps, float *out) {
+    if (dir < 0 || dir > 2 || (dir == 1 && n == 0) || (dir == 2 && p == 0) || (p > 0 && n == 0) || b < 4 ||
+        !coarse || !fine || !coef27 || !out)
         return mfail(-1, "bad arguments");
     Coef27 cf;
     for (int q = 0; q < kNCoef; ++q) cf.c[q] = coef27[q];
     hipStream_t st = M.init ? M.stream : nullptr;
-    const long ns = 2L * (m + 1) * (n + 1);
-    hipLaunchKernelGGL(k_ml_samples, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, dir, m, n, b, coarse, fine,
-                       cf, eps, out);
+    const long ns = 2L * (m + 1) * (n + 1) * (p + 1);
+    hipLaunchKernelGGL(k_ml_samples, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, dir, m, n, p, b, coarse,
+                       fine, cf, eps, out);
     MHIP(hipGetLastError());
     MHIP(hipStreamSynchronize(st));
     return 0;
+}
+
+int mfc_ml_samples(int dir, int m, int n, int b, const double *coarse, const double *fine, const double *coef27,
+                   double eps, float *out) {
+    if (dir > 1) return mfail(-1, "bad arguments");
+    return mfc_ml_samples3(dir, m, n, 0, b, coarse, fine, coef27, eps, out);
 }
 
 } // extern "C"

@@ -38,6 +38,9 @@ def _lib():
         L.mfc_ml_restrict.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.mfc_ml_samples.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                      ctypes.c_void_p, dp, ctypes.c_double, ctypes.c_void_p]
+        L.mfc_ml_restrict3.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p, ctypes.c_void_p]
+        L.mfc_ml_samples3.argtypes = [ctypes.c_int] * 5 + [ctypes.c_void_p, ctypes.c_void_p, dp, ctypes.c_double,
+                                                           ctypes.c_void_p]
         L.mfc_mlp_last_error.restype = ctypes.c_char_p
         L._mlp_declared = True
     return L
@@ -123,28 +126,29 @@ def _torch_done():
     torch.cuda.current_stream().synchronize()
 
 
-def restrict(m: int, n: int, b: int, fine_var, coarse_var):
+def restrict(m: int, n: int, b: int, fine_var, coarse_var, p: int = 0):
     _torch_done()
-    _check(_lib().mfc_ml_restrict(m, n, b, ctypes.c_void_p(fine_var.data_ptr()), ctypes.c_void_p(coarse_var.data_ptr())),
-           "mfc_ml_restrict")
+    _check(_lib().mfc_ml_restrict3(m, n, p, b, ctypes.c_void_p(fine_var.data_ptr()),
+                                   ctypes.c_void_p(coarse_var.data_ptr())), "mfc_ml_restrict3")
 
 
-def samples(direction: int, m: int, n: int, b: int, coarse_var, fine_var, coef27, eps: float):
+def samples(direction: int, m: int, n: int, b: int, coarse_var, fine_var, coef27, eps: float, p: int = 0):
     import torch
-    ns = 2 * (m + 1) * (n + 1)
+    ns = 2 * (m + 1) * (n + 1) * (p + 1)
     out = torch.zeros((13, ns), dtype=torch.float32, device="cuda")
     cf = np.ascontiguousarray(coef27, dtype=np.float64)
     _torch_done()
-    _check(_lib().mfc_ml_samples(direction, m, n, b, ctypes.c_void_p(coarse_var.data_ptr()),
-                                 ctypes.c_void_p(fine_var.data_ptr()),
-                                 cf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), eps,
-                                 ctypes.c_void_p(out.data_ptr())), "mfc_ml_samples")
+    _check(_lib().mfc_ml_samples3(direction, m, n, p, b, ctypes.c_void_p(coarse_var.data_ptr()),
+                                  ctypes.c_void_p(fine_var.data_ptr()),
+                                  cf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), eps,
+                                  ctypes.c_void_p(out.data_ptr())), "mfc_ml_samples3")
     return out
 
 
 def coarse_params(fine: Params) -> Params:
     """The 2x coarser case of a fine one (same boundaries and fluids)."""
-    return Params(m=(fine.m + 1) // 2 - 1, n=(fine.n + 1) // 2 - 1 if fine.n > 0 else 0, p=0,
+    return Params(m=(fine.m + 1) // 2 - 1, n=(fine.n + 1) // 2 - 1 if fine.n > 0 else 0,
+                  p=(fine.p + 1) // 2 - 1 if fine.p > 0 else 0,
                   num_fluids=fine.num_fluids, weno_order=5, time_stepper=fine.time_stepper, weno_eps=fine.weno_eps,
                   gammas=list(fine.gammas), pi_infs=list(fine.pi_infs), bc_beg=tuple(fine.bc_beg),
                   bc_end=tuple(fine.bc_end), precision=0)
@@ -169,7 +173,7 @@ def training_pairs(fine_prm: Params, fine_grids, q_fine, nsnap: int, steps_betwe
     directions and every variable."""
     import torch
     from . import ic
-    assert fine_prm.p == 0 and fine_prm.weno_order == 5
+    assert fine_prm.weno_order == 5
     fine = RhsModule(fine_prm, fine_grids)
     q1 = torch.from_numpy(q_fine).cuda()
     q2 = fine.new_field()
@@ -198,14 +202,14 @@ def training_pairs(fine_prm: Params, fine_grids, q_fine, nsnap: int, steps_betwe
         for qf, pf in snaps:
             qc = coarse.new_field()
             for v in range(cprm.sys_size):
-                restrict(cprm.m, cprm.n, b, qf[v], qc[v])
+                restrict(cprm.m, cprm.n, b, qf[v], qc[v], p=cprm.p)
             pc = coarse.new_field()
             rc = coarse.new_field(ghosted=False)
             coarse.s_compute_rhs(qc, pc, rc, 0)
             coarse.sync()
             for d in range(cprm.num_dims):
                 for v in range(cprm.sys_size):
-                    S = samples(d, cprm.m, cprm.n, b, pc[v], pf[v], coef, cprm.weno_eps)
+                    S = samples(d, cprm.m, cprm.n, b, pc[v], pf[v], coef, cprm.weno_eps, p=cprm.p)
                     S = S[:12, keep(S)]
                     if max_per_snapshot and S.shape[1] > max_per_snapshot:
                         idx = torch.from_numpy(rng.choice(S.shape[1], max_per_snapshot, replace=False)).cuda()

@@ -36,7 +36,7 @@ EXPORTED = [
     "mfc_set_stream", "mfc_rhs_finalize", "mfc_last_error",
     "mfc_timing_enable", "mfc_timing_reset", "mfc_timing_read",
     "mfc_mlp_init", "mfc_mlp_get", "mfc_mlp_gradient", "mfc_mlp_train_step", "mfc_mlp_infer",
-    "mfc_ml_restrict", "mfc_ml_samples", "mfc_mlp_last_error",
+    "mfc_ml_restrict", "mfc_ml_samples", "mfc_ml_restrict3", "mfc_ml_samples3", "mfc_mlp_last_error",
 ]
 KERNEL_IDS = {"bc_fill": 0, "cons_to_prim": 1, "sweep_x": 2, "sweep_y": 3, "sweep_z": 4, "icfl": 5,
               "rk1_update": 6}

@@ -93,6 +93,48 @@ def test_training_pairs_match_oracle_1d(gpu):
     np.testing.assert_array_equal(cc.cpu().numpy()[4:-4], coarse[4:-4])
 
 
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_training_pairs_3d_reduce_to_1d(gpu, d):
+    """A 3D field that varies only along direction d: each of its samples in
+    that direction equals the oracle's 1D sample of the profile (the four
+    fine faces that tile a coarse face carry one value), and its 3D
+    restriction is the 1D one, bit for bit."""
+    torch = gpu
+    prm, _, _ = ic.sod_1d(m=63)
+    cprm = learned.coarse_params(prm)
+    rng = np.random.default_rng(1 + d)
+    prof = np.cumsum(rng.normal(size=64)) + 0.1 * rng.normal(size=64)
+    cprof = 0.5 * (prof[0::2] + prof[1::2])
+    fine1, coarse1 = np.pad(prof, 4, mode="edge"), np.pad(cprof, 4, mode="edge")
+    coef = O_coef(ic.uniform_grids(cprm)[0][0], cprm)
+    ref = Q.samples_1d(coarse1, fine1, coef)
+
+    def field(profile, nt):
+        shape = [nt, nt, nt]
+        shape[2 - d] = len(profile)
+        ax = [1, 1, 1]
+        ax[2 - d] = len(profile)
+        a = np.broadcast_to(profile.reshape(ax), shape)
+        return np.ascontiguousarray(np.pad(a, 4, mode="edge"))
+
+    fine3, coarse3 = field(prof, 4), field(cprof, 2)
+    ext = [1, 1, 1]
+    ext[d] = 31
+    cm, cn, cp = ext
+    got = learned.samples(d, cm, cn, 4, torch.from_numpy(coarse3).cuda(), torch.from_numpy(fine3).cuda(), coef,
+                          1e-16, p=cp).cpu().numpy()
+    got = got.reshape(13, cp + 1, cn + 1, cm + 1, 2)
+    for a in range(2):
+        for b2 in range(2):
+            sel = [(slice(None), a, b2, slice(None)), (slice(None), a, slice(None), b2),
+                   (slice(None), slice(None), a, b2)][d]
+            line = got[sel].reshape(13, -1)
+            np.testing.assert_allclose(line, ref.astype(np.float32), rtol=1e-6, atol=1e-6)
+    cc = torch.zeros(coarse3.shape, dtype=torch.float64, device="cuda")
+    learned.restrict(cm, cn, 4, torch.from_numpy(fine3).cuda(), cc, p=cp)
+    np.testing.assert_array_equal(cc.cpu().numpy()[4:-4, 4:-4, 4:-4], coarse3[4:-4, 4:-4, 4:-4])
+
+
 def O_coef(cb, cprm):
     from oracle import oracle as O
     return O.weno_coefficients(cb, cprm.m, cprm.buff_size, 5)[:, cprm.buff_size]
