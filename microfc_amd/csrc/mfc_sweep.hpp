@@ -14,7 +14,8 @@
 //    staged in LDS; lane l computes face x0-1/2+l; fluxes go through LDS so
 //    lane l can difference faces l and l+1 for cell x0+l (63 or 62 cells per
 //    wave, face- or cell-parallel reconstruction).
-//  * k_sweep_t (y or z, strided, face-parallel): one 256-thread block per
+//  * k_sweep_t (y or z, strided, face-parallel; FP64 with per-cell
+//    coefficients, i.e. non-uniform grids): one 256-thread block per
 //    tile of 16 x-columns by 16 faces along the sweep (15 cells).  Stencil
 //    rows of 16 consecutive x are staged in LDS, thread (x, f) solves face f
 //    of column x, and the fluxes go back through LDS for the divergence.
@@ -332,10 +333,10 @@ __device__ __forceinline__ float ml_value(int var, bool open, const float *z, co
     return ok ? learned : plain;
 }
 
-// Both face states of one variable for the wave's 64 faces.  stR: stencil of
-// the cell left of the lane's face (cells j-2..j+2), stL: of the cell right
-// of it (cells j-1..j+3); cfR/cfL their coefficients; xs the wave's feature
-// rows.
+// Learned face values of one variable for the wave's 64 lanes: the value at
+// the upper face of stencil stR (cells j-2..j+2, coefficients cfR) and at
+// the lower face of stencil stL (cfL); the cell-parallel sweeps pass one
+// cell's stencil as both.  xs is the wave's feature rows.
 template <int NF, int D>
 __device__ __forceinline__ void ml_face_states(int var, const float *stR, const float *stL, const float *cfR,
                                                const float *cfL, float eps, const MlpTRegs &T,
@@ -365,10 +366,10 @@ __device__ __forceinline__ void ml_face_states(int var, const float *stR, const 
 }
 
 // ---------------------------------------------------------------------------
-// x sweep: one wave per tile of a row.  Face-parallel (the learned form, and
-// FP64 on non-uniform grids): lane L reconstructs the two sides of face
-// x0-1+L + 1/2 from the stencils of its two cells; 63 cells per tile are
-// finished.  Cell-parallel (FP32 bulk; FP64 with the uniform-grid
+// x sweep: one wave per tile of a row.  Face-parallel (FP64 on non-uniform
+// grids): lane L reconstructs the two sides of face x0-1+L + 1/2 from the
+// stencils of its two cells; 63 cells per tile are finished.  Cell-parallel
+// (FP32 bulk, the learned form, and FP64 with the uniform-grid
 // coefficients in scalar registers): lane L reconstructs both faces of cell
 // x0-1+L (the smoothness indicators and the reciprocals of their squares
 // serve both), the left-face values move one lane down through LDS, and
@@ -377,7 +378,7 @@ __device__ __forceinline__ void ml_face_states(int var, const float *stR, const 
 // and no faster for FP64 with per-cell coefficient loads, whose registers
 // then cost a wave of occupancy.
 template <bool ML, typename R, bool UNI>
-constexpr bool x_cell_parallel() { return !ML && (sizeof(R) == 4 || UNI); }
+constexpr bool x_cell_parallel() { return ML || sizeof(R) == 4 || UNI; }
 template <bool ML, typename R, bool UNI>
 constexpr int x_tile_cells() { return x_cell_parallel<ML, R, UNI>() ? 62 : 63; }
 
@@ -446,34 +447,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
     const int jf = x0 - 1 + lane;
     R qL[S], qR[S];
     if constexpr (ML) {
-        // every lane takes part in the wave's MFMA passes, valid face or not;
-        // one variable at a time, results through LDS: the left states over
-        // the variable's own staged row (the block is one wave, and each
-        // value is written after the reads it depends on), the right states
-        // over the flux rows, which each lane reads back before writing
+        // cell-parallel like the plain forms: lane = cell jf, whose upper-face
+        // value is face jf+1/2's left state and whose lower-face value is
+        // face jf-1/2's right state, read back one lane over.  Every lane
+        // takes part in the wave's MFMA passes, valid cell or not; one
+        // variable at a time, results through LDS over the variable's own
+        // staged row and the flux rows (the block is one wave, and each value
+        // is written after the reads it depends on)
         __shared__ unsigned short xs[2][64][8];
         MlpTRegs T;
         mlp_load_tregs(a.mlp, lane, T);
-        R cfR[kNCoef], cfL[kNCoef];
-        load_coef<R, WO, UNI>(a, min(jf, a.m), cfR);
-        load_coef<R, WO, UNI>(a, min(jf, a.m) + 1, cfL);
+        R cf[kNCoef];
+        load_coef<R, WO, UNI>(a, min(jf, a.m + 1), cf);
         const R eps = (R)a.eps;
 #pragma unroll 1
         for (int v = 0; v < S; ++v) {
-            R stR[5], stL[5], l, r;
+            R st[5], vr, vl;
 #pragma unroll
-            for (int w = 0; w < 5; ++w) {
-                stR[w] = sp[v][lane + w];
-                stL[w] = sp[v][lane + 1 + w];
-            }
-            ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, T, xs, lane, l, r);
-            sp[v][lane] = l;
-            sf[v][lane] = r;
+            for (int w = 0; w < 5; ++w) st[w] = sp[v][lane + w];
+            ml_face_states<NF, D>(v, st, st, cf, cf, eps, T, xs, lane, vr, vl);
+            sp[v][lane] = vl;
+            sf[v][lane] = vr;
         }
+        __syncthreads();
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            qL[v] = sp[v][lane];
-            qR[v] = sf[v][lane];
+            qL[v] = sf[v][lane];
+            qR[v] = sp[v][min(lane + 1, 63)];
         }
     } else if constexpr (x_cell_parallel<ML, R, UNI>()) {
         // both reconstructions of cell jf (coefficients of a cell past the
@@ -589,6 +589,7 @@ constexpr int t_tile_cells(int dir) {
 template <int NF, int D, int DIR, int WO, typename R, bool ML, bool UNI>
 __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_t(
     const SweepArgs a) {
+    static_assert(!ML, "the learned form runs the cell-parallel sweep");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int kTS = t_faces<ML>();
@@ -600,9 +601,8 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
     constexpr int LDS_BYTES = sizeof(R) * (PRIM_WORDS > FLUX_WORDS ? PRIM_WORDS : FLUX_WORDS);
     // The conservative stage input of the tile's own cells stays in LDS for
     // the finish (rhs accumulation, RK stage) instead of being read from HBM
-    // a second time, while the block still fits four to a CU (not in the
-    // learned form, whose MLP staging takes that LDS).
-    constexpr bool KEEP = !ML && LDS_BYTES + 8 * S * (kTS - 1) * kTX <= 40 * 1024 * (NT / 256);
+    // a second time, while the block still fits four to a CU.
+    constexpr bool KEEP = LDS_BYTES + 8 * S * (kTS - 1) * kTX <= 40 * 1024 * (NT / 256);
     __shared__ R lds[PRIM_WORDS > FLUX_WORDS ? PRIM_WORDS : FLUX_WORDS];
     __shared__ double lcons[KEEP ? S : 1][kTS - 1][kTX];
     R(*sp)[NR][kTX] = reinterpret_cast<R(*)[NR][kTX]>(lds);
@@ -692,50 +692,20 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
     R f[S], us = R(0);
     const bool face_ok = jf <= s;
     R qL[S], qR[S];
-    if constexpr (ML) {
-        __shared__ unsigned short xs[4][2][64][8];
-        __shared__ R mres[2][S][256];
-        const int lane = tid & 63, wv = tid >> 6;
-        MlpTRegs T;
-        mlp_load_tregs(a.mlp, lane, T);
-        R cfR[kNCoef], cfL[kNCoef];
-        load_coef<R, WO, UNI>(a, min(jf, s), cfR);
-        load_coef<R, WO, UNI>(a, min(jf, s) + 1, cfL);
-        const R eps = (R)a.eps;
-#pragma unroll 1
-        for (int v = 0; v < S; ++v) {
-            R stR[5], stL[5], l, r;
-#pragma unroll
-            for (int w = 0; w < 5; ++w) {
-                stR[w] = sp[v][ts + w][tx];
-                stL[w] = sp[v][ts + 1 + w][tx];
-            }
-            ml_face_states<NF, D>(v, stR, stL, cfR, cfL, eps, T, xs[wv], lane, l, r);
-            mres[0][v][tid] = l;
-            mres[1][v][tid] = r;
-        }
-#pragma unroll
-        for (int v = 0; v < S; ++v) {
-            qL[v] = mres[0][v][tid];
-            qR[v] = mres[1][v][tid];
-        }
-    }
     if (face_ok) {
-        if constexpr (!ML) {
-            R cfR[kNCoef], cfL[kNCoef];
-            load_coef<R, WO, UNI>(a, jf, cfR);
-            load_coef<R, WO, UNI>(a, jf + 1, cfL);
-            const R eps = (R)a.eps;
+        R cfR[kNCoef], cfL[kNCoef];
+        load_coef<R, WO, UNI>(a, jf, cfR);
+        load_coef<R, WO, UNI>(a, jf + 1, cfL);
+        const R eps = (R)a.eps;
 #pragma unroll
-            for (int v = 0; v < S; ++v) {
-                R st[WO], dummy;
+        for (int v = 0; v < S; ++v) {
+            R st[WO], dummy;
 #pragma unroll
-                for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + w][tx];
-                weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
+            for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + w][tx];
+            weno_cell<WO, R, false, true>(st, cfR, eps, dummy, qL[v]);
 #pragma unroll
-                for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + 1 + w][tx];
-                weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
-            }
+            for (int w = 0; w < WO; ++w) st[w] = sp[v][ts + 1 + w][tx];
+            weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
         }
         if (jf == -1 && a.bc_beg == -4) {
 #pragma unroll
@@ -791,14 +761,14 @@ __device__ __forceinline__ R row_next(R x) {
     return __shfl_down(x, 1u, W);
 }
 
-template <int NF, int D, int DIR, int WO, typename R, bool UNI>
-__global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR)) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_tc(
-    const SweepArgs a) {
-    static_assert(t_cell_parallel<false, R, UNI>(), "cell-parallel forms only");
+template <int NF, int D, int DIR, int WO, typename R, bool ML, bool UNI>
+__global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
+    __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 8))) void k_sweep_tc(const SweepArgs a) {
+    static_assert(t_cell_parallel<ML, R, UNI>(), "cell-parallel forms only");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int XC = tc_cols<R>(DIR), kTS = tc_rows<R>(DIR), NT = XC * kTS;
-    constexpr int TC = t_tile_cells<false, R, UNI>(DIR);
+    constexpr int TC = t_tile_cells<ML, R, UNI>(DIR);
     constexpr int PN = (WO - 1) / 2;
     constexpr int NR = kTS + 2 * PN; // staged cells y0-1-PN .. y0+kTS-2+PN
     constexpr int P = NR | 1;        // odd column pitches
@@ -808,8 +778,9 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR)) __attribute__((a
     constexpr int WORDS = SP_WORDS > FL_WORDS ? SP_WORDS : FL_WORDS;
     // the stage input of the finished cells stays in LDS for the finish
     // while four 256-thread blocks (or the same threads in fewer blocks)
-    // still fit a CU
-    constexpr bool KEEP = sizeof(R) * WORDS + 8 * S * XC * KP <= 40 * 1024 * (NT / 256);
+    // still fit a CU (not in the learned form, whose MLP staging takes
+    // that LDS)
+    constexpr bool KEEP = !ML && sizeof(R) * WORDS + 8 * S * XC * KP <= 40 * 1024 * (NT / 256);
     __shared__ R lds[WORDS];
     __shared__ double lcons[KEEP ? S : 1][XC][KEEP ? KP : 1];
     R(*sp)[XC][P] = reinterpret_cast<R(*)[XC][P]>(lds);
@@ -871,7 +842,33 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR)) __attribute__((a
     // 2. cell jf = y0-1+ts: both faces; face jf + 1/2
     const int jf = y0 - 1 + ts;
     R qL[S], qR[S];
-    {
+    if constexpr (ML) {
+        // learned: one variable at a time (the wave's MFMA passes take
+        // every lane), each thread's two face values through its own LDS
+        // slots; the lower-face value then moves one lane down the column
+        __shared__ unsigned short xs[NT / 64][2][64][8];
+        __shared__ R mres[2][S][NT];
+        const int lane = tid & 63, wv = tid >> 6;
+        MlpTRegs T;
+        mlp_load_tregs(a.mlp, lane, T);
+        R cf[kNCoef];
+        load_coef<R, WO, UNI>(a, min(jf, s + 1), cf);
+        const R eps = (R)a.eps;
+#pragma unroll 1
+        for (int v = 0; v < S; ++v) {
+            R st[5], vr, vl;
+#pragma unroll
+            for (int w = 0; w < 5; ++w) st[w] = sp[v][tx][ts + w];
+            ml_face_states<NF, D>(v, st, st, cf, cf, eps, T, xs[wv], lane, vr, vl);
+            mres[0][v][tid] = vr;
+            mres[1][v][tid] = vl;
+        }
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            qL[v] = mres[0][v][tid];
+            qR[v] = row_next<kTS>(mres[1][v][tid]);
+        }
+    } else {
         R cf[kNCoef];
         load_coef<R, WO, UNI>(a, min(jf, s + 1), cf);
         const R eps = (R)a.eps;
