@@ -10,7 +10,7 @@ import pytest
 
 from microfc_amd import PREC_FP32_BULK, PREC_FP32_LEARNED, RhsModule, ic, learned
 
-from helpers import perturbed, say, scaled_linf, to_oracle
+from helpers import perturbed, say, scaled_linf, stretched_grids, to_oracle
 from oracle import oracle as O
 from test_mlp_oracle import random_params
 
@@ -30,8 +30,13 @@ def _rhs(torch, prm, grids, q):
 
 
 def _cases():
-    return {"bubble": ic.shock_bubble_2d(N=64), "shear": ic.shear_layer_2d(N=64), "sod": ic.sod_1d(m=199),
-            "droplet3d": ic.shock_droplet_3d(N=28), "cloud3d": ic.bubble_cloud_3d(N=32, nbubbles=16)}
+    out = {"bubble": ic.shock_bubble_2d(N=64), "shear": ic.shear_layer_2d(N=64), "sod": ic.sod_1d(m=199),
+           "droplet3d": ic.shock_droplet_3d(N=28), "cloud3d": ic.bubble_cloud_3d(N=32, nbubbles=16)}
+    # non-uniform grids: per-cell coefficients in the learned sweeps
+    for name, (prm, _, q) in (("bubble_stretched", ic.shock_bubble_2d(N=48)),
+                              ("droplet3d_stretched", ic.shock_droplet_3d(N=28))):
+        out[name] = (prm, stretched_grids(prm), q)
+    return out
 
 
 def test_learned_rhs_matches_oracle(gpu):
