@@ -58,6 +58,13 @@ typedef struct mfc_params {
     double gammas[MFC_NUM_FLUIDS_MAX];  /* fluid_pp(i)%gamma  */
     double pi_infs[MFC_NUM_FLUIDS_MAX]; /* fluid_pp(i)%pi_inf */
     int precision;                      /* MFC_PREC_* */
+    /* fluid_pp(i)%Re(1:2): shear and volume Reynolds numbers.  An entry <= 0
+     * (the reference's default dflt_real = -1e6, or a zeroed struct) leaves
+     * fluid i out of that viscous term (m_global_parameters.fpp:314-338); any
+     * positive entry turns the viscous terms on (m_viscous.fpp,
+     * m_riemann_solvers.fpp:683-902) with finite-difference velocity
+     * gradients (weno_Re_flux = F), in 1D and 2D and in MFC_PREC_FP64. */
+    double Re[MFC_NUM_FLUIDS_MAX][2];
 } mfc_params;
 
 /* Replaces the module set-up chain s_initialize_global_parameters_module,
@@ -159,7 +166,8 @@ const char *mfc_mlp_last_error(void);
 
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * every launch.  Kernel ids: 0 bc_fill, 1 cons_to_prim, 2 sweep_x, 3 sweep_y,
- * 4 sweep_z, 5 icfl, 6 rk1_update.  The reference's only timing is the mean
+ * 4 sweep_z, 5 icfl, 6 rk1_update, 7 viscous (face flux and divergence),
+ * 8 rk_stage (the stage update of viscous runs).  The reference's only timing is the mean
  * CPU time per step (src/simulation/m_time_steppers.fpp:281,352-358). */
 int mfc_timing_enable(int on);
 int mfc_timing_reset(void);

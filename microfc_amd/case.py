@@ -67,6 +67,9 @@ def case_from_params(prm: Params, dt: float, t_step_stop: int, t_step_save: int,
     for i in range(prm.num_fluids):
         c[f"fluid_pp({i + 1})%gamma"] = float(prm.gammas[i])
         c[f"fluid_pp({i + 1})%pi_inf"] = float(prm.pi_infs[i])
+        for r in range(2):
+            if prm.Re and i < len(prm.Re) and prm.Re[i][r] > 0:
+                c[f"fluid_pp({i + 1})%Re({r + 1})"] = float(prm.Re[i][r])
     return c
 
 

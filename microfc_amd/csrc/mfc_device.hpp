@@ -199,9 +199,12 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
 // (m_riemann_solvers.fpp:134-325).  qL is the state left of the face (the
 // right-face reconstruction of the left cell), qR the state right of it.
 // flux[] gets sys_size values, ustar the contact velocity that drives the
-// alpha*div(u) source.
-template <int NF, int D, int DIR, typename R>
-__device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &eos, R *flux, R &ustar) {
+// alpha*div(u) source.  With VSRC, vsrc[0..D-1] receives every component of
+// that star-region velocity, vel_src_rs (:313-323), which the viscous energy
+// flux multiplies.
+template <int NF, int D, int DIR, typename R, bool VSRC = false>
+__device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &eos, R *flux, R &ustar,
+                                          R *vsrc = nullptr) {
     using Y = Sys<NF, D>;
     R vel_L_rms = R(0), vel_R_rms = R(0);
 #pragma unroll
@@ -258,6 +261,14 @@ __device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &e
     for (int i = 0; i < NF; ++i) flux[Y::ADV + i] = xi_M * qL[Y::ADV + i] * mL + xi_P * qR[Y::ADV + i] * mR;
     // dir_flg = 1 for the normal component: xi_M*(u_L + 1*s_M*(xi_L-1)) + ...
     ustar = xi_M * (uL + R(1) * s_M * (xi_L - R(1))) + xi_P * (uR + R(1) * s_P * (xi_R - R(1)));
+    if constexpr (VSRC) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const R flg = (d == DIR) ? R(1) : R(0);
+            vsrc[d] = xi_M * (qL[Y::MOM + d] + flg * s_M * (xi_L - R(1))) +
+                      xi_P * (qR[Y::MOM + d] + flg * s_P * (xi_R - R(1)));
+        }
+    }
 }
 
 } // namespace mfc

@@ -2,6 +2,7 @@
 // plus the small kernels around the sweeps: ghost-cell fill, cons->prim,
 // the CFL max-reduction and the pointwise RK update.
 #include "mfc_launch.hpp"
+#include "mfc_viscous.hpp"
 
 #include <cmath>
 #include <cstdarg>
@@ -56,6 +57,7 @@ struct Dir {
     double *coef = nullptr;   // device, 27 x ncell
     double *inv_ds = nullptr; // device, s+1
     double *ds_dev = nullptr; // device, ds(0..s)
+    double *cc_dev = nullptr; // device, ghosted cell centres s_cc(-b..s+b)
     std::vector<double> coef_h, ds_h;
     bool ready = false;
     bool uniform = false;  // every reconstructed cell has the same coefficients
@@ -71,13 +73,18 @@ struct State {
     hipStream_t own = nullptr, stream = nullptr;
     double *partials = nullptr; // kReduceBlocks doubles
     double *scratch_state = nullptr; // S*ng doubles, RK3 stage buffer
+    bool visc = false;               // any fluid_pp%Re > 0
+    ViscParams vp{};
+    double *visc_fs = nullptr;       // (D+1) x faces of one direction, viscous source flux
+    long visc_nface = 0;
 } G;
 
 // ---------------------------------------------------------------------------
 // Optional per-kernel timing with HIP events on the library's stream.
-enum KernelId { kKBcFill = 0, kKConsToPrim, kKSweepX, kKSweepY, kKSweepZ, kKIcfl, kKRk1, kKCount };
+enum KernelId { kKBcFill = 0, kKConsToPrim, kKSweepX, kKSweepY, kKSweepZ, kKIcfl, kKRk1, kKViscous, kKRkStage,
+                kKCount };
 const char *const kKernelNames[kKCount] = {"bc_fill", "cons_to_prim", "sweep_x", "sweep_y", "sweep_z", "icfl",
-                                           "rk1_update"};
+                                           "rk1_update", "viscous", "rk_stage"};
 
 struct Timing {
     bool on = false;
@@ -165,6 +172,13 @@ int check_params(const mfc_params &p) {
         return fail(-1, "Unsupported precision");
     if (p.precision == MFC_PREC_FP32_LEARNED && p.weno_order != 5)
         return fail(-1, "learned WENO weights need weno_order = 5");
+    bool visc = false;
+    for (int i = 0; i < p.num_fluids; ++i)
+        for (int r = 0; r < 2; ++r) visc = visc || p.Re[i][r] > 0.0;
+    if (visc && D > 2) return fail(-1, "viscous terms are 1D and 2D only, as in the reference");
+    if (visc && p.precision != MFC_PREC_FP64) return fail(-1, "viscous terms run in the FP64 form only");
+    if (visc && p.num_fluids > kViscFluidsMax)
+        return fail(-1, "viscous kernels are built for 1..%d fluids", kViscFluidsMax);
     // The FP64 weight divisions (ndiv) equal IEEE division while the squared
     // smoothness indicators stay within [1e-200, 1e100], which weno_eps >=
     // 1e-100 guarantees from below.
@@ -589,8 +603,41 @@ int sweep(int d, SweepArgs &a) {
     return 0;
 }
 
+// Viscous source flux of direction d and its divergence, added to rhs after
+// the direction's inviscid terms (m_rhs.fpp:591-603, :639-651).
+int viscous(int d, const Fields &qin, const Fields &rhs) {
+    ViscArgs v;
+    memset(&v, 0, sizeof v);
+    v.q = qin;
+    v.rhs = rhs;
+    v.fs = G.visc_fs;
+    v.nface = d == 0 ? (long)(G.prm.m + 2) * (G.prm.n + 1) : (long)(G.prm.m + 1) * (G.prm.n + 2);
+    v.coef = G.dir[d].coef;
+    v.ncell = G.dir[d].s + 2 * G.b + 1;
+    v.ccx = G.dir[0].cc_dev;
+    v.ccy = G.D > 1 ? G.dir[1].cc_dev : nullptr;
+    v.inv_ds = G.dir[d].inv_ds;
+    v.eos = eos();
+    v.eps = G.prm.weno_eps;
+    v.nxg = G.nxg;
+    v.m = G.prm.m;
+    v.n = G.prm.n;
+    v.b = G.b;
+    v.bj = G.prm.n > 0 ? G.b : 0;
+    const int bcs[2][2] = {{G.prm.bc_x_beg, G.prm.bc_x_end}, {G.prm.bc_y_beg, G.prm.bc_y_end}};
+    v.bc_beg = bcs[d][0];
+    v.bc_end = bcs[d][1];
+    v.vp = G.vp;
+    TimedScope ts(kKViscous);
+    if (launch_viscous(G.prm.num_fluids, G.D, d, G.prm.weno_order, v, G.stream))
+        return fail(-3, "viscous launch failed (dir %d): %s", d, hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
 // All directional sweeps of one RHS evaluation on the ghost-filled field qin.
-// The last sweep either writes rhs (rk = kRkNone) or applies the RK stage.
+// The last sweep either writes rhs (rk = kRkNone) or applies the RK stage;
+// with viscous terms every sweep writes rhs, the viscous kernels add to it,
+// and a pointwise kernel applies the RK stage.
 int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, const Fields *qout, double dt) {
     for (int d = 0; d < G.D; ++d) {
         SweepArgs a = sweep_args(d);
@@ -599,13 +646,27 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         a.accumulate = d > 0;
         a.dt = dt;
         const bool last = d == G.D - 1;
-        a.rk = last ? rk : kRkNone;
-        if (last && rk != kRkNone) {
+        a.rk = last && !G.visc ? rk : kRkNone;
+        if (a.rk != kRkNone) {
             a.qbase = *qbase;
             a.qout = *qout;
             a.base_is_in = qbase->v[0] == qin.v[0];
         }
         if (int rc = sweep(d, a)) return rc;
+        if (G.visc)
+            if (int rc = viscous(d, qin, rhs)) return rc;
+    }
+    if (G.visc && rk != kRkNone) {
+        SweepArgs a = sweep_args(0);
+        a.qin = qin;
+        a.rhs = rhs;
+        a.qbase = *qbase;
+        a.qout = *qout;
+        a.rk = rk;
+        a.dt = dt;
+        TimedScope ts(kKRkStage);
+        if (launch_rk_stage(G.prm.num_fluids, G.D, a, G.nint, G.stream))
+            return fail(-3, "RK stage launch failed: %s", hipGetErrorString(hipGetLastError()));
     }
     return 0;
 }
@@ -639,6 +700,19 @@ int mfc_rhs_init(const mfc_params *prm) {
     HIPCHK(hipStreamCreateWithFlags(&G.own, hipStreamNonBlocking));
     G.stream = G.own;
     HIPCHK(hipMalloc(&G.partials, sizeof(double) * kReduceBlocks));
+    // Re_size, Re_idx, Res (m_global_parameters.fpp:314-338, m_viscous.fpp:36-42)
+    for (int r = 0; r < 2; ++r)
+        for (int i = 0; i < prm->num_fluids; ++i)
+            if (prm->Re[i][r] > 0.0) {
+                G.vp.idx[r][G.vp.size[r]] = i;
+                G.vp.res[r][G.vp.size[r]] = prm->Re[i][r];
+                G.vp.size[r]++;
+            }
+    G.visc = G.vp.size[0] > 0 || G.vp.size[1] > 0;
+    if (G.visc) {
+        G.visc_nface = std::max((long)(prm->m + 2) * (prm->n + 1), (long)(prm->m + 1) * (prm->n + 2));
+        HIPCHK(hipMalloc(&G.visc_fs, sizeof(double) * (G.D + 1) * G.visc_nface));
+    }
     G.init = true;
     return 0;
 }
@@ -665,16 +739,25 @@ int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds) {
     }
     std::vector<double> inv(s + 1);
     for (int i = 0; i <= s; ++i) inv[i] = 1.0 / ds[i + b];
+    // cell centres with ghosts (m_start_up.fpp:356-359, :551, :582)
+    std::vector<double> cc(s + 2 * b + 1);
+    for (int i = 0; i <= s; ++i) cc[i + b] = s_cb[i - 1 + 1 + b] + ds[i + b] / 2.0;
+    for (int i = 1; i <= b; ++i) cc[-i + b] = cc[1 - i + b] - (ds[1 - i + b] + ds[-i + b]) / 2.0;
+    for (int i = 1; i <= b; ++i) cc[s + i + b] = cc[s + (i - 1) + b] + (ds[s + (i - 1) + b] + ds[s + i + b]) / 2.0;
     D.ds_h.assign(ds, ds + s + 2 * b + 1);
     if (D.coef) HIPCHK(hipFree(D.coef));
     if (D.inv_ds) HIPCHK(hipFree(D.inv_ds));
     if (D.ds_dev) HIPCHK(hipFree(D.ds_dev));
+    if (D.cc_dev) HIPCHK(hipFree(D.cc_dev));
     D.coef = nullptr;
     D.inv_ds = nullptr;
     D.ds_dev = nullptr;
+    D.cc_dev = nullptr;
     HIPCHK(hipMalloc(&D.coef, sizeof(double) * D.coef_h.size()));
     HIPCHK(hipMalloc(&D.inv_ds, sizeof(double) * inv.size()));
     HIPCHK(hipMalloc(&D.ds_dev, sizeof(double) * (s + 1)));
+    HIPCHK(hipMalloc(&D.cc_dev, sizeof(double) * cc.size()));
+    HIPCHK(hipMemcpy(D.cc_dev, cc.data(), sizeof(double) * cc.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(D.ds_dev, ds + b, sizeof(double) * (s + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(D.coef, D.coef_h.data(), sizeof(double) * D.coef_h.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(D.inv_ds, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
@@ -883,10 +966,12 @@ int mfc_rhs_finalize(void) {
         if (G.dir[d].coef) (void)hipFree(G.dir[d].coef);
         if (G.dir[d].inv_ds) (void)hipFree(G.dir[d].inv_ds);
         if (G.dir[d].ds_dev) (void)hipFree(G.dir[d].ds_dev);
+        if (G.dir[d].cc_dev) (void)hipFree(G.dir[d].cc_dev);
         G.dir[d] = Dir();
     }
     if (G.partials) (void)hipFree(G.partials);
     if (G.scratch_state) (void)hipFree(G.scratch_state);
+    if (G.visc_fs) (void)hipFree(G.visc_fs);
     if (G.own) (void)hipStreamDestroy(G.own);
     G = State();
     return 0;

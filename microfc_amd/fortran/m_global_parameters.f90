@@ -122,10 +122,11 @@ contains
         prm%bc_y_beg = bc_y%beg; prm%bc_y_end = bc_y%end
         prm%bc_z_beg = bc_z%beg; prm%bc_z_end = bc_z%end
         prm%weno_eps = weno_eps
-        prm%gammas = 0d0; prm%pi_infs = 0d0
+        prm%gammas = 0d0; prm%pi_infs = 0d0; prm%Re = dflt_real
         do i = 1, min(num_fluids, num_fluids_max)
             prm%gammas(i) = fluid_pp(i)%gamma
             prm%pi_infs(i) = fluid_pp(i)%pi_inf
+            prm%Re(:, i) = fluid_pp(i)%Re(:)
         end do
         prm%precision = bulk_precision
     end function f_mfc_params

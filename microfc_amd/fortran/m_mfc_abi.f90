@@ -24,6 +24,7 @@ module m_mfc_abi
         real(c_double) :: gammas(mfc_num_fluids_max)
         real(c_double) :: pi_infs(mfc_num_fluids_max)
         integer(c_int) :: precision
+        real(c_double) :: Re(2, mfc_num_fluids_max)
     end type mfc_params
 
     interface

@@ -53,12 +53,12 @@ contains
         if (probe_wrt) call s_abort('probe_wrt is not supported by this build. Exiting ...')
         if (parallel_io) call s_abort('parallel_io is not supported by this build. Exiting ...')
         if (cu_mpi) call s_abort('Unsupported value of cu_mpi. Exiting ...')
-        do i = 1, num_fluids
-            do j = 1, 2
-                if (fluid_pp(i)%Re(j) /= dflt_real) &
-                    call s_abort('Viscous cases (fluid_pp%Re) are not supported by this build. Exiting ...')
-            end do
-        end do
+        ! Viscous terms run with finite-difference velocity gradients.  The
+        ! reference's weno_Re_flux branch takes cell gradients at cells -3 and
+        ! m+3 (m_viscous.fpp:411-423) from face values it reconstructs only on
+        ! [-2, m+2] (:475-479), so its boundary faces read unset memory.
+        if (weno_Re_flux .and. any(fluid_pp(1:num_fluids)%Re(1) > 0d0 .or. fluid_pp(1:num_fluids)%Re(2) > 0d0)) &
+            call s_abort('weno_Re_flux is not supported by this build (use finite-difference gradients). Exiting ...')
         if (n == 0 .and. bc_y%beg == dflt_int) then
             bc_y%beg = -3; bc_y%end = -3
         end if

@@ -39,7 +39,7 @@ EXPORTED = [
     "mfc_ml_restrict", "mfc_ml_samples", "mfc_ml_restrict3", "mfc_ml_samples3", "mfc_mlp_last_error",
 ]
 KERNEL_IDS = {"bc_fill": 0, "cons_to_prim": 1, "sweep_x": 2, "sweep_y": 3, "sweep_z": 4, "icfl": 5,
-              "rk1_update": 6}
+              "rk1_update": 6, "viscous": 7, "rk_stage": 8}
 
 
 class MfcError(RuntimeError):
@@ -58,6 +58,7 @@ class CParams(ctypes.Structure):
         ("gammas", ctypes.c_double * NUM_FLUIDS_MAX),
         ("pi_infs", ctypes.c_double * NUM_FLUIDS_MAX),
         ("precision", ctypes.c_int),
+        ("Re", (ctypes.c_double * 2) * NUM_FLUIDS_MAX),
     ]
 
 
@@ -121,6 +122,13 @@ class Params:
     bc_beg: tuple = (-3, -3, -3)
     bc_end: tuple = (-3, -3, -3)
     precision: int = PREC_FP64
+    # fluid_pp(i)%Re(1:2) per fluid; entries <= 0 leave the fluid out, None
+    # (or no positive entry) is inviscid
+    Re: list = None
+
+    @property
+    def viscous(self) -> bool:
+        return bool(self.Re) and any(r > 0 for pair in self.Re for r in pair)
 
     @property
     def num_dims(self) -> int:
@@ -167,6 +175,9 @@ class Params:
             c.gammas[i] = self.gammas[i]
             c.pi_infs[i] = self.pi_infs[i]
         c.precision = self.precision
+        for i in range(NUM_FLUIDS_MAX):
+            for r in range(2):
+                c.Re[i][r] = self.Re[i][r] if self.Re and i < len(self.Re) else -1e6
         return c
 
 

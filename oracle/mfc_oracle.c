@@ -87,6 +87,17 @@ static inline long ridx(const geom *g, int i, int j, int k) {
 /* ------------------------------------------------------------------------ */
 /* grid ghost buffers: m_start_up.fpp:517-655                                */
 
+/* Cell centres with ghosts from the ghosted cb and ds (m_start_up.fpp:356-359
+ * for the interior, :551 and :582 for the ghosts). */
+static void cell_centres(int s, int b, const double *cb_, const double *ds_, double *cc_) {
+    const double *cb = cb_ + 1 + b, *ds = ds_ + b;
+    double *cc = cc_ + b;
+    int i;
+    for (i = 0; i <= s; i++) cc[i] = cb[i - 1] + ds[i] / 2.0;
+    for (i = 1; i <= b; i++) cc[-i] = cc[1 - i] - (ds[1 - i] + ds[-i]) / 2.0;
+    for (i = 1; i <= b; i++) cc[s + i] = cc[s + (i - 1)] + (ds[s + (i - 1)] + ds[s + i]) / 2.0;
+}
+
 int orc_grid_buffers(int s, int b, int bc_beg, int bc_end, double *cb_, double *ds_, double *cc_) {
     double *cb = cb_ + 1 + b; /* cb[i] = s_cb(i), i in [-1-b, s+b] */
     double *ds = ds_ + b;     /* ds[i] = ds(i),   i in [-b, s+b]   */
@@ -94,7 +105,6 @@ int orc_grid_buffers(int s, int b, int bc_beg, int bc_end, double *cb_, double *
     int i;
     /* m_start_up.fpp:356-359 */
     for (i = 0; i <= s; i++) ds[i] = cb[i] - cb[i - 1];
-    for (i = 0; i <= s; i++) cc[i] = cb[i - 1] + ds[i] / 2.0;
     /* beginning: m_start_up.fpp:527-550 */
     if (bc_beg <= -3) {
         for (i = 1; i <= b; i++) ds[-i] = ds[0];
@@ -106,7 +116,6 @@ int orc_grid_buffers(int s, int b, int bc_beg, int bc_end, double *cb_, double *
         return -1; /* processor boundaries are not part of a single-domain run */
     }
     for (i = 1; i <= b; i++) cb[-1 - i] = cb[-i] - ds[-i];
-    for (i = 1; i <= b; i++) cc[-i] = cc[1 - i] - (ds[1 - i] + ds[-i]) / 2.0;
     /* end: m_start_up.fpp:557-582 */
     if (bc_end <= -3) {
         for (i = 1; i <= b; i++) ds[s + i] = ds[s];
@@ -118,7 +127,8 @@ int orc_grid_buffers(int s, int b, int bc_beg, int bc_end, double *cb_, double *
         return -1;
     }
     for (i = 1; i <= b; i++) cb[s + i] = cb[s + (i - 1)] + ds[s + i];
-    for (i = 1; i <= b; i++) cc[s + i] = cc[s + (i - 1)] + (ds[s + (i - 1)] + ds[s + i]) / 2.0;
+    cell_centres(s, b, cb_, ds_, cc_);
+    (void)cc;
     return 0;
 }
 
@@ -420,8 +430,10 @@ void orc_weno5_learned_cell(double weno_eps, const double *cf, const double *v, 
 static inline double fmin_ref(double a, double b) { return (a < b) ? a : b; }
 static inline double fmax_ref(double a, double b) { return (a > b) ? a : b; }
 
-void orc_hllc_face(const orc_case *c, int dir, const double *qL, const double *qR, double *flux,
-                   double *ustar) {
+/* HLLC at one face; vsrc (may be NULL) receives vel_src_rs(:, :, 1:num_dims),
+ * the velocity the viscous energy flux multiplies (:313-323). */
+static void hllc_face_full(const orc_case *c, int dir, const double *qL, const double *qR, double *flux,
+                           double *ustar, double *vsrc) {
     const int nf = c->num_fluids, D = orc_num_dims(c);
     const int contxe = nf - 1, E = nf + D, advxb = nf + D + 1;
     double velL[3], velR[3];
@@ -485,6 +497,16 @@ void orc_hllc_face(const orc_case *c, int dir, const double *qL, const double *q
         flux[i] = xi_M * qL[i] * (uL + s_M * (xi_L - 1.0)) + xi_P * qR[i] * (uR + s_P * (xi_R - 1.0));
     /* contact velocity, :314-325 (dir_flg = 1 for the normal component) */
     *ustar = xi_M * (uL + 1.0 * s_M * (xi_L - 1.0)) + xi_P * (uR + 1.0 * s_P * (xi_R - 1.0));
+    if (vsrc)
+        for (i = 0; i < D; i++) {
+            const double flg = (i == dir) ? 1.0 : 0.0;
+            vsrc[i] = xi_M * (velL[i] + flg * s_M * (xi_L - 1.0)) + xi_P * (velR[i] + flg * s_P * (xi_R - 1.0));
+        }
+}
+
+void orc_hllc_face(const orc_case *c, int dir, const double *qL, const double *qR, double *flux,
+                   double *ustar) {
+    hllc_face_full(c, dir, qL, qR, flux, ustar, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -580,6 +602,194 @@ int orc_bc_fill(const orc_case *c, double *q) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Viscous terms with finite-difference velocity gradients (weno_Re_flux = F):
+ * m_viscous.fpp:159-370 (gradients), m_riemann_solvers.fpp:169-231 (face
+ * Reynolds numbers), :683-902 (viscous source flux), m_rhs.fpp:591-603 and
+ * :639-651 (its divergence).  1D and 2D, as in the reference.               */
+
+typedef struct {
+    int size[2];                       /* Re_size(1:2) */
+    int idx[2][ORC_NUM_FLUIDS_MAX];    /* Re_idx, 0-based fluid index */
+    double res[2][ORC_NUM_FLUIDS_MAX]; /* Res */
+    /* Face velocity gradients, D components of ng doubles each, over the
+     * ghosted grid: Lx1/Rx1 = dqL/dqR_prim_dx_n(1) and Ly2/Ry2 =
+     * dqL/dqR_prim_dy_n(2) (one-sided differences), Lx2/Rx2 = dq*_prim_dx_n(2)
+     * and Ly1/Ry1 = dq*_prim_dy_n(1) (their four-point averages).  Entries the
+     * reference does not compute hold NaN. */
+    double *Lx1, *Rx1, *Ly2, *Ry2, *Lx2, *Rx2, *Ly1, *Ry1;
+} visc;
+
+/* Re_size, Re_idx, Res: m_global_parameters.fpp:314-338, m_viscous.fpp:36-42. */
+static int visc_setup(const orc_case *c, visc *w) {
+    int i, r;
+    memset(w, 0, sizeof *w);
+    for (r = 0; r < 2; r++)
+        for (i = 0; i < c->num_fluids; i++)
+            if (c->Re[i][r] > 0) {
+                w->idx[r][w->size[r]] = i;
+                w->res[r][w->size[r]] = c->Re[i][r];
+                w->size[r]++;
+            }
+    return w->size[0] > 0 || w->size[1] > 0;
+}
+
+static void visc_free(visc *w) {
+    free(w->Lx1); free(w->Rx1); free(w->Ly2); free(w->Ry2);
+    free(w->Lx2); free(w->Rx2); free(w->Ly1); free(w->Ry1);
+}
+
+/* s_get_viscous, finite-difference branch (m_viscous.fpp:231-355) with the
+ * index ranges s_compute_rhs sets first (m_rhs.fpp:415-419): x in [-b, m+b],
+ * y in [-b, n+b] (2D) or {0} (1D).  The cell-centred gradients the branch also
+ * computes (s_compute_fd_gradient, :340-352) feed nothing when weno_Re_flux is
+ * off and are left out. */
+static int visc_gradients(const orc_case *c, const geom *g, const double *qp, const double *ccx_,
+                          const double *ccy_, visc *w) {
+    const long tot = (long)g->D * g->ng;
+    const int b = g->b, m = g->m, n = g->n;
+    const int y0 = g->D > 1 ? -b : 0, y1 = g->D > 1 ? n + b : 0;
+    const double *ccx = ccx_ + b, *ccy = g->D > 1 ? ccy_ + b : NULL;
+    double **arr[8] = {&w->Lx1, &w->Rx1, &w->Ly2, &w->Ry2, &w->Lx2, &w->Rx2, &w->Ly1, &w->Ry1};
+    int a, i, x, y;
+    long o;
+    (void)c;
+    for (a = 0; a < (g->D > 1 ? 8 : 2); a++) {
+        *arr[a] = (double *)malloc(sizeof(double) * tot);
+        if (!*arr[a]) return -2;
+        for (o = 0; o < tot; o++) (*arr[a])[o] = NAN;
+    }
+#define U(x, y) qp[(long)(g->momxb + i) * g->ng + gidx(g, x, y, 0)]
+#define AT(A, x, y) A[(long)i * g->ng + gidx(g, x, y, 0)]
+    for (i = 0; i < g->D; i++) {
+        for (y = y0; y <= y1; y++)
+            for (x = -b + 1; x <= m + b; x++)
+                AT(w->Lx1, x, y) = (U(x, y) - U(x - 1, y)) / (ccx[x] - ccx[x - 1]);
+        for (y = y0; y <= y1; y++)
+            for (x = -b; x <= m + b - 1; x++)
+                AT(w->Rx1, x, y) = (U(x + 1, y) - U(x, y)) / (ccx[x + 1] - ccx[x]);
+        if (g->D == 1) continue;
+        for (y = -b + 1; y <= n + b; y++)
+            for (x = -b; x <= m + b; x++)
+                AT(w->Ly2, x, y) = (U(x, y) - U(x, y - 1)) / (ccy[y] - ccy[y - 1]);
+        for (y = -b; y <= n + b - 1; y++)
+            for (x = -b; x <= m + b; x++)
+                AT(w->Ry2, x, y) = (U(x, y + 1) - U(x, y)) / (ccy[y + 1] - ccy[y]);
+    }
+    if (g->D > 1)
+        for (i = 0; i < g->D; i++) {
+            for (y = -b + 1; y <= n + b; y++)
+                for (x = -b + 1; x <= m + b - 1; x++) {
+                    double t = AT(w->Lx1, x, y) + AT(w->Rx1, x, y) + AT(w->Lx1, x, y - 1) + AT(w->Rx1, x, y - 1);
+                    AT(w->Lx2, x, y) = 25e-2 * t;
+                }
+            for (y = -b; y <= n + b - 1; y++)
+                for (x = -b + 1; x <= m + b - 1; x++) {
+                    double t = AT(w->Lx1, x, y + 1) + AT(w->Rx1, x, y + 1) + AT(w->Lx1, x, y) + AT(w->Rx1, x, y);
+                    AT(w->Rx2, x, y) = 25e-2 * t;
+                }
+            for (y = -b + 1; y <= n + b - 1; y++)
+                for (x = -b + 1; x <= m + b; x++) {
+                    double t = AT(w->Ly2, x, y) + AT(w->Ry2, x, y) + AT(w->Ly2, x - 1, y) + AT(w->Ry2, x - 1, y);
+                    AT(w->Ly1, x, y) = 25e-2 * t;
+                }
+            for (y = -b + 1; y <= n + b - 1; y++)
+                for (x = -b; x <= m + b - 1; x++) {
+                    double t = AT(w->Ly2, x + 1, y) + AT(w->Ry2, x + 1, y) + AT(w->Ly2, x, y) + AT(w->Ry2, x, y);
+                    AT(w->Ry1, x, y) = 25e-2 * t;
+                }
+        }
+#undef U
+#undef AT
+    return 0;
+}
+
+/* Harmonic face Reynolds numbers Re_avg(1:2) from the reconstructed volume
+ * fractions of the two face states (m_riemann_solvers.fpp:169-231). */
+static void visc_re_avg(const visc *w, const geom *g, const double *ql, const double *qr, double *re_avg) {
+    int r, q;
+    for (r = 0; r < 2; r++) {
+        double reL, reR;
+        if (w->size[r] == 0) { re_avg[r] = 0.0; continue; }
+        reL = 0.0;
+        reR = 0.0;
+        for (q = 0; q < w->size[r]; q++) {
+            reL = ql[g->advxb + w->idx[r][q]] / w->res[r][q] + reL;
+            reR = qr[g->advxb + w->idx[r][q]] / w->res[r][q] + reR;
+        }
+        reL = 1.0 / fmax_ref(reL, SGM_EPS);
+        reR = 1.0 / fmax_ref(reR, SGM_EPS);
+        re_avg[r] = 2.0 / (1.0 / reL + 1.0 / reR);
+    }
+}
+
+/* s_compute_viscous_source_flux at one face (m_riemann_solvers.fpp:683-902),
+ * with the caller's argument swap (m_rhs.fpp:545-556: the solver's left
+ * gradients are dqR_prim_d*_n, evaluated at the face's left cell, its right
+ * ones dqL_prim_d*_n at the right cell).  (x, y) is the face's left cell;
+ * fs[0..D-1] momentum, fs[D] energy. */
+static void visc_face_flux(const visc *w, const geom *g, int dir, int x, int y, const double *vsrc,
+                           const double *re_avg, double *fs) {
+    int i;
+    const int D = g->D;
+#define AT(A, c, xx, yy) A[(long)(c) * g->ng + gidx(g, xx, yy, 0)]
+    for (i = 0; i <= D; i++) fs[i] = 0.0;
+    if (dir == 0) {
+        if (w->size[0] > 0) {
+            const double dx1 = 5e-1 * (AT(w->Rx1, 0, x, y) + AT(w->Lx1, 0, x + 1, y));
+            const double t11 = (4.0 / 3.0) * dx1 / re_avg[0];
+            fs[0] = fs[0] - t11;
+            fs[D] = fs[D] - vsrc[0] * t11;
+        }
+        if (w->size[1] > 0) {
+            const double dx1 = 5e-1 * (AT(w->Rx1, 0, x, y) + AT(w->Lx1, 0, x + 1, y));
+            const double t11 = dx1 / re_avg[1];
+            fs[0] = fs[0] - t11;
+            fs[D] = fs[D] - vsrc[0] * t11;
+        }
+        if (D == 1) return;
+        if (w->size[0] > 0) {
+            double dy[2], dx2, tau[2];
+            for (i = 0; i < 2; i++) dy[i] = 5e-1 * (AT(w->Ry1, i, x, y) + AT(w->Ly1, i, x + 1, y));
+            dx2 = 5e-1 * (AT(w->Rx1, 1, x, y) + AT(w->Lx1, 1, x + 1, y));
+            tau[0] = -((2.0 / 3.0) * dy[1] / re_avg[0]);
+            tau[1] = (dy[0] + dx2) / re_avg[0];
+            for (i = 0; i < 2; i++) {
+                fs[i] = fs[i] - tau[i];
+                fs[D] = fs[D] - vsrc[i] * tau[i];
+            }
+        }
+        if (w->size[1] > 0) {
+            const double dy2 = 5e-1 * (AT(w->Ry1, 1, x, y) + AT(w->Ly1, 1, x + 1, y));
+            const double t11 = dy2 / re_avg[1];
+            fs[0] = fs[0] - t11;
+            fs[D] = fs[D] - vsrc[0] * t11;
+        }
+    } else {
+        if (w->size[0] > 0) {
+            double dx[2], dy[2], tau[2];
+            for (i = 0; i < 2; i++) {
+                dx[i] = 5e-1 * (AT(w->Rx2, i, x, y) + AT(w->Lx2, i, x, y + 1));
+                dy[i] = 5e-1 * (AT(w->Ry2, i, x, y) + AT(w->Ly2, i, x, y + 1));
+            }
+            tau[0] = (dy[0] + dx[1]) / re_avg[0];
+            tau[1] = (4.0 * dy[1] - 2.0 * dx[0]) / (3.0 * re_avg[0]);
+            for (i = 0; i < 2; i++) {
+                fs[i] = fs[i] - tau[i];
+                fs[D] = fs[D] - vsrc[i] * tau[i];
+            }
+        }
+        if (w->size[1] > 0) {
+            const double dx1 = 5e-1 * (AT(w->Rx2, 0, x, y) + AT(w->Lx2, 0, x, y + 1));
+            const double dy2 = 5e-1 * (AT(w->Ry2, 1, x, y) + AT(w->Ly2, 1, x, y + 1));
+            const double t22 = (dx1 + dy2) / re_avg[1];
+            fs[1] = fs[1] - t22;
+            fs[D] = fs[D] - vsrc[1] * t22;
+        }
+    }
+#undef AT
+}
+
+/* ------------------------------------------------------------------------ */
 /* s_compute_rhs: m_rhs.fpp:405-679                                          */
 
 /* One pencil of one direction of s_compute_rhs: WENO (m_weno.fpp:470-536)
@@ -590,7 +800,7 @@ int orc_bc_fill(const orc_case *c, double *q) {
  * reference also reconstructs further ghost cells and ghost rows whose values
  * never reach the interior fluxes. */
 static void pencil(const orc_case *c, const geom *gp, int dir, int s, int k1, int k2, const double *coef,
-                   const double *qc, const double *qp, const double *dsd, double *rhs) {
+                   const double *qc, const double *qp, const double *dsd, const visc *w, double *rhs) {
     const geom g = *gp;
     const long nc = s + 2 * g.b + 1;
     const long ns = s + 3; /* cells -1..s+1 */
@@ -598,6 +808,7 @@ static void pencil(const orc_case *c, const geom *gp, int dir, int s, int k1, in
     double *vR = (double *)malloc(sizeof(double) * g.S * ns);
     double *flux = (double *)malloc(sizeof(double) * g.S * (s + 2));
     double *ustar = (double *)malloc(sizeof(double) * (s + 2));
+    double *fsv = w ? (double *)malloc(sizeof(double) * (gp->D + 1) * (s + 2)) : NULL;
     int j, q, v;
 #define POS(jj, I, J, K)                                                   \
     do {                                                                   \
@@ -636,8 +847,17 @@ static void pencil(const orc_case *c, const geom *gp, int dir, int s, int k1, in
             ql[v] = vR[v * ns + (j + 1)];
             qr[v] = vL[v * ns + (j + 2)];
         }
-        orc_hllc_face(c, dir, ql, qr, f, &ustar[j + 1]);
+        double vsrc[3], re_avg[2], fs[4];
+        hllc_face_full(c, dir, ql, qr, f, &ustar[j + 1], vsrc);
         for (v = 0; v < g.S; v++) flux[v * (s + 2) + (j + 1)] = f[v];
+        if (w) {
+            int I, J, K;
+            POS(j, I, J, K);
+            (void)K;
+            visc_re_avg(w, &g, ql, qr, re_avg);
+            visc_face_flux(w, &g, dir, I, J, vsrc, re_avg, fs);
+            for (v = 0; v <= g.D; v++) fsv[v * (s + 2) + (j + 1)] = fs[v];
+        }
     }
     for (v = 0; v < g.S; v++)
         for (j = 0; j <= s; j++) {
@@ -658,7 +878,17 @@ static void pencil(const orc_case *c, const geom *gp, int dir, int s, int k1, in
             r = (long)v * (g.m + 1) * (g.n + 1) * (g.p + 1) + ridx(&g, I, J, K);
             rhs[r] = rhs[r] + 1.0 / dsd[j + g.b] * qc[v * g.ng + gidx(&g, I, J, K)] * (ustar[j + 1] - ustar[j]);
         }
+    if (w) /* m_rhs.fpp:591-603 (x), :639-651 (y): momentum and energy */
+        for (v = 0; v <= g.D; v++)
+            for (j = 0; j <= s; j++) {
+                int I, J, K;
+                long r;
+                POS(j, I, J, K);
+                r = (long)(g.momxb + v) * (g.m + 1) * (g.n + 1) * (g.p + 1) + ridx(&g, I, J, K);
+                rhs[r] = rhs[r] + 1.0 / dsd[j + g.b] * (fsv[v * (s + 2) + j] - fsv[v * (s + 2) + j + 1]);
+            }
 #undef POS
+    free(fsv);
     free(vL);
     free(vR);
     free(flux);
@@ -669,7 +899,8 @@ int orc_compute_rhs(const orc_case *c, const double *const cb[3], const double *
                     const double *q_cons, double *q_prim, double *rhs, int t_step) {
     geom g;
     double *qc, *qp, *coef;
-    int dir;
+    int dir, viscous;
+    visc w;
     mkgeom(c, &g);
     if (c->weno_order != 1 && c->weno_order != 3 && c->weno_order != 5) return -1;
     qc = (double *)malloc(sizeof(double) * g.S * g.ng);
@@ -680,6 +911,21 @@ int orc_compute_rhs(const orc_case *c, const double *const cb[3], const double *
     if (orc_bc_fill(c, qc) != 0) { free(qc); free(qp); return -1; }
     orc_cons_to_prim(c, qc, qp);
     if (t_step == c->t_step_stop) { free(qc); free(qp); return 1; } /* :452 */
+    viscous = visc_setup(c, &w);
+    if (viscous) { /* :455-464 */
+        double *cc[2] = {NULL, NULL};
+        int rc = 0;
+        if (g.D > 2) rc = -1;
+        for (dir = 0; dir < g.D && rc == 0; dir++) {
+            const int s = dir == 0 ? g.m : g.n;
+            cc[dir] = (double *)malloc(sizeof(double) * (s + 2 * g.b + 1));
+            cell_centres(s, g.b, cb[dir], ds[dir], cc[dir]);
+        }
+        if (rc == 0) rc = visc_gradients(c, &g, qp, cc[0], cc[1], &w);
+        free(cc[0]);
+        free(cc[1]);
+        if (rc != 0) { visc_free(&w); free(qc); free(qp); return rc; }
+    }
 
     for (dir = 0; dir < g.D; dir++) {
         /* Extents along the sweep direction (s) and the two transverse ones. */
@@ -695,9 +941,11 @@ int orc_compute_rhs(const orc_case *c, const double *const cb[3], const double *
          * may run on several threads; the arithmetic of each is unchanged. */
 #pragma omp parallel for schedule(dynamic, 4)
         for (pc = 0; pc < npencil; pc++)
-            pencil(c, &g, dir, s, (int)(pc % (t1 + 1)), (int)(pc / (t1 + 1)), coef, qc, qp, ds[dir], rhs);
+            pencil(c, &g, dir, s, (int)(pc % (t1 + 1)), (int)(pc / (t1 + 1)), coef, qc, qp, ds[dir],
+                   viscous ? &w : NULL, rhs);
         free(coef);
     }
+    if (viscous) visc_free(&w);
     if (q_prim) memcpy(q_prim, qp, sizeof(double) * g.S * g.ng); /* :659-675 */
     free(qc);
     free(qp);

@@ -35,6 +35,7 @@ class _Case(ctypes.Structure):
         ("gammas", ctypes.c_double * NUM_FLUIDS_MAX),
         ("pi_infs", ctypes.c_double * NUM_FLUIDS_MAX),
         ("bc_beg", ctypes.c_int * 3), ("bc_end", ctypes.c_int * 3),
+        ("Re", (ctypes.c_double * 2) * NUM_FLUIDS_MAX),
     ]
 
 
@@ -90,6 +91,11 @@ class Case:
     pi_infs: list = field(default_factory=lambda: [0.0])
     bc_beg: tuple = (-3, -3, -3)
     bc_end: tuple = (-3, -3, -3)
+    Re: list = None  # per fluid (Re(1), Re(2)); a value <= 0 leaves the fluid out
+
+    @property
+    def viscous(self):
+        return bool(self.Re) and any(r > 0 for pair in self.Re for r in pair)
 
     @property
     def num_dims(self):
@@ -130,6 +136,9 @@ class Case:
         for d in range(3):
             s.bc_beg[d] = self.bc_beg[d]
             s.bc_end[d] = self.bc_end[d]
+        for i in range(NUM_FLUIDS_MAX):
+            for r in range(2):
+                s.Re[i][r] = self.Re[i][r] if self.Re and i < len(self.Re) else -1e6
         return s
 
 

@@ -1,0 +1,132 @@
+"""Viscous terms (finite-difference velocity gradients) on the GPU against
+the CPU FP64 oracle, whose viscous restatement tests/test_oracle_viscous.py
+pins.  Bar as for the inviscid path: relative L-infinity <= 1e-12 per RHS and
+per step (bitwise equality is printed)."""
+import numpy as np
+import pytest
+
+from microfc_amd import RhsModule, ic
+from microfc_amd.rhs import MfcError, PREC_FP32_BULK
+from oracle import oracle as O
+
+from helpers import assert_parity, multi_fluid_2d, perturbed, say, stretched_grids, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def with_velocity(prm, grids, q, amp):
+    """Adds a smooth velocity field of amplitude amp to every component at
+    unchanged density and pressure (the kinetic energy goes into E), so the
+    viscous terms act everywhere."""
+    q = q.copy()
+    nf, D = prm.num_fluids, prm.num_dims
+    X, Y, _ = ic.mesh(prm, grids)
+    qi = prm.interior(q)
+    rho = sum(qi[i] for i in range(nf))
+    ke0 = sum(0.5 * qi[nf + d] ** 2 for d in range(D)) / rho
+    for d in range(D):
+        phase = 2 * np.pi * (X + (0.0 if Y is None else (d + 1) * Y))
+        qi[nf + d] = qi[nf + d] + rho * amp * np.sin(phase + d)
+    qi[nf + D] = qi[nf + D] + sum(0.5 * qi[nf + d] ** 2 for d in range(D)) / rho - ke0
+    return q
+
+
+def _cases():
+    out = []
+    prm, g, q = ic.sod_1d(m=199)
+    prm.Re = [(1e3, 2e3)]
+    out.append(("visc_sod1d_w5_rk3", prm, g, with_velocity(prm, g, perturbed(prm, q), 0.2)))
+    prm, g, q = ic.sod_1d(m=63, weno_order=1)
+    prm.time_stepper = 1
+    prm.bc_beg = (-2, -3, -3)
+    prm.bc_end = (-4, -3, -3)
+    prm.Re = [(50.0, -1.0)]
+    g = ic.uniform_grids(prm)
+    out.append(("visc_sod1d_w1_rk1_reflect_extrap", prm, g, with_velocity(prm, g, perturbed(prm, q), 0.2)))
+    prm, g, q = ic.shock_bubble_2d(N=48)
+    prm.Re = [(0.05, 0.1), (0.1, 0.2)]
+    out.append(("visc_bubble2d_48", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
+    prm, g, q = ic.shock_bubble_2d(N=40, weno_order=3)
+    prm.time_stepper = 2
+    prm.bc_beg = (-1, -2, -3)
+    prm.bc_end = (-1, -4, -3)
+    prm.Re = [(0.05, -1.0), (0.1, -1.0)]
+    g = ic.uniform_grids(prm)
+    out.append(("visc_bubble2d_w3_rk2_periodic_reflect", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
+    prm, g, q = ic.shock_bubble_2d(N=40)
+    prm.Re = [(-1.0, 0.05), (-1.0, 0.1)]
+    g = stretched_grids(prm)
+    out.append(("visc_bubble2d_stretched_bulk_only", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
+    prm, g, q = multi_fluid_2d(3, N=32)
+    prm.Re = [(0.3, -1.0), (-1.0, 0.7), (0.9, 1.1)]
+    out.append(("visc_fluids3_2d", prm, g, q))
+    prm, g, q = ic.shear_layer_2d(N=48)
+    prm.Re = [(200.0, -1.0)]
+    out.append(("visc_shear2d_48", prm, g, perturbed(prm, q)))
+    return out
+
+
+CASES = _cases()
+IDS = [c[0] for c in CASES]
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_viscous_rhs_parity(gpu, case):
+    torch = gpu
+    name, prm, grids, q = case
+    say(f"[viscous rhs] {name} shape={prm.ghost_shape} Re={prm.Re}")
+    oc = to_oracle(prm)
+    ref = O.compute_rhs(oc, grids, q.copy())
+    inv = O.compute_rhs(O.Case(**{**oc.__dict__, "Re": None}), grids, q.copy())
+    mod = RhsModule(prm, grids)
+    try:
+        qd = torch.from_numpy(q.copy()).cuda()
+        rd = mod.new_field(ghosted=False)
+        mod.s_compute_rhs(qd, None, rd, 0)
+        mod.sync()
+        got = rd.cpu().numpy()
+        assert_parity(got, ref, what=f"{name} rhs")
+        # the viscous terms are not a rounding-level change here
+        nf, D = prm.num_fluids, prm.num_dims
+        moved = np.max(np.abs(ref[nf:nf + D + 1] - inv[nf:nf + D + 1])) / np.max(np.abs(inv[nf:nf + D + 1]))
+        say(f"  viscous share of the momentum/energy rhs: {moved:.2e}")
+        assert moved > 1e-8
+    finally:
+        mod.finalize()
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_viscous_tvd_rk_parity(gpu, case):
+    torch = gpu
+    name, prm, grids, q = case
+    nsteps = 3
+    say(f"[viscous tvd_rk] {name} stepper={prm.time_stepper} steps={nsteps}")
+    oc = to_oracle(prm)
+    mod = RhsModule(prm, grids)
+    try:
+        qd = torch.from_numpy(q.copy()).cuda()
+        dt = 0.2 * mod.cfl_dt(qd, 1.0)
+        qo = q.copy()
+        q2o = np.zeros_like(q)
+        q2d = mod.new_field()
+        rd = mod.new_field(ghosted=False)
+        for t in range(nsteps):
+            O.tvd_rk_step(oc, grids, dt, qo, q2o, t_step=t)
+            mod.s_tvd_rk(t, (qd, q2d), None, rd, dt)
+            mod.sync()
+            assert_parity(prm.interior(qd.cpu().numpy()), prm.interior(qo), what=f"{name} step {t}")
+    finally:
+        mod.finalize()
+
+
+def test_viscous_rejections(gpu):
+    """Viscous terms are 1D/2D and FP64, as the library documents."""
+    prm, g, _ = ic.shock_droplet_3d(N=28)
+    prm.Re = [(1.0, -1.0), (1.0, -1.0)]
+    with pytest.raises(MfcError, match="1D and 2D"):
+        RhsModule(prm, g)
+    prm, g, _ = ic.shock_bubble_2d(N=32)
+    prm.Re = [(1.0, -1.0), (1.0, -1.0)]
+    prm.precision = PREC_FP32_BULK
+    with pytest.raises(MfcError, match="FP64"):
+        RhsModule(prm, g)

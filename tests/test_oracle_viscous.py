@@ -1,0 +1,185 @@
+"""Pins the oracle's viscous terms (finite-difference velocity gradients,
+src/simulation/m_viscous.fpp:159-370, m_riemann_solvers.fpp:169-231 and
+:683-902, m_rhs.fpp:591-603 and :639-651) by what the mathematics fixes.
+
+For a velocity field that is polynomial of degree <= 2 (bilinear for the cross
+terms), at uniform density and pressure, the one-sided differences and their
+four-point averages are exact, so the viscous part of the RHS -- the oracle's
+RHS with Reynolds numbers minus the same RHS without them -- equals the
+Navier-Stokes stress divergence in closed form:
+  momentum  d_j tau_ij,  tau = mu_s (grad u + grad u^T - 2/3 div u I) + mu_b div u I
+  energy    d_j (u_i tau_ij)
+with mu_s = 1/Re(1), mu_b = 1/Re(2).  A wrong 4/3 or 2/3, a sign, a
+transposed gradient or a dropped bulk term fails one of these."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GAMMA = 2.5  # fluid_pp%gamma = 1/(1.4 - 1)
+INTERIOR = np.s_[3:-3]
+
+
+def _case(D, N, Re, nf=1, bc=-3):
+    n = N - 1 if D > 1 else 0
+    return O.Case(m=N - 1, n=n, num_fluids=nf, gammas=[GAMMA] * nf, pi_infs=[0.0] * nf,
+                  bc_beg=(bc,) * 3, bc_end=(bc,) * 3, Re=Re)
+
+
+def _centres(case, grid):
+    out = []
+    b = case.buff_size
+    for d, s in enumerate(case.extents()):
+        cb, ds = grid[d]
+        out.append(cb[b: b + s + 1] + ds[b: b + s + 1] / 2.0)
+    return out
+
+
+def _state(case, grid, vel, alphas=None):
+    """Ghosted conservative state with rho = p = 1 and the given velocity
+    functions of the cell centres (ghost cells are filled by the RHS)."""
+    nf, D = case.num_fluids, case.num_dims
+    c = _centres(case, grid)
+    X, Y = (np.meshgrid(c[0], c[1], indexing="xy") if D > 1 else (c[0][None, :], None))
+    prim = np.zeros((case.sys_size,) + X.shape)
+    al = alphas or [1.0 / nf] * nf
+    for i in range(nf):
+        prim[i] = al[i]           # alpha_i rho_i with rho_i = 1
+        prim[nf + D + 1 + i] = al[i]
+    for d in range(D):
+        prim[nf + d] = vel[d](X, Y)
+    prim[nf + D] = 1.0
+    q = np.zeros(case.ghost_shape)
+    b = case.buff_size
+    cons = O.prim_to_cons(case, prim)
+    if D > 1:
+        q[:, 0, b:-b, b:-b] = cons
+    else:
+        q[:, 0, 0, b:-b] = cons[:, 0, :]
+    return q, X, Y
+
+
+def _viscous_part(case, vel, alphas=None):
+    grid = O.uniform_grid(case)
+    q, X, Y = _state(case, grid, vel, alphas)
+    inviscid = O.Case(**{**case.__dict__, "Re": None})
+    r = O.compute_rhs(case, grid, q.copy()) - O.compute_rhs(inviscid, grid, q.copy())
+    return r[:, 0], X, Y
+
+
+def _check(got, want, scale, tol=1e-9):
+    g = got[INTERIOR, INTERIOR] if got.ndim == 2 and got.shape[0] > 1 else got[..., INTERIOR]
+    w = want[INTERIOR, INTERIOR] if np.ndim(want) == 2 and np.shape(want)[0] > 1 else \
+        (want[..., INTERIOR] if np.ndim(want) else want)
+    err = np.max(np.abs(g - w)) / scale
+    assert err < tol, err
+
+
+def test_1d_quadratic_velocity():
+    """u = a x^2: momentum (4/3 mu_s + mu_b) u_xx; the energy term adds the
+    HLLC face velocity times the stress, close to d_x(u tau_xx)."""
+    a, R1, R2 = 0.3, 0.01, 0.02
+    case = _case(1, 64, Re=[(R1, R2)])
+    r, X, _ = _viscous_part(case, [lambda x, y: a * x * x])
+    mu = 4.0 / 3.0 / R1 + 1.0 / R2
+    _check(r[1], np.full_like(X, mu * 2 * a), mu * 2 * a)
+    # d_x(u mu u_x) = mu (u_x^2 + u u_xx) = 6 mu a^2 x^2, to the O(dx^2) of the
+    # reconstructed face velocity
+    _check(r[2], 6 * mu * a * a * X * X, 6 * mu * a * a, tol=2e-3)
+    assert np.all(r[0] == 0.0) and np.all(r[3] == 0.0)  # mass and volume fraction untouched
+
+
+@pytest.mark.parametrize("which", ["couette_x", "couette_y"])
+def test_2d_couette_heating(which):
+    """u = a y (or v = a x): stresses are uniform, so the momentum terms
+    vanish and the energy gains the viscous heating mu_s a^2 exactly."""
+    a, R1 = 0.7, 0.05
+    case = _case(2, 40, Re=[(R1, -1.0)])
+    vel = [lambda x, y: a * y, lambda x, y: 0.0 * x] if which == "couette_x" else \
+          [lambda x, y: 0.0 * x, lambda x, y: a * x]
+    r, X, Y = _viscous_part(case, vel)
+    heat = a * a / R1
+    _check(r[1], 0.0, heat)
+    _check(r[2], 0.0, heat)
+    _check(r[3], np.full_like(X, heat), heat)
+
+
+def test_2d_quadratic_shear_and_normal():
+    """u = a y^2 + c x^2, v = e x^2 + f y^2: momentum
+    x: mu_s (2a) + (4/3 mu_s + mu_b)(2c);  y: mu_s (2e) + (4/3 mu_s + mu_b)(2f)."""
+    a, c, e, f = 0.4, -0.25, 0.15, 0.35
+    R1, R2 = 0.02, 0.03
+    case = _case(2, 40, Re=[(R1, R2)])
+    r, X, _ = _viscous_part(case, [lambda x, y: a * y * y + c * x * x, lambda x, y: e * x * x + f * y * y])
+    ms, mb = 1.0 / R1, 1.0 / R2
+    wx = ms * 2 * a + (4.0 / 3.0 * ms + mb) * 2 * c
+    wy = ms * 2 * e + (4.0 / 3.0 * ms + mb) * 2 * f
+    _check(r[1], np.full_like(X, wx), abs(wx))
+    _check(r[2], np.full_like(X, wy), abs(wy))
+
+
+def test_2d_bilinear_cross_terms():
+    """u = a x y: tau_xx = 4/3 mu_s a y + mu_b a y, tau_xy = mu_s a x,
+    tau_yy = -2/3 mu_s a y + mu_b a y; momentum x: 0, y: mu_s a/3 + mu_b a."""
+    a, R1, R2 = 0.6, 0.02, 0.05
+    case = _case(2, 40, Re=[(R1, R2)])
+    r, X, _ = _viscous_part(case, [lambda x, y: a * x * y, lambda x, y: 0.0 * x])
+    ms, mb = 1.0 / R1, 1.0 / R2
+    scale = ms * a
+    _check(r[1], np.zeros_like(X), scale)
+    _check(r[2], np.full_like(X, ms * a / 3.0 + mb * a), scale)
+
+
+def test_two_fluid_reynolds_mixture():
+    """Two fluids at alpha = (0.25, 0.75) with different shear Reynolds
+    numbers: the face Reynolds number is 1/(0.25/R_1 + 0.75/R_2), so the
+    Couette heating is a^2 (0.25/R_1 + 0.75/R_2); a fluid without Re(1)
+    drops out of the sum."""
+    a = 0.5
+    for Re, mu in (([(0.01, -1.0), (0.04, -1.0)], 0.25 / 0.01 + 0.75 / 0.04),
+                   ([(0.01, -1.0), (-1.0, -1.0)], 0.25 / 0.01)):
+        case = _case(2, 32, Re=Re, nf=2)
+        r, X, _ = _viscous_part(case, [lambda x, y: a * y, lambda x, y: 0.0 * x], alphas=[0.25, 0.75])
+        heat = a * a * mu
+        _check(r[2 + 2], np.full_like(X, heat), heat)
+
+
+def test_2d_y_uniform_equals_1d():
+    """A 2D field that does not vary in y gives the 1D viscous RHS in every
+    row, bit for bit (the extra 2D terms add exact zeros; 16 rows make the y
+    spacing, and so every y coefficient, exact)."""
+    rng = np.random.default_rng(5)
+    N = 48
+    c1 = _case(1, N, Re=[(0.02, 0.03)])
+    c2 = _case(2, N, Re=[(0.02, 0.03)])
+    c2.n = 15
+    g1, g2 = O.uniform_grid(c1), O.uniform_grid(c2)
+    b = c1.buff_size
+    prim1 = np.zeros((c1.sys_size, N))
+    prim1[0] = 1.0 + 0.2 * rng.random(N)
+    prim1[1] = rng.normal(size=N)
+    prim1[2] = 1.0 + 0.1 * rng.random(N)
+    prim1[3] = 1.0
+    q1 = np.zeros(c1.ghost_shape)
+    q1[:, 0, 0, b:-b] = O.prim_to_cons(c1, prim1)
+    prim2 = np.zeros((c2.sys_size, c2.n + 1, N))
+    prim2[0] = prim1[0]
+    prim2[1] = prim1[1]
+    prim2[3] = prim1[2]
+    prim2[4] = 1.0
+    q2 = np.zeros(c2.ghost_shape)
+    q2[:, 0, b:-b, b:-b] = O.prim_to_cons(c2, prim2)
+    r1 = O.compute_rhs(c1, g1, q1)[:, 0, 0]
+    r2 = O.compute_rhs(c2, g2, q2)[:, 0]
+    for j in range(c2.n + 1):
+        np.testing.assert_array_equal(r2[[0, 1, 3, 4], j], r1)
+
+
+def test_inviscid_entries_leave_rhs_unchanged():
+    """Reynolds numbers <= 0 (the reference's default -1e6 included) are not
+    viscous: the RHS is bitwise the inviscid one."""
+    case = _case(2, 24, Re=[(-1e6, 0.0)])
+    grid = O.uniform_grid(case)
+    q, _, _ = _state(case, grid, [lambda x, y: x * y, lambda x, y: x - y])
+    inv = O.Case(**{**case.__dict__, "Re": None})
+    np.testing.assert_array_equal(O.compute_rhs(case, grid, q.copy()), O.compute_rhs(inv, grid, q.copy()))
