@@ -33,11 +33,16 @@ PROFILED = {("shock_bubble_2d", 2048, "fp64", "sweep_y"): {"traffic": 678.5e6 + 
 
 
 def sweep_bytes_per_step(prm):
-    """Algorithmic HBM bytes per RK3 step of each sweep kernel, per cell:
-    the stencil input is read once (S doubles), rhs is written by the first
+    """Algorithmic HBM bytes per RK3 step of each kernel, per cell: the
+    stencil input is read once (S doubles), rhs is written by the first
     sweep, read and written by middle ones, read by the last, which reads the
-    RK base (stages 2 and 3) and writes the stage output."""
+    RK base (stages 2 and 3) and writes the stage output.  With viscous terms
+    every sweep leaves rhs in HBM; per direction the viscous face kernel
+    reads the stencil input and writes D+1 face fluxes, the divergence kernel
+    reads them and updates D+1 rhs variables; the RK stage kernel reads base,
+    stage input and rhs and writes the stage output."""
     S, D = prm.sys_size, prm.num_dims
+    fused = not prm.viscous
     out = {}
     names = ["sweep_x", "sweep_y", "sweep_z"][:D]
     for d, nm in enumerate(names):
@@ -48,7 +53,7 @@ def sweep_bytes_per_step(prm):
             w = 0
             if not first:
                 r += S  # rhs partial
-            if last:
+            if last and fused:
                 if stage > 0:
                     r += S  # RK base q1
                 w += S  # stage output
@@ -56,6 +61,9 @@ def sweep_bytes_per_step(prm):
                 w += S  # rhs
             per_stage.append(8 * (r + w))
         out[nm] = sum(per_stage)
+    if not fused:
+        out["viscous"] = 3 * D * 8 * (S + 4 * (D + 1))
+        out["rk_stage"] = 3 * 8 * 4 * S
     return out
 
 
@@ -65,7 +73,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--case", default="shock_bubble_2d",
-                    choices=["shock_bubble_2d", "shock_droplet_3d", "shear_layer_2d", "bubble_cloud_3d", "sod_1d"])
+                    choices=["shock_bubble_2d", "shock_droplet_3d", "shear_layer_2d", "bubble_cloud_3d", "sod_1d",
+                             "viscous_shear_2d"])
     ap.add_argument("--N", type=int, default=0, help="cells per direction (default: the case's size)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_bulk", "fp32_learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -105,6 +114,12 @@ def main():
         N = args.N or 768
         prm, grids, q = ic.bubble_cloud_3d(N=N, precision=prec)
         workload = f"3D shock-bubble cloud water/air {N}^3 (64 air bubbles), WENO5+HLLC+RK3"
+    elif args.case == "viscous_shear_2d":
+        N = args.N or 2048
+        prm, grids, q = ic.viscous_shear_2d(N=N)
+        prm.precision = prec
+        workload = (f"2D viscous shear layer (the reference's 2D_viscous case) {N}x{N}, two fluids, Re 1e-4, "
+                    "WENO5+HLLC+viscous+RK3")
     elif args.case == "shear_layer_2d":
         N = args.N or 4096
         prm, grids, q = ic.shear_layer_2d(N=N, precision=prec)
@@ -128,6 +143,8 @@ def main():
     rhs = mod.new_field(ghosted=False)
     cfl = 0.4
     dt = mod.cfl_dt(q1, cfl)
+    if prm.viscous:
+        dt = min(dt, ic.viscous_dt(prm, grids, q))
     for t in range(args.warmup):
         mod.s_tvd_rk(t, (q1, q2), None, rhs, dt)
     mod.sync()
@@ -152,7 +169,7 @@ def main():
     per_step = sweep_bytes_per_step(prm)
     dom = max(per_step, key=lambda k: timing[k][0])
     dom_ms, dom_n = timing[dom]
-    dom_bytes_per_launch = per_step[dom] * ncell / 3.0  # three launches per step
+    dom_bytes_per_launch = per_step[dom] * ncell / (timing[dom][1] / args.steps)
     dom_avg_s = dom_ms / 1e3 / dom_n
     achieved = dom_bytes_per_launch / dom_avg_s / 1e9
     prof = PROFILED.get((args.case, N, args.precision, dom), {})
@@ -186,6 +203,16 @@ def main():
                       "peak_TFLOPs": FP64_PEAK_TFLOPS, "frac": f_alg * value / 1e12 / FP64_PEAK_TFLOPS},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in timing.items() if v[1]},
     }
+    # every kernel of the step against HBM peak on its algorithmic bytes
+    kr = {}
+    for k, bps in per_step.items():
+        ms, n = timing[k]
+        if n:
+            per_launch = bps * ncell / (n / args.steps)
+            gbs = per_launch / (ms / 1e3 / n) / 1e9
+            kr[k] = {"avg_launch_ms": ms / n, "algorithmic_bytes_per_launch": per_launch, "achieved_GBps": gbs,
+                     "frac": gbs / HBM_PEAK_GBS, "launches_per_step": n / args.steps}
+    out["kernels_roofline"] = kr
     if not args.no_aux:
         out["aux_kernels"] = aux_kernels(mod, prm, grids, q1, dt, args, RhsModule, torch)
     else:
@@ -259,7 +286,8 @@ def cpu_baseline(args, prm, grids, q, np):
     from oracle import oracle as O
     oc = O.Case(m=prm.m, n=prm.n, p=prm.p, num_fluids=prm.num_fluids, weno_order=prm.weno_order,
                 time_stepper=prm.time_stepper, weno_eps=prm.weno_eps, gammas=list(prm.gammas),
-                pi_infs=list(prm.pi_infs), bc_beg=prm.bc_beg, bc_end=prm.bc_end)
+                pi_infs=list(prm.pi_infs), bc_beg=prm.bc_beg, bc_end=prm.bc_end,
+                Re=[tuple(r) for r in prm.Re] if prm.Re else None)
     ncell = (prm.m + 1) * (prm.n + 1) * (prm.p + 1)
     if ncell > 20_000_000:
         return {"value": None, "unit": "cell-updates/s", "cores": None, "kind": "port",

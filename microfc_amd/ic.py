@@ -195,6 +195,47 @@ def shear_layer_2d(N: int = 4096, weno_order: int = 5, seed: int = 0, modes: int
     return prm, grids, ghosted(prm, prim_to_cons(prm, prim))
 
 
+def viscous_shear_2d(N: int = 2048, weno_order: int = 5, Re: float = 1e-4):
+    """The reference's viscous case (examples/2D_viscous/case.py) at N^2 cells:
+    [-0.5, 0.5]^2, two water-like stiffened gases (gamma 4.4, p_inf 6e8) with
+    shear and volume Reynolds numbers Re; the lower half holds both fluids at
+    alpha = 0.5, alpha_rho = 1000 each, u = +500; the upper half (patch 2,
+    centred at y = 0.25, length 0.5) only fluid 2 at alpha_rho = 1, u = -500;
+    p = 1e6 everywhere.  Periodic in x; extrapolation in y (the example's bc
+    -6 fills ghosts by extrapolation too).  The example's weno_Re_flux = T is
+    replaced by finite-difference gradients."""
+    g, pi = eos_inputs(4.4, 6.0e8)
+    prm = Params(m=N - 1, n=N - 1, num_fluids=2, weno_order=weno_order, gammas=[g, g], pi_infs=[pi, pi],
+                 bc_beg=(-1, -6, -3), bc_end=(-1, -6, -3), Re=[(Re, Re), (Re, Re)])
+    grids = uniform_grids(prm, lo=(-0.5, -0.5, 0.0), hi=(0.5, 0.5, 1.0))
+    X, Y, _ = mesh(prm, grids)
+    top = (Y >= 0.0) & (Y <= 0.5)  # patch 2: |y - 0.25| <= 0.25
+    prim = np.zeros(prm.interior_shape)
+    prim[0] = np.where(top, 0.0, 1000.0)
+    prim[1] = np.where(top, 1.0, 1000.0)
+    prim[2] = np.where(top, -500.0, 500.0)
+    prim[3] = 0.0
+    prim[4] = 1.0e6
+    prim[5] = np.where(top, 0.0, 0.5)
+    prim[6] = np.where(top, 1.0, 0.5)
+    return prm, grids, ghosted(prm, prim_to_cons(prm, prim))
+
+
+def viscous_dt(prm: Params, grids, q, safety: float = 0.2) -> float:
+    """Explicit viscous time-step bound safety * min(dx)^2 / nu_max, with
+    nu = (4/3 mu_s + mu_b) / rho and mu = 1/Re of the most viscous fluid (the
+    CFL reduction bounds only the inviscid wave speeds)."""
+    nf = prm.num_fluids
+    mu = 0.0
+    for pair in prm.Re or []:
+        ms = 1.0 / pair[0] if pair[0] > 0 else 0.0
+        mb = 1.0 / pair[1] if pair[1] > 0 else 0.0
+        mu = max(mu, 4.0 / 3.0 * ms + mb)
+    rho = sum(prm.interior(q)[i] for i in range(nf))
+    dx = min(float(np.min(g[1][prm.buff_size: -prm.buff_size])) for g in grids)
+    return safety * dx * dx * float(np.min(rho)) / mu if mu > 0 else math.inf
+
+
 def bubble_cloud_centres(nbubbles: int = 64, seed: int = 0, r_lo: float = 0.02, r_hi: float = 0.05,
                          gap: float = 0.01, x_lo: float = 0.2, margin: float = 0.05):
     """Seeded, non-overlapping bubble centres and radii in [0,1]^3: radius

@@ -43,3 +43,21 @@ def test_cloud_slabs_equal_whole_domain_evaluation():
     shocked = (xc < 0.15)[None, None, :]
     expect = np.broadcast_to(np.where(shocked, 1e9, 1e5), p.shape)
     assert np.allclose(p, expect, rtol=1e-6, atol=0)
+
+
+def test_viscous_shear_matches_the_reference_example():
+    """examples/2D_viscous/case.py: lower half both fluids at alpha 0.5 and
+    alpha_rho 1000, u = 500; upper half fluid 2 only, alpha_rho 1, u = -500;
+    p = 1e6; Re(1:2) = 1e-4 for both fluids; periodic x."""
+    prm, grids, q = ic.viscous_shear_2d(N=32)
+    assert prm.Re == [(1e-4, 1e-4), (1e-4, 1e-4)] and prm.viscous
+    assert prm.bc_beg[0] == prm.bc_end[0] == -1
+    qi = prm.interior(q)[:, 0]
+    low, high = qi[:, 4, 4], qi[:, 28, 4]  # y = -0.36 and 0.39
+    assert np.allclose(low[[0, 1, 2, 5, 6]], [1000.0, 1000.0, 2000.0 * 500.0, 0.5, 0.5])
+    assert np.allclose(high[[0, 1, 2, 5, 6]], [0.0, 1.0, -500.0, 0.0, 1.0])
+    # E = Gamma p + Pi + rho u^2 / 2 with Gamma = 1/3.4, Pi = 4.4 * 6e8 / 3.4
+    G, P = 1 / 3.4, 4.4 * 6e8 / 3.4
+    assert np.isclose(high[4], G * 1e6 + P + 0.5 * 500.0 ** 2)
+    dt = ic.viscous_dt(prm, grids, q)
+    assert 0 < dt < 0.2 * (1 / 32) ** 2 * 1.0 / (4 / 3 * 1e4) * 1.0001
