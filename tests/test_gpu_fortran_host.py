@@ -64,6 +64,24 @@ def test_fortran_driver_bubble_with_run_time_info(gpu, tmp_path):
     assert abs(float(lines[0][2]) - icfl0) < 1e-6
 
 
+def test_fortran_driver_reference_viscous_example(gpu, tmp_path):
+    """examples/2D_viscous/case.py as the reference runs it (m = n = 50,
+    dt = 5e-10, 10 steps, Re 1e-4 for both fluids, periodic x) but with
+    finite-difference gradients: the Fortran host's fluid_pp%Re reach the
+    library, and the run equals the oracle's viscous steps."""
+    prm, grids, q = ic.viscous_shear_2d(N=51)
+    q = perturbed(prm, q)
+    dt, nsteps = 5e-10, 10
+    _run(tmp_path, prm, grids, q, dt, nsteps, 5)
+    for t in (5, 10):
+        got = C.read_serial(str(tmp_path), t, prm)
+        assert_parity(got, _oracle_steps(prm, grids, q, dt, nsteps, t), what=f"viscous example step {t}")
+    # the viscous terms moved the state: the inviscid oracle differs
+    inv = _oracle_steps(prm.__class__(**{**prm.__dict__, "Re": None}), grids, q, dt, nsteps, 10)
+    ref = _oracle_steps(prm, grids, q, dt, nsteps, 10)
+    assert np.max(np.abs(inv[2] - ref[2])) > 1e-6 * np.max(np.abs(ref[2]))
+
+
 def test_fortran_driver_aborts_on_cfl_violation(gpu, tmp_path):
     prm, grids, q = ic.sod_1d(m=99)
     case = C.case_from_params(prm, dt=5e-2, t_step_stop=4, t_step_save=2, run_time_info=True)
