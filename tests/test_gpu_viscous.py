@@ -57,6 +57,10 @@ def _cases():
     prm.Re = [(-1.0, 0.05), (-1.0, 0.1)]
     g = stretched_grids(prm)
     out.append(("visc_bubble2d_stretched_bulk_only", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
+    prm, g, q = ic.shock_bubble_2d(N=40)
+    prm.Re = [(0.05, 0.1), (0.1, -1.0)]
+    g = stretched_grids(prm)
+    out.append(("visc_bubble2d_stretched_shear_bulk", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
     prm, g, q = multi_fluid_2d(3, N=32)
     prm.Re = [(0.3, -1.0), (-1.0, 0.7), (0.9, 1.1)]
     out.append(("visc_fluids3_2d", prm, g, q))
