@@ -362,53 +362,69 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Per-cell ICFL (m_data_output.fpp:197-244) and a block max-reduction by
-// wavefront shuffles; block results go to partials[blockIdx.x].  FROM_CONS
-// converts the conservative field in place of reading q_prim_vf; the values
-// are identical because the reference's q_prim_vf is that same conversion.
+// ICFL of one cell (m_data_output.fpp:197-244) from its S values: primitive
+// ones, or conservative ones converted first (FROM_CONS; the values are
+// identical because the reference's q_prim_vf is that same conversion).
+template <int NF, int D, bool FROM_CONS>
+__device__ __forceinline__ double icfl_cell(const double *cv, const Eos &eos, const double *dx, const double *dy,
+                                            const double *dz, int i, int j, int k, double dt) {
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    double pr[S];
+    if constexpr (FROM_CONS) {
+        cons_to_prim<NF, D>(cv, pr, eos);
+    } else {
+#pragma unroll
+        for (int v = 0; v < S; ++v) pr[v] = cv[v];
+    }
+    double rho = 0.0, gam = 0.0, pinf = 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        rho = rho + pr[f];
+        gam = gam + pr[Y::ADV + f] * eos.gammas[f];
+        pinf = pinf + pr[Y::ADV + f] * eos.pi_infs[f];
+    }
+    const double pres = pr[Y::E];
+    const double c = sqrt(((gam + 1.0) * pres + pinf) / (gam * rho));
+    if constexpr (D == 1) {
+        return (dt / dx[i]) * (fabs(pr[Y::MOM]) + c);
+    } else if constexpr (D == 2) {
+        return dt / min_ref(dx[i] / (fabs(pr[Y::MOM]) + c), dy[j] / (fabs(pr[Y::MOM + 1]) + c));
+    } else {
+        return dt / min_ref(min_ref(dx[i] / (fabs(pr[Y::MOM]) + c), dy[j] / (fabs(pr[Y::MOM + 1]) + c)),
+                            dz[k] / (fabs(pr[Y::MOM + 2]) + c));
+    }
+}
+
+// Maximum ICFL over the interior and a block max-reduction by wavefront
+// shuffles; block results go to partials[blockIdx.x].  A block takes chunks
+// of 512 consecutive interior cells (x fastest), two per thread with both
+// cells' loads issued before either is converted.
 template <int NF, int D, bool FROM_CONS>
 __global__ __launch_bounds__(256) void k_icfl(Fields q, Eos eos, Box x, const double *dx, const double *dy,
                                               const double *dz, double dt, long nint, double *partials) {
-    using Y = Sys<NF, D>;
-    constexpr int S = Y::S;
+    constexpr int S = Sys<NF, D>::S;
     double mx = -INFINITY;
-    // blocks take whole x rows (no per-cell index division), threads stride
-    // along the row
-    const long nrows = (long)(x.n + 1) * (x.p + 1);
-    for (long r = blockIdx.x; r < nrows; r += gridDim.x)
-    for (int i = threadIdx.x; i <= x.m; i += blockDim.x) {
-        const int j = (int)(r % (x.n + 1));
-        const int k = (int)(r / (x.n + 1));
-        const long o = box_off(x, i, j, k);
-        double pr[S];
-        if constexpr (FROM_CONS) {
-            double c[S];
+    for (long c0 = (long)blockIdx.x * 512; c0 < nint; c0 += (long)gridDim.x * 512) {
+        double cv[2][S];
+        int ci[2], cj[2], ck[2];
+        bool ok[2];
 #pragma unroll
-            for (int v = 0; v < S; ++v) c[v] = q.v[v][o];
-            cons_to_prim<NF, D>(c, pr, eos);
-        } else {
+        for (int h = 0; h < 2; ++h) {
+            const long e = c0 + threadIdx.x + 256 * h;
+            ok[h] = e < nint;
+            const long ee = ok[h] ? e : 0;
+            const long r = ee / (x.m + 1);
+            ci[h] = (int)(ee - r * (x.m + 1));
+            cj[h] = (int)(r % (x.n + 1));
+            ck[h] = (int)(r / (x.n + 1));
+            const long o = box_off(x, ci[h], cj[h], ck[h]);
 #pragma unroll
-            for (int v = 0; v < S; ++v) pr[v] = q.v[v][o];
+            for (int v = 0; v < S; ++v) cv[h][v] = q.v[v][o];
         }
-        double rho = 0.0, gam = 0.0, pinf = 0.0;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            rho = rho + pr[f];
-            gam = gam + pr[Y::ADV + f] * eos.gammas[f];
-            pinf = pinf + pr[Y::ADV + f] * eos.pi_infs[f];
-        }
-        const double pres = pr[Y::E];
-        const double c = sqrt(((gam + 1.0) * pres + pinf) / (gam * rho));
-        double icfl;
-        if constexpr (D == 1) {
-            icfl = (dt / dx[i]) * (fabs(pr[Y::MOM]) + c);
-        } else if constexpr (D == 2) {
-            icfl = dt / min_ref(dx[i] / (fabs(pr[Y::MOM]) + c), dy[j] / (fabs(pr[Y::MOM + 1]) + c));
-        } else {
-            icfl = dt / min_ref(min_ref(dx[i] / (fabs(pr[Y::MOM]) + c), dy[j] / (fabs(pr[Y::MOM + 1]) + c)),
-                                dz[k] / (fabs(pr[Y::MOM + 2]) + c));
-        }
-        mx = max_nan(mx, icfl);
+        for (int h = 0; h < 2; ++h)
+            if (ok[h]) mx = max_nan(mx, icfl_cell<NF, D, FROM_CONS>(cv[h], eos, dx, dy, dz, ci[h], cj[h], ck[h], dt));
     }
     __shared__ double red[4];
     mx = wave_max(mx);
@@ -836,8 +852,7 @@ static int icfl_reduce(double *const *q, bool from_cons, double dt, double *out)
     const Box x = box();
     const Eos e = eos();
     double *dxs[3] = {G.dir[0].ds_dev, G.dir[1].ds_dev, G.dir[2].ds_dev};
-    const long nrows = (long)(G.prm.n + 1) * (G.prm.p + 1);
-    const int nb = (int)std::min<long>(kReduceBlocks, std::max<long>(1, nrows));
+    const int nb = (int)std::min<long>(kReduceBlocks, std::max<long>(1, (G.nint + 511) / 512));
     int rc;
     {
     TimedScope ts(kKIcfl); // the per-cell pass (the single-block final reduction is not included)
