@@ -630,6 +630,8 @@ int viscous(int d, const Fields &qin, const Fields &rhs) {
     v.nface = d == 0 ? (long)(G.prm.m + 2) * (G.prm.n + 1) : (long)(G.prm.m + 1) * (G.prm.n + 2);
     v.coef = G.dir[d].coef;
     v.ncell = G.dir[d].s + 2 * G.b + 1;
+    v.uniform = G.dir[d].uniform ? 1 : 0;
+    for (int q = 0; q < kNCoef; ++q) v.ucf[q] = G.dir[d].ucf[q];
     v.ccx = G.dir[0].cc_dev;
     v.ccy = G.D > 1 ? G.dir[1].cc_dev : nullptr;
     v.inv_ds = G.dir[d].inv_ds;

@@ -5,14 +5,14 @@
 // The inviscid sweeps keep face states in registers and LDS only, so the
 // viscous flux, which needs the HLLC star velocity and the reconstructed
 // volume fractions of every face (m_riemann_solvers.fpp:169-231, :313-323),
-// recomputes them here: one thread per face reconstructs the two face states
-// of its stencil (the same weno_cell and hllc_face as the sweeps, hence the
-// same bits), takes the velocity gradients from the conservative field, and
-// writes the momentum and energy source flux of the face
-// (m_riemann_solvers.fpp:683-902).  A second kernel adds its divergence to
-// rhs after the direction's inviscid terms, in the reference's order
-// (m_rhs.fpp:591-603, :639-651).  No viscous benchmark case exists, so these
-// kernels favour plainness over speed.
+// recomputes them here with the sweeps' own weno_cell and hllc_face (hence
+// the same bits): each cell is converted to primitive and reconstructed once
+// (both faces), in LDS and lane shuffles along x and in a register window
+// marching along y; the velocity gradients come from the primitive values
+// and the conservative field; the kernel writes the momentum and energy
+// source flux of every face (m_riemann_solvers.fpp:683-902).  A second
+// kernel adds its divergence to rhs after the direction's inviscid terms, in
+// the reference's order (m_rhs.fpp:591-603, :639-651).
 #include "mfc_viscous.hpp"
 
 namespace mfc {
@@ -35,19 +35,6 @@ __device__ __forceinline__ double cell_vel(const ViscArgs &a, int i, int j, int 
     return a.q.v[Sys<NF, D>::MOM + d][o] / rho;
 }
 
-// One-sided x difference across the face between cells x and x+1
-// (dqR_prim_dx_n(1) at x, dqL_prim_dx_n(1) at x+1, m_viscous.fpp:234-255).
-template <int NF, int D>
-__device__ __forceinline__ double dx1(const ViscArgs &a, int x, int y, int d) {
-    return (cell_vel<NF, D>(a, x + 1, y, d) - cell_vel<NF, D>(a, x, y, d)) / (a.ccx[x + 1 + a.b] - a.ccx[x + a.b]);
-}
-// One-sided y difference across the face between cells y and y+1
-// (dqR_prim_dy_n(2) at y, dqL_prim_dy_n(2) at y+1, :259-283).
-template <int NF, int D>
-__device__ __forceinline__ double dy2(const ViscArgs &a, int x, int y, int d) {
-    return (cell_vel<NF, D>(a, x, y + 1, d) - cell_vel<NF, D>(a, x, y, d)) / (a.ccy[y + 1 + a.b] - a.ccy[y + a.b]);
-}
-
 // Harmonic face Reynolds numbers (m_riemann_solvers.fpp:169-231).
 template <int NF, int D>
 __device__ __forceinline__ void re_avg(const ViscArgs &a, const double *ql, const double *qr, double *ra) {
@@ -67,108 +54,52 @@ __device__ __forceinline__ void re_avg(const ViscArgs &a, const double *ql, cons
     }
 }
 
-// Viscous source flux of one face.  DIR 0: faces f = -1..m of row y, left
-// cell (f, y); DIR 1: faces f = -1..n of column x, left cell (x, f).
-template <int NF, int D, int DIR, int WO>
-__global__ __launch_bounds__(256) void k_visc_face(ViscArgs a) {
-    using Y = Sys<NF, D>;
-    constexpr int S = Y::S;
-    constexpr int P = (WO - 1) / 2;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nface) return;
-    int f, x, y, s;
-    if constexpr (DIR == 0) {
-        f = (int)(t % (a.m + 2)) - 1;
-        y = (int)(t / (a.m + 2));
-        x = f;
-        s = a.m;
-    } else {
-        x = (int)(t % (a.m + 1));
-        f = (int)(t / (a.m + 1)) - 1;
-        y = f;
-        s = a.n;
-    }
-    // face states: right face of cell f and left face of cell f+1
-    double pr[2 * P + 2][S];
+// WENO coefficients of cell c of the direction: kernel arguments (scalar
+// registers) on uniform grids, as in the sweeps, else per-cell loads; the
+// same values either way.
+template <bool UNI>
+__device__ __forceinline__ void cell_coef(const ViscArgs &a, int c, double *cf) {
 #pragma unroll
-    for (int r = 0; r < 2 * P + 2; ++r) {
-        const int c = f - P + r;
-        const long o = DIR == 0 ? goff(a, c, y) : goff(a, x, c);
-        double cv[S];
-#pragma unroll
-        for (int v = 0; v < S; ++v) cv[v] = a.q.v[v][o];
-        cons_to_prim<NF, D>(cv, pr[r], a.eos);
-    }
-    double cf0[kNCoef], cf1[kNCoef];
-#pragma unroll
-    for (int q = 0; q < kNCoef; ++q) {
-        cf0[q] = a.coef[(long)q * a.ncell + (f + a.b)];
-        cf1[q] = a.coef[(long)q * a.ncell + (f + 1 + a.b)];
-    }
-    double ql[S], qr[S];
-#pragma unroll
-    for (int v = 0; v < S; ++v) {
-        double st0[WO], st1[WO], dummy;
-#pragma unroll
-        for (int r = 0; r < WO; ++r) {
-            st0[r] = pr[r][v];
-            st1[r] = pr[r + 1][v];
-        }
-        weno_cell<WO, double, false, true>(st0, cf0, a.eps, dummy, ql[v]);
-        weno_cell<WO, double, true, false>(st1, cf1, a.eps, qr[v], dummy);
-    }
-    // Riemann-state extrapolation (bc = -4, m_riemann_solvers.fpp:480-550)
-    if (f == -1 && a.bc_beg == -4) {
-#pragma unroll
-        for (int v = 0; v < S; ++v) ql[v] = qr[v];
-    }
-    if (f == s && a.bc_end == -4) {
-#pragma unroll
-        for (int v = 0; v < S; ++v) qr[v] = ql[v];
-    }
-    double flux[S], ustar, vsrc[D];
+    for (int q = 0; q < kNCoef; ++q) cf[q] = UNI ? a.ucf[q] : a.coef[(long)q * a.ncell + (c + a.b)];
+}
+
+// Viscous source flux of one face (m_riemann_solvers.fpp:683-902) from its
+// two face states, star velocity and gradients.  The reference passes
+// dqR_prim_d*_n as the solver's left gradients and dqL_prim_d*_n as its right
+// ones (m_rhs.fpp:545-556); both sides of a face evaluate the same
+// difference, so each average below is of two equal terms, kept as written.
+// gxn[d]: normal-direction one-sided difference of velocity component d
+// across the face; gt[d]: the four-point average of the transverse
+// differences (2D).
+template <int NF, int D, int DIR>
+__device__ __forceinline__ void face_flux(const ViscArgs &a, const double *ql, const double *qr, const double *gxn,
+                                          const double *gt, double *fs) {
+    double flux[Sys<NF, D>::S], ustar, vsrc[D];
     hllc_face<NF, D, DIR, double, true>(ql, qr, a.eos, flux, ustar, vsrc);
     double ra[2];
     re_avg<NF, D>(a, ql, qr, ra);
-
-    // The reference passes dqR_prim_d*_n as the solver's left gradients and
-    // dqL_prim_d*_n as its right ones (m_rhs.fpp:545-556); both sides of a
-    // face evaluate the same difference, so each average below is of two
-    // equal terms, kept as written (m_riemann_solvers.fpp:719-720 etc.).
-    double fs[D + 1];
 #pragma unroll
     for (int v = 0; v <= D; ++v) fs[v] = 0.0;
     if constexpr (DIR == 0) {
-        const double gxu = dx1<NF, D>(a, x, y, 0);
+        // gxn = d(u, v)/dx across the face, gt = d(u, v)/dy averaged
         if (a.vp.size[0] > 0) {
-            const double dvx = 5e-1 * (gxu + gxu);
+            const double dvx = 5e-1 * (gxn[0] + gxn[0]);
             const double t11 = (4.0 / 3.0) * dvx / ra[0];
             fs[0] = fs[0] - t11;
             fs[D] = fs[D] - vsrc[0] * t11;
         }
         if (a.vp.size[1] > 0) {
-            const double dvx = 5e-1 * (gxu + gxu);
+            const double dvx = 5e-1 * (gxn[0] + gxn[0]);
             const double t11 = dvx / ra[1];
             fs[0] = fs[0] - t11;
             fs[D] = fs[D] - vsrc[0] * t11;
         }
         if constexpr (D == 2) {
-            // y derivatives at the x face: four-point averages of the
-            // one-sided y differences of cells x and x+1
-            // (dqR_prim_dy_n(1) at x, dqL_prim_dy_n(1) at x+1, m_viscous.fpp:318-336)
-            double gy[2];
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                const double sum = dy2<NF, D>(a, x + 1, y - 1, d) + dy2<NF, D>(a, x + 1, y, d) +
-                                   dy2<NF, D>(a, x, y - 1, d) + dy2<NF, D>(a, x, y, d);
-                gy[d] = 25e-2 * sum;
-            }
             if (a.vp.size[0] > 0) {
                 double dvy[2], tau[2];
 #pragma unroll
-                for (int d = 0; d < 2; ++d) dvy[d] = 5e-1 * (gy[d] + gy[d]);
-                const double gxv = dx1<NF, D>(a, x, y, 1);
-                const double dvx2 = 5e-1 * (gxv + gxv);
+                for (int d = 0; d < 2; ++d) dvy[d] = 5e-1 * (gt[d] + gt[d]);
+                const double dvx2 = 5e-1 * (gxn[1] + gxn[1]);
                 tau[0] = -((2.0 / 3.0) * dvy[1] / ra[0]);
                 tau[1] = (dvy[0] + dvx2) / ra[0];
 #pragma unroll
@@ -178,30 +109,20 @@ __global__ __launch_bounds__(256) void k_visc_face(ViscArgs a) {
                 }
             }
             if (a.vp.size[1] > 0) {
-                const double dvy2 = 5e-1 * (gy[1] + gy[1]);
+                const double dvy2 = 5e-1 * (gt[1] + gt[1]);
                 const double t11 = dvy2 / ra[1];
                 fs[0] = fs[0] - t11;
                 fs[D] = fs[D] - vsrc[0] * t11;
             }
         }
     } else {
-        // x derivatives at the y face: four-point averages of the one-sided
-        // x differences of rows y and y+1 (dqR_prim_dx_n(2) at y,
-        // dqL_prim_dx_n(2) at y+1, m_viscous.fpp:285-315)
-        double gx[2], gyv[2];
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-            const double sum = dx1<NF, D>(a, x - 1, y + 1, d) + dx1<NF, D>(a, x, y + 1, d) +
-                               dx1<NF, D>(a, x - 1, y, d) + dx1<NF, D>(a, x, y, d);
-            gx[d] = 25e-2 * sum;
-            gyv[d] = dy2<NF, D>(a, x, y, d);
-        }
+        // gxn = d(u, v)/dy across the face, gt = d(u, v)/dx averaged
         if (a.vp.size[0] > 0) {
             double dvx[2], dvy[2], tau[2];
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
-                dvx[d] = 5e-1 * (gx[d] + gx[d]);
-                dvy[d] = 5e-1 * (gyv[d] + gyv[d]);
+                dvx[d] = 5e-1 * (gt[d] + gt[d]);
+                dvy[d] = 5e-1 * (gxn[d] + gxn[d]);
             }
             tau[0] = (dvy[0] + dvx[1]) / ra[0];
             tau[1] = (4.0 * dvy[1] - 2.0 * dvx[0]) / (3.0 * ra[0]);
@@ -212,15 +133,210 @@ __global__ __launch_bounds__(256) void k_visc_face(ViscArgs a) {
             }
         }
         if (a.vp.size[1] > 0) {
-            const double dvx1 = 5e-1 * (gx[0] + gx[0]);
-            const double dvy2 = 5e-1 * (gyv[1] + gyv[1]);
+            const double dvx1 = 5e-1 * (gt[0] + gt[0]);
+            const double dvy2 = 5e-1 * (gxn[1] + gxn[1]);
             const double t22 = (dvx1 + dvy2) / ra[1];
             fs[1] = fs[1] - t22;
             fs[D] = fs[D] - vsrc[1] * t22;
         }
     }
+}
+
+// x faces: one wave per 63 faces f0 .. f0+62 of row y (f0 = -1 + 63*tile).
+// Lane l converts stencil cells into LDS once, then reconstructs both faces
+// of cell c = f0 + l (the smoothness indicators shared, as in the
+// cell-parallel sweeps); face f = c takes its right state, the left-face
+// value of cell c+1, from the next lane.  No register cap: the kernel takes
+// 230 VGPRs (2 waves per SIMD) for two fluids in 2D; capped at 4 or 3 waves
+// it spills and is slower.
+template <int NF, int D, int WO, bool UNI>
+__global__ __launch_bounds__(64) void k_visc_face_x(ViscArgs a, int ntile) {
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    constexpr int P = (WO - 1) / 2;
+    constexpr int NS = 65 + 2 * P; // staged cells f0-P .. f0+64+P
+    __shared__ double sp[S][NS];
+    const int lane = threadIdx.x;
+    const int tile = (int)(blockIdx.x % ntile);
+    const int y = (int)(blockIdx.x / ntile);
+    const int f0 = -1 + 63 * tile;
+    const int hi = a.m + a.b; // last ghost cell
+    for (int k = lane; k < NS; k += 64) {
+        const int c = min(f0 - P + k, hi);
+        const long o = goff(a, c, y);
+        double cv[S], pv[S];
+#pragma unroll
+        for (int v = 0; v < S; ++v) cv[v] = a.q.v[v][o];
+        cons_to_prim<NF, D>(cv, pv, a.eos);
+#pragma unroll
+        for (int v = 0; v < S; ++v) sp[v][k] = pv[v];
+    }
+    __syncthreads();
+    const int c = min(f0 + lane, hi - P); // this lane's cell (clamped past the last face)
+    double cf[kNCoef];
+    cell_coef<UNI>(a, c, cf);
+    double vl[S], ql[S], qr[S];
+#pragma unroll
+    for (int v = 0; v < S; ++v) {
+        double st[WO];
+#pragma unroll
+        for (int r = 0; r < WO; ++r) st[r] = sp[v][lane + r];
+        weno_cell<WO, double, true, true>(st, cf, a.eps, vl[v], ql[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < S; ++v) qr[v] = __shfl_down(vl[v], 1, 64);
+    // velocities of rows y-1 and y+1 of this lane's cell, then of the next
+    // lane's (2D transverse gradients)
+    double gt[2] = {0.0, 0.0};
+    if constexpr (D == 2) {
+        double um[2], up[2], um1[2], up1[2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            um[d] = cell_vel<NF, D>(a, c, y - 1, d);
+            up[d] = cell_vel<NF, D>(a, c, y + 1, d);
+        }
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            um1[d] = __shfl_down(um[d], 1, 64);
+            up1[d] = __shfl_down(up[d], 1, 64);
+        }
+        const double dym = a.ccy[y + a.b] - a.ccy[y - 1 + a.b], dyp = a.ccy[y + 1 + a.b] - a.ccy[y + a.b];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const double u0 = sp[Y::MOM + d][lane + P], u1 = sp[Y::MOM + d][lane + P + 1];
+            // four-point average of the one-sided y differences of cells
+            // x+1 and x (dqR_prim_dy_n(1) at x, m_viscous.fpp:318-336)
+            const double sum = (u1 - um1[d]) / dym + (up1[d] - u1) / dyp + (u0 - um[d]) / dym + (up[d] - u0) / dyp;
+            gt[d] = 25e-2 * sum;
+        }
+    }
+    const int f = f0 + lane;
+    if (lane == 63 || f > a.m) return;
+    if (f == -1 && a.bc_beg == -4) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) ql[v] = qr[v];
+    }
+    if (f == a.m && a.bc_end == -4) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) qr[v] = ql[v];
+    }
+    // one-sided x differences across the face (dqR_prim_dx_n(1) at x)
+    double gxn[2] = {0.0, 0.0};
+    const double dxf = a.ccx[f + 1 + a.b] - a.ccx[f + a.b];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gxn[d] = (sp[Y::MOM + d][lane + P + 1] - sp[Y::MOM + d][lane + P]) / dxf;
+    double fs[D + 1];
+    face_flux<NF, D, 0>(a, ql, qr, gxn, gt, fs);
+    const long t = (long)y * (a.m + 2) + (f + 1);
 #pragma unroll
     for (int v = 0; v <= D; ++v) a.fs[(long)v * a.nface + t] = fs[v];
+}
+
+// y faces (2D): thread = x column, marching along y over the L faces
+// f0 .. f0+L-1 of its chunk (f0 = -1 + L*chunk) with the primitive stencil
+// in a register window, so each cell is converted and reconstructed once
+// (both faces); x-neighbour velocities come from the conservative field.
+// The window and three face states hold about 250 VGPRs (one wave per SIMD);
+// capped at 3 or 4 waves the kernel spills 0.5-0.9 KB per lane and is
+// slower.
+template <int NF, int WO, int L, bool UNI>
+__global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
+    constexpr int D = 2;
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    constexpr int P = (WO - 1) / 2;
+    constexpr int W = 2 * P + 1;
+    const int x = (int)(blockIdx.x % nxb) * 256 + threadIdx.x;
+    const int chunk = (int)(blockIdx.x / nxb);
+    if (x > a.m) return;
+    const int f0 = -1 + L * chunk;
+    const int hi = a.n + a.b;
+    auto prim = [&](int cy, double *pv) {
+        const long o = goff(a, x, min(cy, hi));
+        double cv[S];
+#pragma unroll
+        for (int v = 0; v < S; ++v) cv[v] = a.q.v[v][o];
+        cons_to_prim<NF, D>(cv, pv, a.eos);
+    };
+    double w[W][S]; // cells cur-P .. cur+P
+#pragma unroll
+    for (int r = 0; r < W; ++r) prim(f0 - P + r, w[r]);
+    const double dxm = a.ccx[x + a.b] - a.ccx[x - 1 + a.b], dxp = a.ccx[x + 1 + a.b] - a.ccx[x + a.b];
+    double cf[kNCoef], ql[S], dummy;
+    cell_coef<UNI>(a, min(f0, hi - P), cf);
+#pragma unroll
+    for (int v = 0; v < S; ++v) {
+        double st[W];
+#pragma unroll
+        for (int r = 0; r < W; ++r) st[r] = w[r][v];
+        weno_cell<WO, double, false, true>(st, cf, a.eps, dummy, ql[v]);
+    }
+    // row f0: own velocity and the x neighbours'
+    double uo[2], uw[2], ue[2];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        uo[d] = w[P][Y::MOM + d];
+        uw[d] = cell_vel<NF, D>(a, x - 1, min(f0, hi), d);
+        ue[d] = cell_vel<NF, D>(a, x + 1, min(f0, hi), d);
+    }
+#pragma unroll 1
+    for (int k = 0; k < L; ++k) {
+        const int f = f0 + k;
+        // window to cells f+1-P .. f+1+P
+#pragma unroll
+        for (int r = 0; r + 1 < W; ++r)
+#pragma unroll
+            for (int v = 0; v < S; ++v) w[r][v] = w[r + 1][v];
+        prim(f + 1 + P, w[W - 1]);
+        cell_coef<UNI>(a, min(f + 1, hi - P), cf);
+        double qr[S], qn[S];
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            double st[W];
+#pragma unroll
+            for (int r = 0; r < W; ++r) st[r] = w[r][v];
+            weno_cell<WO, double, true, true>(st, cf, a.eps, qr[v], qn[v]);
+        }
+        double no[2], nw[2], ne[2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            no[d] = w[P][Y::MOM + d];
+            nw[d] = cell_vel<NF, D>(a, x - 1, min(f + 1, hi), d);
+            ne[d] = cell_vel<NF, D>(a, x + 1, min(f + 1, hi), d);
+        }
+        if (f <= a.n) {
+            double l2[S], r2[S];
+#pragma unroll
+            for (int v = 0; v < S; ++v) {
+                l2[v] = (f == -1 && a.bc_beg == -4) ? qr[v] : ql[v];
+                r2[v] = (f == a.n && a.bc_end == -4) ? ql[v] : qr[v];
+            }
+            const double dyf = a.ccy[f + 1 + a.b] - a.ccy[f + a.b];
+            double gxn[2], gt[2];
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                gxn[d] = (no[d] - uo[d]) / dyf;
+                // four-point average of the one-sided x differences of rows
+                // f+1 and f (dqR_prim_dx_n(2) at f, m_viscous.fpp:285-315)
+                const double sum = (no[d] - nw[d]) / dxm + (ne[d] - no[d]) / dxp + (uo[d] - uw[d]) / dxm +
+                                   (ue[d] - uo[d]) / dxp;
+                gt[d] = 25e-2 * sum;
+            }
+            double fs[D + 1];
+            face_flux<NF, D, 1>(a, l2, r2, gxn, gt, fs);
+            const long t = (long)(f + 1) * (a.m + 1) + x;
+#pragma unroll
+            for (int v = 0; v <= D; ++v) a.fs[(long)v * a.nface + t] = fs[v];
+        }
+#pragma unroll
+        for (int v = 0; v < S; ++v) ql[v] = qn[v];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            uo[d] = no[d];
+            uw[d] = nw[d];
+            ue[d] = ne[d];
+        }
+    }
 }
 
 // rhs(mom..E) += (1/ds)(fs(c-1) - fs(c)) over the interior
@@ -271,18 +387,39 @@ __global__ __launch_bounds__(256) void k_rk_stage(SweepArgs a, long nint) {
     for (int v = 0; v < S; ++v) a.qout.v[v][go] = o[v];
 }
 
+constexpr int kYChunk = 8; // y faces per thread of k_visc_face_y
+
+template <int NF, int D, int DIR, int WO, bool UNI>
+void launch_face_u(const ViscArgs &a, hipStream_t st) {
+    if constexpr (DIR == 0) {
+        const int ntile = (a.m + 2 + 62) / 63;
+        hipLaunchKernelGGL((k_visc_face_x<NF, D, WO, UNI>), dim3((unsigned)((long)ntile * (a.n + 1))), dim3(64), 0,
+                           st, a, ntile);
+    } else {
+        const int nxb = (a.m + 1 + 255) / 256;
+        const int nchunk = (a.n + 2 + kYChunk - 1) / kYChunk;
+        hipLaunchKernelGGL((k_visc_face_y<NF, WO, kYChunk, UNI>), dim3((unsigned)((long)nxb * nchunk)), dim3(256), 0,
+                           st, a, nxb);
+    }
+}
+
+template <int NF, int D, int DIR, int WO>
+void launch_face(const ViscArgs &a, hipStream_t st) {
+    if (a.uniform) launch_face_u<NF, D, DIR, WO, true>(a, st);
+    else launch_face_u<NF, D, DIR, WO, false>(a, st);
+}
+
 template <int NF, int D, int DIR>
 int launch_dir(int weno_order, const ViscArgs &a, hipStream_t st) {
-    const dim3 blk(256), gf((unsigned)((a.nface + 255) / 256));
     switch (weno_order) {
-    case 1: hipLaunchKernelGGL((k_visc_face<NF, D, DIR, 1>), gf, blk, 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_visc_face<NF, D, DIR, 3>), gf, blk, 0, st, a); break;
-    case 5: hipLaunchKernelGGL((k_visc_face<NF, D, DIR, 5>), gf, blk, 0, st, a); break;
+    case 1: launch_face<NF, D, DIR, 1>(a, st); break;
+    case 3: launch_face<NF, D, DIR, 3>(a, st); break;
+    case 5: launch_face<NF, D, DIR, 5>(a, st); break;
     default: return -1;
     }
     if (hipGetLastError() != hipSuccess) return -1;
     const long nint = (long)(a.m + 1) * (a.n + 1);
-    hipLaunchKernelGGL((k_visc_add<NF, D, DIR>), dim3((unsigned)((nint + 255) / 256)), blk, 0, st, a, nint);
+    hipLaunchKernelGGL((k_visc_add<NF, D, DIR>), dim3((unsigned)((nint + 255) / 256)), dim3(256), 0, st, a, nint);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -25,6 +25,8 @@ struct ViscArgs {
     long nface;
     const double *coef;   // WENO coefficients of the sweep direction, 27 planes of ncell
     long ncell;
+    int uniform;          // every reconstructed cell has the coefficients ucf (bit for bit)
+    double ucf[kNCoef];
     const double *ccx, *ccy; // ghosted cell centres x_cc(-b:m+b), y_cc(-b:n+b)
     const double *inv_ds; // 1/ds(i), i = 0..s, of the sweep direction
     Eos eos;
