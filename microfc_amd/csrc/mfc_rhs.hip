@@ -437,10 +437,12 @@ __global__ __launch_bounds__(256) void k_icfl(Fields q, Eos eos, Box x, const do
     }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_max(double *partials, int n) {
+// Final pass over the block partials: one block of 1024 threads (eight
+// partials per thread at 8192 blocks).
+__global__ __launch_bounds__(1024) void k_reduce_max(double *partials, int n) {
     double mx = -INFINITY;
     for (int t = threadIdx.x; t < n; t += blockDim.x) mx = max_nan(mx, partials[t]);
-    __shared__ double red[4];
+    __shared__ double red[16];
     mx = wave_max(mx);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
     __syncthreads();
@@ -870,7 +872,7 @@ static int icfl_reduce(double *const *q, bool from_cons, double dt, double *out)
     });
     }
     if (rc) return rc;
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, G.stream, G.partials, nb);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, G.stream, G.partials, nb);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, G.partials, sizeof(double), hipMemcpyDeviceToHost, G.stream));
     HIPCHK(hipStreamSynchronize(G.stream));
