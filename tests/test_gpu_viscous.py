@@ -64,6 +64,16 @@ def _cases():
     prm, g, q = multi_fluid_2d(3, N=32)
     prm.Re = [(0.3, -1.0), (-1.0, 0.7), (0.9, 1.1)]
     out.append(("visc_fluids3_2d", prm, g, q))
+    prm, g, q = multi_fluid_2d(4, N=32)
+    prm.Re = [(0.3, 0.5), (0.4, -1.0), (-1.0, 0.8), (0.9, 1.1)]
+    out.append(("visc_fluids4_2d", prm, g, q))
+    prm, g, q = ic.shock_bubble_2d(N=36, weno_order=1)
+    prm.time_stepper = 1
+    prm.bc_beg = (-4, -1, -3)
+    prm.bc_end = (-2, -1, -3)
+    prm.Re = [(0.05, 0.1), (0.1, 0.2)]
+    g = ic.uniform_grids(prm)
+    out.append(("visc_bubble2d_w1_rk1_extrap_periodic", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
     prm, g, q = ic.shear_layer_2d(N=48)
     prm.Re = [(200.0, -1.0)]
     out.append(("visc_shear2d_48", prm, g, perturbed(prm, q)))
