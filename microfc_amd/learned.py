@@ -240,10 +240,13 @@ def train(mlp: Mlp, data, epochs: int = 5, batch: int = 65536, lr: float = 1e-3,
     return losses
 
 
-def run(prm: Params, grids, q, t_end: float, cfl: float = 0.4, dt: float | None = None):
+def run(prm: Params, grids, q, t_end: float, cfl: float = 0.4, dt: float | None = None, progress=None,
+        progress_every: int = 100):
     """Runs a case to t_end with TVD-RK3 and a fixed dt (from the CFL
     reduction at t = 0 unless given).  Returns the interior conservative
-    field (numpy), the number of steps and the GPU seconds of the steps."""
+    field (numpy), the number of steps and the GPU seconds of the steps.
+    progress(t), when given, is called after every progress_every steps
+    have completed on the GPU (for long runs)."""
     import math
 
     import torch
@@ -262,6 +265,9 @@ def run(prm: Params, grids, q, t_end: float, cfl: float = 0.4, dt: float | None 
         e0.record(stream)
         for t in range(nsteps):
             mod.s_tvd_rk(t, (q1, q2), None, rhs, dt)
+            if progress is not None and (t + 1) % progress_every == 0:
+                mod.sync()
+                progress(t + 1)
         e1.record(stream)
         mod.sync()
         e1.synchronize()
