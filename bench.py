@@ -36,15 +36,23 @@ def sweep_bytes_per_step(prm):
     """Algorithmic HBM bytes per RK3 step of each kernel, per cell: the
     stencil input is read once (S doubles), rhs is written by the first
     sweep, read and written by middle ones, read by the last, which reads the
-    RK base (stages 2 and 3) and writes the stage output.  With viscous terms
-    every sweep leaves rhs in HBM; per direction the viscous face kernel
-    reads the stencil input and writes D+1 face fluxes, the divergence kernel
-    reads them and updates D+1 rhs variables; the RK stage kernel reads base,
-    stage input and rhs and writes the stage output."""
-    S, D = prm.sys_size, prm.num_dims
-    fused = not prm.viscous
+    RK base (stages 2 and 3) and writes the stage output.  Viscous runs use
+    the viscous kernels in place of the sweeps: per direction the face kernel
+    reads the stencil input and writes a face record (HLLC flux, contact
+    velocity, viscous flux: S+2+D doubles), the finish reads it and the
+    volume fractions, and reads/writes rhs like a sweep (or, in the last
+    direction, reads the RK base and stage input and writes the output)."""
+    S, D, NF = prm.sys_size, prm.num_dims, prm.num_fluids
     out = {}
     names = ["sweep_x", "sweep_y", "sweep_z"][:D]
+    if prm.viscous:
+        total = 0
+        for d in range(D):
+            first, last = d == 0, d == D - 1
+            per = S + 2 * (S + 2 + D) + NF + (0 if first else S) + (3 * S if last else S)
+            total += 3 * 8 * per
+        out["viscous"] = total
+        return out
     for d, nm in enumerate(names):
         first, last = d == 0, d == D - 1
         per_stage = []
@@ -53,7 +61,7 @@ def sweep_bytes_per_step(prm):
             w = 0
             if not first:
                 r += S  # rhs partial
-            if last and fused:
+            if last:
                 if stage > 0:
                     r += S  # RK base q1
                 w += S  # stage output
@@ -61,9 +69,6 @@ def sweep_bytes_per_step(prm):
                 w += S  # rhs
             per_stage.append(8 * (r + w))
         out[nm] = sum(per_stage)
-    if not fused:
-        out["viscous"] = 3 * D * 8 * (S + 4 * (D + 1))
-        out["rk_stage"] = 3 * 8 * 4 * S
     return out
 
 

@@ -166,8 +166,8 @@ const char *mfc_mlp_last_error(void);
 
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * every launch.  Kernel ids: 0 bc_fill, 1 cons_to_prim, 2 sweep_x, 3 sweep_y,
- * 4 sweep_z, 5 icfl, 6 rk1_update, 7 viscous (face flux and divergence),
- * 8 rk_stage (the stage update of viscous runs).  The reference's only timing is the mean
+ * 4 sweep_z, 5 icfl, 6 rk1_update, 7 viscous (one direction of a viscous
+ * run: face fluxes and finish).  The reference's only timing is the mean
  * CPU time per step (src/simulation/m_time_steppers.fpp:281,352-358). */
 int mfc_timing_enable(int on);
 int mfc_timing_reset(void);

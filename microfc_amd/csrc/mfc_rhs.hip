@@ -75,16 +75,15 @@ struct State {
     double *scratch_state = nullptr; // S*ng doubles, RK3 stage buffer
     bool visc = false;               // any fluid_pp%Re > 0
     ViscParams vp{};
-    double *visc_fs = nullptr;       // (D+1) x faces of one direction, viscous source flux
+    double *visc_fs = nullptr;       // (S+2+D) x faces of one direction: HLLC flux, contact velocity, viscous flux
     long visc_nface = 0;
 } G;
 
 // ---------------------------------------------------------------------------
 // Optional per-kernel timing with HIP events on the library's stream.
-enum KernelId { kKBcFill = 0, kKConsToPrim, kKSweepX, kKSweepY, kKSweepZ, kKIcfl, kKRk1, kKViscous, kKRkStage,
-                kKCount };
+enum KernelId { kKBcFill = 0, kKConsToPrim, kKSweepX, kKSweepY, kKSweepZ, kKIcfl, kKRk1, kKViscous, kKCount };
 const char *const kKernelNames[kKCount] = {"bc_fill", "cons_to_prim", "sweep_x", "sweep_y", "sweep_z", "icfl",
-                                           "rk1_update", "viscous", "rk_stage"};
+                                           "rk1_update", "viscous"};
 
 struct Timing {
     bool on = false;
@@ -621,9 +620,10 @@ int sweep(int d, SweepArgs &a) {
     return 0;
 }
 
-// Viscous source flux of direction d and its divergence, added to rhs after
-// the direction's inviscid terms (m_rhs.fpp:591-603, :639-651).
-int viscous(int d, const Fields &qin, const Fields &rhs) {
+// Direction d of a viscous RHS evaluation (inviscid and viscous fluxes of
+// every face, then the finish: m_rhs.fpp:565-652), with sa the direction's
+// finish arguments (rhs assignment or accumulation, or the RK stage).
+int viscous(int d, const Fields &qin, const Fields &rhs, const SweepArgs &sa) {
     ViscArgs v;
     memset(&v, 0, sizeof v);
     v.q = qin;
@@ -649,15 +649,15 @@ int viscous(int d, const Fields &qin, const Fields &rhs) {
     v.bc_end = bcs[d][1];
     v.vp = G.vp;
     TimedScope ts(kKViscous);
-    if (launch_viscous(G.prm.num_fluids, G.D, d, G.prm.weno_order, v, G.stream))
+    if (launch_viscous(G.prm.num_fluids, G.D, d, G.prm.weno_order, v, sa, G.stream))
         return fail(-3, "viscous launch failed (dir %d): %s", d, hipGetErrorString(hipGetLastError()));
     return 0;
 }
 
 // All directional sweeps of one RHS evaluation on the ghost-filled field qin.
-// The last sweep either writes rhs (rk = kRkNone) or applies the RK stage;
-// with viscous terms every sweep writes rhs, the viscous kernels add to it,
-// and a pointwise kernel applies the RK stage.
+// The last sweep either writes rhs (rk = kRkNone) or applies the RK stage.
+// Viscous runs use the viscous kernels for each direction instead, with the
+// same finish semantics.
 int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, const Fields *qout, double dt) {
     for (int d = 0; d < G.D; ++d) {
         SweepArgs a = sweep_args(d);
@@ -666,27 +666,13 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         a.accumulate = d > 0;
         a.dt = dt;
         const bool last = d == G.D - 1;
-        a.rk = last && !G.visc ? rk : kRkNone;
+        a.rk = last ? rk : kRkNone;
         if (a.rk != kRkNone) {
             a.qbase = *qbase;
             a.qout = *qout;
             a.base_is_in = qbase->v[0] == qin.v[0];
         }
-        if (int rc = sweep(d, a)) return rc;
-        if (G.visc)
-            if (int rc = viscous(d, qin, rhs)) return rc;
-    }
-    if (G.visc && rk != kRkNone) {
-        SweepArgs a = sweep_args(0);
-        a.qin = qin;
-        a.rhs = rhs;
-        a.qbase = *qbase;
-        a.qout = *qout;
-        a.rk = rk;
-        a.dt = dt;
-        TimedScope ts(kKRkStage);
-        if (launch_rk_stage(G.prm.num_fluids, G.D, a, G.nint, G.stream))
-            return fail(-3, "RK stage launch failed: %s", hipGetErrorString(hipGetLastError()));
+        if (int rc = G.visc ? viscous(d, qin, rhs, a) : sweep(d, a)) return rc;
     }
     return 0;
 }
@@ -731,7 +717,7 @@ int mfc_rhs_init(const mfc_params *prm) {
     G.visc = G.vp.size[0] > 0 || G.vp.size[1] > 0;
     if (G.visc) {
         G.visc_nface = std::max((long)(prm->m + 2) * (prm->n + 1), (long)(prm->m + 1) * (prm->n + 2));
-        HIPCHK(hipMalloc(&G.visc_fs, sizeof(double) * (G.D + 1) * G.visc_nface));
+        HIPCHK(hipMalloc(&G.visc_fs, sizeof(double) * (G.S + 2 + G.D) * G.visc_nface));
     }
     G.init = true;
     return 0;

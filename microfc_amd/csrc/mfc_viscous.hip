@@ -1,6 +1,5 @@
 // mfc_viscous.hip -- viscous terms of the RHS with finite-difference velocity
-// gradients, and the pointwise TVD-RK stage the viscous path applies after
-// them.
+// gradients: the whole RHS and RK stage of viscous runs.
 //
 // The inviscid sweeps keep face states in registers and LDS only, so the
 // viscous flux, which needs the HLLC star velocity and the reconstructed
@@ -9,10 +8,13 @@
 // the same bits): each cell is converted to primitive and reconstructed once
 // (both faces), in LDS and lane shuffles along x and in a register window
 // marching along y; the velocity gradients come from the primitive values
-// and the conservative field; the kernel writes the momentum and energy
-// source flux of every face (m_riemann_solvers.fpp:683-902).  A second
-// kernel adds its divergence to rhs after the direction's inviscid terms, in
-// the reference's order (m_rhs.fpp:591-603, :639-651).
+// and the conservative field.  The face kernels store, per face, the HLLC
+// flux, the contact velocity and the viscous source flux
+// (m_riemann_solvers.fpp:683-902); a finish kernel per direction applies the
+// flux divergence, the alpha*div(u) source and the viscous divergence in the
+// reference's order (m_rhs.fpp:565-652), and in the last direction the RK
+// stage.  Viscous runs therefore use these kernels in place of the inviscid
+// sweeps.
 #include "mfc_viscous.hpp"
 
 namespace mfc {
@@ -73,8 +75,8 @@ __device__ __forceinline__ void cell_coef(const ViscArgs &a, int c, double *cf) 
 // differences (2D).
 template <int NF, int D, int DIR>
 __device__ __forceinline__ void face_flux(const ViscArgs &a, const double *ql, const double *qr, const double *gxn,
-                                          const double *gt, double *fs) {
-    double flux[Sys<NF, D>::S], ustar, vsrc[D];
+                                          const double *gt, double *flux, double &ustar, double *fs) {
+    double vsrc[D];
     hllc_face<NF, D, DIR, double, true>(ql, qr, a.eos, flux, ustar, vsrc);
     double ra[2];
     re_avg<NF, D>(a, ql, qr, ra);
@@ -140,6 +142,19 @@ __device__ __forceinline__ void face_flux(const ViscArgs &a, const double *ql, c
             fs[D] = fs[D] - vsrc[1] * t22;
         }
     }
+}
+
+// The face record: the HLLC flux (S), the contact velocity and the viscous
+// source flux (D+1), plane p of face t at fs[p * nface + t].
+template <int NF, int D>
+__device__ __forceinline__ void store_face(const ViscArgs &a, long t, const double *flux, double ustar,
+                                           const double *fs) {
+    constexpr int S = Sys<NF, D>::S;
+#pragma unroll
+    for (int v = 0; v < S; ++v) a.fs[(long)v * a.nface + t] = flux[v];
+    a.fs[(long)S * a.nface + t] = ustar;
+#pragma unroll
+    for (int v = 0; v <= D; ++v) a.fs[(long)(S + 1 + v) * a.nface + t] = fs[v];
 }
 
 // x faces: one wave per 63 faces f0 .. f0+62 of row y (f0 = -1 + 63*tile).
@@ -225,11 +240,9 @@ __global__ __launch_bounds__(64) void k_visc_face_x(ViscArgs a, int ntile) {
     const double dxf = a.ccx[f + 1 + a.b] - a.ccx[f + a.b];
 #pragma unroll
     for (int d = 0; d < D; ++d) gxn[d] = (sp[Y::MOM + d][lane + P + 1] - sp[Y::MOM + d][lane + P]) / dxf;
-    double fs[D + 1];
-    face_flux<NF, D, 0>(a, ql, qr, gxn, gt, fs);
-    const long t = (long)y * (a.m + 2) + (f + 1);
-#pragma unroll
-    for (int v = 0; v <= D; ++v) a.fs[(long)v * a.nface + t] = fs[v];
+    double flux[S], ustar, fs[D + 1];
+    face_flux<NF, D, 0>(a, ql, qr, gxn, gt, flux, ustar, fs);
+    store_face<NF, D>(a, (long)y * (a.m + 2) + (f + 1), flux, ustar, fs);
 }
 
 // y faces (2D): thread = x column, marching along y over the L faces
@@ -322,11 +335,9 @@ __global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
                                    (ue[d] - uo[d]) / dxp;
                 gt[d] = 25e-2 * sum;
             }
-            double fs[D + 1];
-            face_flux<NF, D, 1>(a, l2, r2, gxn, gt, fs);
-            const long t = (long)(f + 1) * (a.m + 1) + x;
-#pragma unroll
-            for (int v = 0; v <= D; ++v) a.fs[(long)v * a.nface + t] = fs[v];
+            double flux[S], ustar, fs[D + 1];
+            face_flux<NF, D, 1>(a, l2, r2, gxn, gt, flux, ustar, fs);
+            store_face<NF, D>(a, (long)(f + 1) * (a.m + 1) + x, flux, ustar, fs);
         }
 #pragma unroll
         for (int v = 0; v < S; ++v) ql[v] = qn[v];
@@ -339,11 +350,15 @@ __global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
     }
 }
 
-// rhs(mom..E) += (1/ds)(fs(c-1) - fs(c)) over the interior
-// (m_rhs.fpp:591-603 for x, :639-651 for y).
+// Finish of one direction for every interior cell, in the reference's
+// order (m_rhs.fpp:565-652): the flux divergence (assigned in the first
+// direction, added after), the alpha*div(u) source, the viscous divergence
+// of momentum and energy; then the rhs store, or the RK stage in the last
+// direction (the sweeps' finish_cell and rk_combine arithmetic).
 template <int NF, int D, int DIR>
-__global__ __launch_bounds__(256) void k_visc_add(ViscArgs a, long nint) {
+__global__ __launch_bounds__(256) void k_visc_finish(ViscArgs a, SweepArgs s, long nint) {
     using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nint) return;
     const int i = (int)(t % (a.m + 1));
@@ -359,32 +374,34 @@ __global__ __launch_bounds__(256) void k_visc_add(ViscArgs a, long nint) {
         fr = fl + (a.m + 1);
         inv = a.inv_ds[j];
     }
-#pragma unroll
-    for (int v = 0; v <= D; ++v) {
-        double *r = a.rhs.v[Y::MOM + v];
-        r[t] = r[t] + inv * (a.fs[(long)v * a.nface + fl] - a.fs[(long)v * a.nface + fr]);
-    }
-}
-
-template <int NF, int D>
-__global__ __launch_bounds__(256) void k_rk_stage(SweepArgs a, long nint) {
-    constexpr int S = Sys<NF, D>::S;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nint) return;
-    const int i = (int)(t % (a.m + 1));
-    const int j = (int)(t / (a.m + 1));
-    const long go = ghost_off(a, i, j, 0);
-    FinishIn<S> f;
-    double r[S], o[S];
+    const long go = ghost_off(s, i, j, 0);
+    auto F = [&](int p, long f) { return a.fs[(long)p * a.nface + f]; };
+    double r[S];
 #pragma unroll
     for (int v = 0; v < S; ++v) {
-        f.base[v] = a.qbase.v[v][go];
-        f.in[v] = a.qin.v[v][go];
-        r[v] = a.rhs.v[v][t];
+        const double df = F(v, fl) - F(v, fr);
+        r[v] = s.accumulate ? s.rhs.v[v][t] + inv * df : inv * df;
     }
-    rk_combine<S>(a, f, r, o);
+    const double dus = F(S, fr) - F(S, fl);
 #pragma unroll
-    for (int v = 0; v < S; ++v) a.qout.v[v][go] = o[v];
+    for (int k = 0; k < NF; ++k) r[Y::ADV + k] = r[Y::ADV + k] + inv * s.qin.v[Y::ADV + k][go] * dus;
+#pragma unroll
+    for (int v = 0; v <= D; ++v) r[Y::MOM + v] = r[Y::MOM + v] + inv * (F(S + 1 + v, fl) - F(S + 1 + v, fr));
+    if (s.rk == kRkNone) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) s.rhs.v[v][t] = r[v];
+        return;
+    }
+    FinishIn<S> fin;
+    double o[S];
+#pragma unroll
+    for (int v = 0; v < S; ++v) {
+        fin.base[v] = s.qbase.v[v][go];
+        fin.in[v] = s.qin.v[v][go];
+    }
+    rk_combine<S>(s, fin, r, o);
+#pragma unroll
+    for (int v = 0; v < S; ++v) s.qout.v[v][go] = o[v];
 }
 
 constexpr int kYChunk = 8; // y faces per thread of k_visc_face_y
@@ -410,7 +427,7 @@ void launch_face(const ViscArgs &a, hipStream_t st) {
 }
 
 template <int NF, int D, int DIR>
-int launch_dir(int weno_order, const ViscArgs &a, hipStream_t st) {
+int launch_dir(int weno_order, const ViscArgs &a, const SweepArgs &sa, hipStream_t st) {
     switch (weno_order) {
     case 1: launch_face<NF, D, DIR, 1>(a, st); break;
     case 3: launch_face<NF, D, DIR, 3>(a, st); break;
@@ -419,45 +436,28 @@ int launch_dir(int weno_order, const ViscArgs &a, hipStream_t st) {
     }
     if (hipGetLastError() != hipSuccess) return -1;
     const long nint = (long)(a.m + 1) * (a.n + 1);
-    hipLaunchKernelGGL((k_visc_add<NF, D, DIR>), dim3((unsigned)((nint + 255) / 256)), dim3(256), 0, st, a, nint);
+    hipLaunchKernelGGL((k_visc_finish<NF, D, DIR>), dim3((unsigned)((nint + 255) / 256)), dim3(256), 0, st, a, sa,
+                       nint);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int NF>
-int launch_nf(int D, int dir, int weno_order, const ViscArgs &a, hipStream_t st) {
-    if (D == 1 && dir == 0) return launch_dir<NF, 1, 0>(weno_order, a, st);
-    if (D == 2 && dir == 0) return launch_dir<NF, 2, 0>(weno_order, a, st);
-    if (D == 2 && dir == 1) return launch_dir<NF, 2, 1>(weno_order, a, st);
+int launch_nf(int D, int dir, int weno_order, const ViscArgs &a, const SweepArgs &sa, hipStream_t st) {
+    if (D == 1 && dir == 0) return launch_dir<NF, 1, 0>(weno_order, a, sa, st);
+    if (D == 2 && dir == 0) return launch_dir<NF, 2, 0>(weno_order, a, sa, st);
+    if (D == 2 && dir == 1) return launch_dir<NF, 2, 1>(weno_order, a, sa, st);
     return -1;
-}
-
-template <int NF>
-int rk_nf(int D, const SweepArgs &a, long nint, hipStream_t st) {
-    const dim3 blk(256), g((unsigned)((nint + 255) / 256));
-    if (D == 1) hipLaunchKernelGGL((k_rk_stage<NF, 1>), g, blk, 0, st, a, nint);
-    else if (D == 2) hipLaunchKernelGGL((k_rk_stage<NF, 2>), g, blk, 0, st, a, nint);
-    else return -1;
-    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 } // namespace
 
-int launch_viscous(int nf, int D, int dir, int weno_order, const ViscArgs &a, hipStream_t st) {
+int launch_viscous(int nf, int D, int dir, int weno_order, const ViscArgs &a, const SweepArgs &sa,
+                   hipStream_t st) {
     switch (nf) {
-    case 1: return launch_nf<1>(D, dir, weno_order, a, st);
-    case 2: return launch_nf<2>(D, dir, weno_order, a, st);
-    case 3: return launch_nf<3>(D, dir, weno_order, a, st);
-    case 4: return launch_nf<4>(D, dir, weno_order, a, st);
-    default: return -1;
-    }
-}
-
-int launch_rk_stage(int nf, int D, const SweepArgs &a, long nint, hipStream_t st) {
-    switch (nf) {
-    case 1: return rk_nf<1>(D, a, nint, st);
-    case 2: return rk_nf<2>(D, a, nint, st);
-    case 3: return rk_nf<3>(D, a, nint, st);
-    case 4: return rk_nf<4>(D, a, nint, st);
+    case 1: return launch_nf<1>(D, dir, weno_order, a, sa, st);
+    case 2: return launch_nf<2>(D, dir, weno_order, a, sa, st);
+    case 3: return launch_nf<3>(D, dir, weno_order, a, sa, st);
+    case 4: return launch_nf<4>(D, dir, weno_order, a, sa, st);
     default: return -1;
     }
 }

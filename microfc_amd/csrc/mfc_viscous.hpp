@@ -21,7 +21,7 @@ struct ViscParams {
 struct ViscArgs {
     Fields q;             // ghost-filled conservative stage input
     Fields rhs;           // interior rhs the divergence is added to
-    double *fs;           // face fluxes, (D+1) planes of nface doubles
+    double *fs;           // face records, S+2+D planes of nface doubles (HLLC flux, contact velocity, viscous flux)
     long nface;
     const double *coef;   // WENO coefficients of the sweep direction, 27 planes of ncell
     long ncell;
@@ -37,12 +37,10 @@ struct ViscArgs {
     ViscParams vp;
 };
 
-// Face fluxes (k_visc_face) then their divergence (k_visc_add) for direction
-// dir, after that direction's inviscid sweep has written or added to rhs.
-int launch_viscous(int nf, int D, int dir, int weno_order, const ViscArgs &a, hipStream_t stream);
-
-// Pointwise TVD-RK stage a.rk from a.qbase, a.qin and a.rhs into a.qout over
-// the interior (the stage update the inviscid path fuses into its last sweep).
-int launch_rk_stage(int nf, int D, const SweepArgs &a, long nint, hipStream_t stream);
+// Direction dir of a viscous RHS evaluation: the face records, then the
+// finish, which assigns (sa.accumulate = 0) or adds to sa.rhs, or applies
+// the RK stage sa.rk from sa.qbase and sa.qin into sa.qout.
+int launch_viscous(int nf, int D, int dir, int weno_order, const ViscArgs &a, const SweepArgs &sa,
+                   hipStream_t stream);
 
 } // namespace mfc
