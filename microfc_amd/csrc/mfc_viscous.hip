@@ -1,20 +1,19 @@
-// mfc_viscous.hip -- viscous terms of the RHS with finite-difference velocity
-// gradients: the whole RHS and RK stage of viscous runs.
+// mfc_viscous.hip -- the RHS and RK stage of viscous runs: inviscid and
+// viscous fluxes with finite-difference velocity gradients.
 //
-// The inviscid sweeps keep face states in registers and LDS only, so the
-// viscous flux, which needs the HLLC star velocity and the reconstructed
-// volume fractions of every face (m_riemann_solvers.fpp:169-231, :313-323),
-// recomputes them here with the sweeps' own weno_cell and hllc_face (hence
-// the same bits): each cell is converted to primitive and reconstructed once
-// (both faces), in LDS and lane shuffles along x and in a register window
-// marching along y; the velocity gradients come from the primitive values
-// and the conservative field.  The face kernels store, per face, the HLLC
-// flux, the contact velocity and the viscous source flux
-// (m_riemann_solvers.fpp:683-902); a finish kernel per direction applies the
-// flux divergence, the alpha*div(u) source and the viscous divergence in the
-// reference's order (m_rhs.fpp:565-652), and in the last direction the RK
-// stage.  Viscous runs therefore use these kernels in place of the inviscid
-// sweeps.
+// The viscous flux needs, per face, the HLLC star velocity and the
+// reconstructed volume fractions (m_riemann_solvers.fpp:169-231, :313-323),
+// which the inviscid sweeps keep in registers and LDS only.  Viscous runs
+// therefore use these kernels in place of the sweeps.  The face kernels
+// reconstruct with the sweeps' own weno_cell and hllc_face (hence the same
+// bits), each cell converted to primitive and reconstructed once (both
+// faces): in LDS and lane shuffles along x, in a register window marching
+// along y.  The velocity gradients come from the primitive values and the
+// conservative field.  Per face they store the HLLC flux, the contact
+// velocity and the viscous source flux (m_riemann_solvers.fpp:683-902); a
+// finish kernel per direction applies the flux divergence, the
+// alpha*div(u) source and the viscous divergence in the reference's order
+// (m_rhs.fpp:565-652), and in the last direction the RK stage.
 #include "mfc_viscous.hpp"
 
 namespace mfc {
