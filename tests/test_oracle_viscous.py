@@ -183,3 +183,27 @@ def test_inviscid_entries_leave_rhs_unchanged():
     q, _, _ = _state(case, grid, [lambda x, y: x * y, lambda x, y: x - y])
     inv = O.Case(**{**case.__dict__, "Re": None})
     np.testing.assert_array_equal(O.compute_rhs(case, grid, q.copy()), O.compute_rhs(inv, grid, q.copy()))
+
+
+def test_shear_wave_decays_at_the_discrete_viscous_rate():
+    """v = A sin(2 pi x), u = 0 at rho = p = 1, periodic: the HLLC flux of
+    the tangential momentum vanishes (equal pressure and zero normal
+    velocity), so d(rho v)/dt is the viscous term alone, mu times the
+    three-point second difference of v.  After RK3 steps the amplitude
+    follows exp(-lambda t) with lambda = nu (2 - 2 cos(k dx)) / dx^2 (nu = mu
+    here), up to the RK3 error and O(A^2) terms."""
+    N, A, Re, dt, nsteps = 32, 1e-3, 10.0, 1e-3, 200
+    case = _case(2, N, Re=[(Re, -1.0)], bc=-1)
+    case.n = 15
+    grid = O.uniform_grid(case)
+    q, X, _ = _state(case, grid, [lambda x, y: 0.0 * x, lambda x, y: A * np.sin(2 * np.pi * x)])
+    q2 = np.zeros_like(q)
+    for t in range(nsteps):
+        O.tvd_rk_step(case, grid, dt, q, q2, t_step=t)
+    b = case.buff_size
+    v = q[2, 0, b:-b, b:-b] / q[0, 0, b:-b, b:-b]
+    amp = 2.0 / N * np.sum(v * np.sin(2 * np.pi * X), axis=1)
+    dx = 1.0 / N
+    lam = (1.0 / Re) * (2.0 - 2.0 * np.cos(2 * np.pi * dx)) / dx ** 2
+    want = A * np.exp(-lam * dt * nsteps)
+    assert np.max(np.abs(amp - want)) / want < 1e-5, (amp, want)
