@@ -259,7 +259,8 @@ def bubble_cloud_centres(nbubbles: int = 64, seed: int = 0, r_lo: float = 0.02, 
 
 
 def bubble_cloud_3d(N: int = 768, nbubbles: int = 64, seed: int = 0, weno_order: int = 5,
-                    alpha_min: float = 1e-6, x_shock: float = 0.15, precision: int = 0, slab: int = 32):
+                    alpha_min: float = 1e-6, x_shock: float = 0.15, precision: int = 0, slab: int = 32,
+                    lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
     """Cloud of air bubbles (rho 1, p 1e5; radii U(0.02, 0.05), seeded
     non-overlapping centres ahead of the shock) in water (rho 1000, p 1e5)
     hit by a planar water shock at x = x_shock (post-shock p = 1e9, u = 200,
@@ -267,14 +268,16 @@ def bubble_cloud_3d(N: int = 768, nbubbles: int = 64, seed: int = 0, weno_order:
     bubble's smoothed indicator is evaluated within R + 24 dx of its centre
     (beyond it tanh rounds to 1 and the indicator is exactly 0 in FP64); the
     cloud's indicator is the maximum over bubbles.  Written slab by slab in z, so the host holds
-    one ghosted state and a slab of temporaries (768^3: 30 GB + 1.2 GB)."""
+    one ghosted state and a slab of temporaries (768^3: 30 GB + 1.2 GB).
+    lo, hi: a sub-box of the unit cube to grid instead (same cloud and shock;
+    N cells per direction, cubic cells when the box is)."""
     gw, piw = WATER
     ga, pia = AIR
     prm = Params(m=N - 1, n=N - 1, p=N - 1, num_fluids=2, weno_order=weno_order, gammas=[gw, ga],
                  pi_infs=[piw, pia], precision=precision)
-    grids = uniform_grids(prm)
+    grids = uniform_grids(prm, lo=lo, hi=hi)
     xc, yc, zc = cell_centres(prm, grids)
-    dxmin = 1.0 / N
+    dxmin = min(h - l for l, h in zip(lo, hi)) / N
     centres, radii = bubble_cloud_centres(nbubbles, seed)
     rho_w1, p1 = 1000.0, 1.0e5
     rho_w2 = stiffened_hugoniot_density(rho_w1, p1, 1.0e9, 4.4, 6.0e8)

@@ -1,10 +1,20 @@
 """The 3D shock-bubble cloud at its benchmark size: 768^3 cells (453 M),
 64 air bubbles in water, 300 TVD-RK3 steps at CFL 0.3, in the three forms;
 the FP32 bulk and the learned weights compared with plain FP64 on the same
-grid.  The network is trained as in test_gpu_cloud.py (3D pairs from a
-64^3 plain FP64 run of another cloud).
+grid, and all three with a 2x refinement of a 384^3 sub-box.  The network
+is trained as in test_gpu_cloud.py (3D pairs from a 64^3 plain FP64 run of
+another cloud).
 
-A study rather than a check: about ten GPU-minutes and 150 GB of host
+The sub-box, x in [0, 0.5], y in [346/768, +0.5], z in [92/768, +0.5], is
+aligned with the 768^3 cells.  Its refinement is a plain FP64 run of the
+same cloud on the sub-box alone (768^3 cells of 1/1536, 600 steps of dt/2)
+with extrapolation boundaries, restricted 2x2x2.  Within the time window
+the boundaries see only states the shock has not changed, or the planar
+shock itself, except near the y and z faces, where a bubble hit at the end
+of the window is cut; the comparison leaves out a band of 38 coarse cells
+(about 0.05) along those faces.
+
+A study rather than a check: about twelve GPU-minutes and 150 GB of host
 memory, so it runs only with MFC_STUDIES=1 in the environment."""
 import os
 
@@ -32,7 +42,11 @@ def _rel_l1(a, b):
     return np.array(out)
 
 
-def test_cloud_768_forms_against_fp64(gpu):
+Y0, Z0, HALF = 346, 92, 384  # sub-box origin and size in 768^3 cells
+BAND = 38
+
+
+def test_cloud_768_forms_against_fp64_and_refined_subbox(gpu):
     torch = gpu
     fprm, fg, fq = ic.bubble_cloud_3d(N=64, seed=1)
     data = learned.training_pairs(fprm, fg, fq, nsnap=4, steps_between=8, cfl=0.3)
@@ -73,3 +87,23 @@ def test_cloud_768_forms_against_fp64(gpu):
         err = _rel_l1(res[prec], e64)
         say(f"  {name:11s} rel L1 vs FP64 {N}^3 {NAMES}: {np.array2string(err, precision=2)}")
     assert _rel_l1(res[PREC_FP32_BULK], e64).max() <= 1e-4
+    # the sub-box of each coarse run, then the refined sub-box run
+    sub = {k: np.ascontiguousarray(v[:, Z0:Z0 + HALF, Y0:Y0 + HALF, 0:HALF]) for k, v in res.items()}
+    del res, e64, q, out
+    lo = (0.0, Y0 / N, Z0 / N)
+    hi = (HALF / N, (Y0 + HALF) / N, (Z0 + HALF) / N)
+    say(f"[cloud 768] writing the refined sub-box {lo} .. {hi} at {2 * HALF}^3")
+    fprm, fgrids, fq = ic.bubble_cloud_3d(N=2 * HALF, lo=lo, hi=hi)
+    say("  refined sub-box plain FP64: running")
+    fine, fn, fsec = learned.run(fprm, fgrids, fq, t_end, dt=dt / 2, progress=lambda t: say(f"    step {t}"),
+                                 progress_every=100)
+    del fq
+    assert fn == 2 * nsteps and bool(np.all(np.isfinite(fine)))
+    say(f"  refined sub-box {2 * HALF}^3 (dx 1/{2 * N}), {fn} steps, GPU s {fsec:.2f}")
+    judge = learned.restrict_host(fine)
+    del fine
+    inner = np.s_[:, BAND:-BAND, BAND:-BAND, :]
+    ref = judge[inner]
+    for prec, name in ((0, "plain FP64"), (PREC_FP32_BULK, "FP32 bulk"), (PREC_FP32_LEARNED, "learned")):
+        err = _rel_l1(sub[prec][inner], ref)
+        say(f"  {name:11s} rel L1 vs the refined sub-box {NAMES}: {np.array2string(err, precision=3)}")
