@@ -61,3 +61,12 @@ def test_viscous_shear_matches_the_reference_example():
     assert np.isclose(high[4], G * 1e6 + P + 0.5 * 500.0 ** 2)
     dt = ic.viscous_dt(prm, grids, q)
     assert 0 < dt < 0.2 * (1 / 32) ** 2 * 1.0 / (4 / 3 * 1e4) * 1.0001
+
+
+def test_cloud_on_a_sub_box_equals_the_region_of_the_whole_box():
+    """The cloud written on a sub-box at the whole box's spacing is that
+    box's region, bit for bit (the refined sub-box judge of the cloud
+    configuration relies on it)."""
+    prm, g, q = ic.bubble_cloud_3d(N=32)
+    sprm, sg, sq = ic.bubble_cloud_3d(N=16, lo=(0.0, 14 / 32, 4 / 32), hi=(0.5, 30 / 32, 20 / 32))
+    np.testing.assert_array_equal(sprm.interior(sq), prm.interior(q)[:, 4:20, 14:30, 0:16])
