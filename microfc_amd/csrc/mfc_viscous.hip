@@ -248,9 +248,10 @@ __global__ __launch_bounds__(64) void k_visc_face_x(ViscArgs a, int ntile) {
 // f0 .. f0+L-1 of its chunk (f0 = -1 + L*chunk) with the primitive stencil
 // in a register window, so each cell is converted and reconstructed once
 // (both faces); x-neighbour velocities come from the conservative field.
-// The window and three face states hold 256 VGPRs and 72 AGPRs (one wave
-// per SIMD); capped at 2, 3 or 4 waves the kernel spills (0.1-0.9 KB per
-// lane) and is slower.
+// The kernel holds 256 VGPRs and 72 AGPRs (one wave per SIMD), taken by the
+// per-face WENO, HLLC and viscous arithmetic (the window in an LDS ring
+// leaves the count unchanged); capped at 2, 3 or 4 waves it spills (0.1-0.9
+// KB per lane) and is slower.
 template <int NF, int WO, int L, bool UNI>
 __global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
     constexpr int D = 2;
