@@ -40,9 +40,12 @@ class _Case(ctypes.Structure):
 
 
 def lib():
+    """The compiled oracle.  MFC_ORACLE_LIB names another build of it (the
+    mutation checks of tests/test_oracle_mutants.py load deliberately broken
+    copies this way)."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "libmfc_oracle.so")
+        path = os.environ.get("MFC_ORACLE_LIB") or os.path.join(_HERE, "libmfc_oracle.so")
         if not os.path.exists(path):
             import subprocess
             subprocess.run(["make", "-C", _HERE], check=True, capture_output=True)

@@ -121,6 +121,46 @@ def multi_fluid_2d(nf: int, N: int = 40, seed: int = 3):
     return prm, grids, ic.ghosted(prm, ic.prim_to_cons(prm, prim))
 
 
+def stretched(case, amps, phases):
+    """Non-uniform grids s_cb(xi) = f(xi) - f(0), f(xi) = xi + a sin(2 pi xi + phi)/(2 pi),
+    per direction, ghosted by the case's BCs (case: an oracle Case or Params)."""
+    out = []
+    for d, s in enumerate(case.extents()):
+        xi = np.arange(s + 2, dtype=np.float64) / (s + 1)
+        f = lambda t: t + amps[d] * np.sin(2 * np.pi * t + phases[d]) / (2 * np.pi)  # noqa: E731
+        cbi = f(xi) - f(0.0)
+        cb, ds, _ = O.grid_buffers(cbi, case.buff_size, case.bc_beg[d], case.bc_end[d])
+        out.append((cb, ds))
+    return out
+
+
+def smooth_prim_3d(case, seed, w_zero=False):
+    """Two-fluid primitive field with a material interface (a sphere of
+    water in air), a pressure pulse and non-zero velocities in every
+    component (w optionally zero); every term is periodic or negligible at
+    the domain faces, so periodic wraps carry no jump."""
+    rng = np.random.default_rng(seed)
+    m, n, p = case.m, case.n, case.p
+    Z, Y, X = np.meshgrid((np.arange(p + 1) + 0.5) / (p + 1), (np.arange(n + 1) + 0.5) / (n + 1),
+                          (np.arange(m + 1) + 0.5) / (m + 1), indexing="ij")
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2)
+    a1 = 1e-6 + (1 - 2e-6) * 0.5 * (1 + np.tanh((0.2 - r) / 0.1))
+    prim = np.zeros(case.interior_shape)
+    prim[0] = a1 * 1000.0 * (1 + 0.01 * np.sin(2 * np.pi * (X + 2 * Y + 3 * Z)))
+    prim[1] = (1 - a1) * 1.2 * (1 + 0.02 * np.cos(2 * np.pi * (2 * X - Y + Z)))
+    phase = rng.uniform(0, 2 * np.pi, 3)
+    prim[2] = 40.0 + 30.0 * np.sin(2 * np.pi * Y + phase[0])
+    prim[3] = -25.0 + 20.0 * np.cos(2 * np.pi * Z + phase[1])
+    prim[4] = 0.0 if w_zero else 15.0 + 35.0 * np.sin(2 * np.pi * X + phase[2])
+    prim[5] = 1e5 * (1.0 + 3.0 * np.exp(-((X - 0.45) ** 2 + (Y - 0.55) ** 2 + (Z - 0.5) ** 2) / 0.01))
+    prim[6] = a1
+    prim[7] = 1 - a1
+    q = np.zeros(case.ghost_shape)
+    b = case.buff_size
+    q[:, b:b + p + 1, b:b + n + 1, b:b + m + 1] = O.prim_to_cons(case, prim)
+    return q
+
+
 def small_cases():
     """(name, params, grids, q) for the parity matrix; sizes keep the oracle
     to well under a second per RHS."""
@@ -174,6 +214,35 @@ def small_cases():
     for nf in (3, 4):
         prm, g, q = multi_fluid_2d(nf)
         out.append((f"fluids{nf}_2d", prm, g, q))
+    out.extend(walls_3d_cases())
+    return out
+
+
+def field_3d_case(m, n, p, bcb, bce, weno_order=5, grid="uniform", seed=7, time_stepper=3):
+    """Two fluids in 3D with non-zero u, v, w at every boundary (the smooth
+    field of smooth_prim_3d), on a uniform or a stretched grid."""
+    gw, piw = ic.WATER
+    ga, pia = ic.AIR
+    prm = Params(m=m, n=n, p=p, num_fluids=2, weno_order=weno_order, time_stepper=time_stepper,
+                 gammas=[gw, ga], pi_infs=[piw, pia], bc_beg=tuple(bcb), bc_end=tuple(bce))
+    g = ic.uniform_grids(prm) if grid == "uniform" else stretched(prm, (0.5, 0.3, 0.4), (0.2, 1.3, 2.9))
+    return prm, g, smooth_prim_3d(prm, seed)
+
+
+def walls_3d_cases():
+    """3D cases whose reflective y and z walls carry non-zero transverse and
+    normal momentum, and z boundaries that are periodic (-1) and Riemann
+    extrapolation (-4), in the uniform-grid and the per-cell-coefficient
+    kernels."""
+    out = []
+    prm, g, q = field_3d_case(23, 19, 15, (-3, -2, -2), (-4, -2, -2))
+    out.append(("field3d_walls_yz", prm, g, q))
+    prm, g, q = field_3d_case(17, 21, 19, (-4, -3, -1), (-3, -4, -1), grid="stretched")
+    out.append(("field3d_z_periodic_stretched", prm, g, q))
+    prm, g, q = field_3d_case(19, 15, 23, (-1, -2, -4), (-1, -2, -4), weno_order=3, time_stepper=2)
+    out.append(("field3d_w3_z_riemann", prm, g, q))
+    prm, g, q = field_3d_case(16, 17, 18, (-2, -2, -2), (-2, -2, -2), grid="stretched")
+    out.append(("field3d_all_walls_stretched", prm, g, q))
     return out
 
 
