@@ -235,13 +235,13 @@ def walls_3d_cases():
     extrapolation (-4), in the uniform-grid and the per-cell-coefficient
     kernels."""
     out = []
-    prm, g, q = field_3d_case(23, 19, 15, (-3, -2, -2), (-4, -2, -2))
+    prm, g, q = field_3d_case(27, 25, 26, (-3, -2, -2), (-4, -2, -2))
     out.append(("field3d_walls_yz", prm, g, q))
-    prm, g, q = field_3d_case(17, 21, 19, (-4, -3, -1), (-3, -4, -1), grid="stretched")
+    prm, g, q = field_3d_case(25, 29, 27, (-4, -3, -1), (-3, -4, -1), grid="stretched")
     out.append(("field3d_z_periodic_stretched", prm, g, q))
     prm, g, q = field_3d_case(19, 15, 23, (-1, -2, -4), (-1, -2, -4), weno_order=3, time_stepper=2)
     out.append(("field3d_w3_z_riemann", prm, g, q))
-    prm, g, q = field_3d_case(16, 17, 18, (-2, -2, -2), (-2, -2, -2), grid="stretched")
+    prm, g, q = field_3d_case(26, 25, 28, (-2, -2, -2), (-2, -2, -2), grid="stretched")
     out.append(("field3d_all_walls_stretched", prm, g, q))
     return out
 

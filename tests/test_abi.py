@@ -63,3 +63,18 @@ def test_invalid_cases_are_rejected(kw, msg):
     rc, err = _init(**kw)
     assert rc == -1
     assert msg in err
+
+
+def test_parity_matrix_cases_pass_the_input_checks():
+    """Every case of the GPU parity matrix is one the library accepts: the
+    input checks run before any device call, so on a machine without a GPU
+    a valid case fails later (stream creation), never with rc -1."""
+    from helpers import small_cases
+    lib = R.load_library()
+    for name, prm, _, _ in small_cases():
+        c = prm.to_c()
+        rc = lib.mfc_rhs_init(ctypes.byref(c))
+        msg = lib.mfc_last_error().decode()
+        if rc == 0:
+            lib.mfc_rhs_finalize()
+        assert rc != -1, (name, msg)
