@@ -117,6 +117,18 @@ int mfc_icfl_max(double *const *q_prim_vf, double dt, double *icfl_max);
  * dt (its dt is an input); this is the same max-reduction inverted. */
 int mfc_cfl_dt(double *const *q_cons_vf, double cfl, double *dt_out);
 
+/* mfc_cfl_dt without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight time loops, harnesses that forbid hipMalloc).  Same result as
+ * mfc_cfl_dt, bit for bit; the block partials live in a fixed 64 KiB device
+ * array (8192 doubles, one copy per GPU, kept until the process ends) in
+ * place of the buffer mfc_rhs_init allocates.  The reduction is a
+ * grid-stride pass over at most 8192 blocks, so every grid size fits.  Calls
+ * that overlap on one GPU (several host threads, or several streams set with
+ * mfc_set_stream) share that array and race.  Needs mfc_rhs_init and
+ * mfc_rhs_set_grid first, like mfc_cfl_dt; blocks until dt is on the host. */
+int mfc_cfl_dt_noalloc(double *const *q_cons_vf, double cfl, double *dt_out);
+
 /* Sizes implied by the initialised case. */
 int mfc_sizes(int *sys_size, int *buff_size, long *ghost_len, long *interior_len);
 

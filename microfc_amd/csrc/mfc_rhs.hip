@@ -29,10 +29,22 @@ template <> int launch_sweep<4, true>(const SweepLaunch &, const SweepArgs &, hi
 
 using namespace mfc;
 
+constexpr int kReduceBlocks = 8192; // 32 blocks of 256 threads per CU, two cells per thread at 2048^2
+constexpr int kMaxDevices = 64;
+
+// Block partials of the allocation-free CFL reduction (mfc_cfl_dt_noalloc):
+// one copy of this array per GPU, 64 KiB, held until the process ends.  In
+// host code its name is not a device address; kernels get the address that
+// hipGetSymbolAddress returned for the current device (g_scratch_dev).
+static __device__ __align__(256) double scratch[kReduceBlocks];
+
 namespace {
 
+// Device address of `scratch` on each GPU, looked up once per device by
+// mfc_rhs_init with that device current.
+double *g_scratch_dev[kMaxDevices] = {};
+
 constexpr int kNumFluidsBuilt = 4;
-constexpr int kReduceBlocks = 8192; // 32 blocks of 256 threads per CU, two cells per thread at 2048^2
 
 thread_local std::string g_err;
 
@@ -72,6 +84,7 @@ struct State {
     Dir dir[3];
     hipStream_t own = nullptr, stream = nullptr;
     double *partials = nullptr; // kReduceBlocks doubles
+    double *scratch = nullptr;  // this device's `scratch` array (kReduceBlocks doubles)
     double *scratch_state = nullptr; // S*ng doubles, RK3 stage buffer
     bool visc = false;               // any fluid_pp%Re > 0
     ViscParams vp{};
@@ -110,8 +123,8 @@ struct Timing {
 struct TimedScope {
     int id;
     hipEvent_t a = nullptr;
-    explicit TimedScope(int k) : id(k) {
-        if (T.on && (a = T.get())) (void)hipEventRecord(a, G.stream);
+    explicit TimedScope(int k) : id(k) { // k < 0: not timed
+        if (k >= 0 && T.on && (a = T.get())) (void)hipEventRecord(a, G.stream);
     }
     ~TimedScope() {
         if (!a) return;
@@ -706,6 +719,17 @@ int mfc_rhs_init(const mfc_params *prm) {
     HIPCHK(hipStreamCreateWithFlags(&G.own, hipStreamNonBlocking));
     G.stream = G.own;
     HIPCHK(hipMalloc(&G.partials, sizeof(double) * kReduceBlocks));
+    {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kMaxDevices) return fail(-3, "device ordinal %d outside 0..%d", dev, kMaxDevices - 1);
+        if (!g_scratch_dev[dev]) {
+            void *ptr = nullptr;
+            HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            g_scratch_dev[dev] = static_cast<double *>(ptr);
+        }
+        G.scratch = g_scratch_dev[dev];
+    }
     // Re_size, Re_idx, Res (m_global_parameters.fpp:314-338, m_viscous.fpp:36-42)
     for (int r = 0; r < 2; ++r)
         for (int i = 0; i < prm->num_fluids; ++i)
@@ -834,7 +858,12 @@ int mfc_cons_to_prim(double *const *q_cons_vf, double *const *q_prim_vf) {
     return cons_to_prim_launch(qc, qp);
 }
 
-static int icfl_reduce(double *const *q, bool from_cons, double dt, double *out) {
+// The ICFL max-reduction into `partials` (capacity doubles), a grid-stride
+// pass of at most kReduceBlocks blocks, so every grid size fits.  `timed`
+// records HIP events around the pass (event creation may allocate, so the
+// allocation-free entry point passes false).
+static int icfl_reduce(double *const *q, bool from_cons, double dt, double *out, double *partials, int capacity,
+                       bool timed) {
     if (int rc = require_ready()) return rc;
     if (!out) return fail(-1, "NULL output");
     Fields f;
@@ -843,39 +872,51 @@ static int icfl_reduce(double *const *q, bool from_cons, double dt, double *out)
     const Eos e = eos();
     double *dxs[3] = {G.dir[0].ds_dev, G.dir[1].ds_dev, G.dir[2].ds_dev};
     const int nb = (int)std::min<long>(kReduceBlocks, std::max<long>(1, (G.nint + 511) / 512));
+    if (!partials || nb > capacity)
+        return fail(-1, "CFL reduction needs %d partials, the buffer holds %d", nb, capacity);
     int rc;
     {
-    TimedScope ts(kKIcfl); // the per-cell pass (the single-block final reduction is not included)
+    TimedScope ts(timed ? kKIcfl : -1); // the per-cell pass (the single-block final reduction is not included)
     rc = dispatch_nf_d([&](auto NF, auto D) -> int {
         if (from_cons)
             hipLaunchKernelGGL((k_icfl<decltype(NF)::value, decltype(D)::value, true>), dim3(nb), dim3(256), 0,
-                               G.stream, f, e, x, dxs[0], dxs[1], dxs[2], dt, G.nint, G.partials);
+                               G.stream, f, e, x, dxs[0], dxs[1], dxs[2], dt, G.nint, partials);
         else
             hipLaunchKernelGGL((k_icfl<decltype(NF)::value, decltype(D)::value, false>), dim3(nb), dim3(256), 0,
-                               G.stream, f, e, x, dxs[0], dxs[1], dxs[2], dt, G.nint, G.partials);
+                               G.stream, f, e, x, dxs[0], dxs[1], dxs[2], dt, G.nint, partials);
         HIPCHK(hipGetLastError());
         return 0;
     });
     }
     if (rc) return rc;
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, G.stream, G.partials, nb);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, G.stream, partials, nb);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, G.partials, sizeof(double), hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipMemcpyAsync(out, partials, sizeof(double), hipMemcpyDeviceToHost, G.stream));
     HIPCHK(hipStreamSynchronize(G.stream));
     return 0;
 }
 
 int mfc_icfl_max(double *const *q_prim_vf, double dt, double *icfl_max) {
-    return icfl_reduce(q_prim_vf, false, dt, icfl_max);
+    return icfl_reduce(q_prim_vf, false, dt, icfl_max, G.partials, kReduceBlocks, true);
 }
 
-int mfc_cfl_dt(double *const *q_cons_vf, double cfl, double *dt_out) {
+static int cfl_dt(double *const *q_cons_vf, double cfl, double *dt_out, double *partials, bool timed) {
     if (!(cfl > 0.0)) return fail(-1, "cfl must be positive");
+    if (!dt_out) return fail(-1, "NULL output");
     double mx = 0.0;
-    if (int rc = icfl_reduce(q_cons_vf, true, 1.0, &mx)) return rc;
+    if (int rc = icfl_reduce(q_cons_vf, true, 1.0, &mx, partials, kReduceBlocks, timed)) return rc;
     if (!(mx > 0.0) || mx != mx) return fail(-5, "CFL reduction gave %g (NaN or non-positive wave speed)", mx);
     *dt_out = cfl / mx;
     return 0;
+}
+
+int mfc_cfl_dt(double *const *q_cons_vf, double cfl, double *dt_out) {
+    return cfl_dt(q_cons_vf, cfl, dt_out, G.partials, true);
+}
+
+int mfc_cfl_dt_noalloc(double *const *q_cons_vf, double cfl, double *dt_out) {
+    if (!G.init) return fail(-4, "mfc_rhs_init has not been called");
+    return cfl_dt(q_cons_vf, cfl, dt_out, G.scratch, false);
 }
 
 int mfc_sizes(int *sys_size, int *buff_size, long *ghost_len, long *interior_len) {

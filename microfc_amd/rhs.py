@@ -30,7 +30,7 @@ PREC_FP32_LEARNED = 2
 
 EXPORTED = [
     "mfc_rhs_init", "mfc_rhs_set_grid", "mfc_compute_rhs", "mfc_tvd_rk_step",
-    "mfc_cons_to_prim", "mfc_icfl_max", "mfc_cfl_dt", "mfc_sizes",
+    "mfc_cons_to_prim", "mfc_icfl_max", "mfc_cfl_dt", "mfc_cfl_dt_noalloc", "mfc_sizes",
     "mfc_weno_coefficients", "mfc_device_alloc", "mfc_device_free",
     "mfc_memcpy_h2d", "mfc_memcpy_d2h", "mfc_sync", "mfc_stream",
     "mfc_set_stream", "mfc_rhs_finalize", "mfc_last_error",
@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
     L.mfc_cons_to_prim.argtypes = [pp, pp]
     L.mfc_icfl_max.argtypes = [pp, ctypes.c_double, dp]
     L.mfc_cfl_dt.argtypes = [pp, ctypes.c_double, dp]
+    L.mfc_cfl_dt_noalloc.argtypes = [pp, ctypes.c_double, dp]
     L.mfc_sizes.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                             ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
     L.mfc_weno_coefficients.argtypes = [ctypes.c_int, dp]
