@@ -10,7 +10,9 @@
  *   ghosted fields (q_cons_vf, q_prim_vf, q_cons_ts): (m+2b+1) x (n+2b+1) x
  *     (p+2b+1) doubles, x fastest (a collapsed direction has extent 1);
  *   rhs_vf: interior only, (m+1) x (n+1) x (p+1) doubles, x fastest.
- * b = buff_size = (weno_order-1)/2 + 2.  All work is queued on the library's
+ * b = buff_size as the reference sizes it (src/simulation/m_global_parameters.fpp:
+ * 355-359): polyn + 2, or 2*polyn + 2 when any fluid_pp(i)%Re is positive,
+ * with polyn = (weno_order-1)/2; mfc_sizes reports it.  All work is queued on the library's
  * stream (mfc_stream()); calls return before the device has finished unless
  * they return a host value.  Call mfc_sync() before reading results on the host.
  */
@@ -29,6 +31,9 @@ extern "C" {
 #define MFC_PREC_FP64 0      /* plain path: FP64 throughout, reference evaluation order */
 #define MFC_PREC_FP32_BULK 1 /* WENO and HLLC in FP32; cons->prim, flux divergence and RK in FP64 */
 #define MFC_PREC_FP32_LEARNED 2 /* FP32 bulk with learned WENO5 weights (needs mfc_mlp_init) */
+#define MFC_PREC_FP64_FAST 3 /* FP64 throughout with FMA contraction and reciprocal-multiply divisions
+                                (WENO weights normalised by one reciprocal of their sum); within the
+                                reference's 1e-12 relative bar of the plain form, not bit-equal */
 
 /* Learned WENO5 weights: a 5 -> 32 -> 32 -> 3 MLP (ReLU) whose logits z
  * modulate the plain weights, w_k ~ w_plain_k exp(g z_k), with the gate
@@ -109,8 +114,20 @@ int mfc_cons_to_prim(double *const *q_cons_vf, double *const *q_prim_vf);
 
 /* Maximum ICFL over the interior, from a primitive field, as computed by
  * s_write_run_time_information (src/simulation/m_data_output.fpp:197-251).
- * Blocks until the value is on the host.  NaN anywhere gives NaN. */
+ * Blocks until the value is on the host.  A cell whose ICFL is NaN (a NaN
+ * density, pressure or volume fraction) makes the maximum NaN. */
 int mfc_icfl_max(double *const *q_prim_vf, double dt, double *icfl_max);
+
+/* The stability criteria of s_write_run_time_information
+ * (src/simulation/m_data_output.fpp:197-263) over the interior, from a
+ * primitive field: out[0] = maximum ICFL (as mfc_icfl_max), and in viscous
+ * cases out[1] = maximum VCFL and out[2] = minimum cell Reynolds number Rc,
+ * with the mixture Reynolds numbers of
+ * src/common/m_variables_conversion.fpp:210-223 (0 and 0 when inviscid).
+ * Blocks until the values are on the host.  A cell whose criterion is NaN
+ * makes that extremum NaN (min and max drop a NaN operand, as written in the
+ * reference, so e.g. a NaN velocity alone does not reach ICFL). */
+int mfc_stability_criteria(double *const *q_prim_vf, double dt, double *out);
 
 /* CFL time step: dt = cfl / max over cells of the ICFL at dt = 1, evaluated
  * from a conservative field (cons->prim fused).  The reference has no adaptive

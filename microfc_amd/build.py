@@ -7,8 +7,10 @@ for results:
   -ffp-contract=off   every multiply and add rounds separately, as in the
                       reference's statements (FP64 parity is bit-level);
   -ffp-contract=fast  only for the *_f32.hip units (the FP32-bulk and
-                      learned sweeps, whose error against FP64 is measured,
-                      not bit-level): products feeding a sum become FMAs;
+                      learned sweeps) and the *_f64fast.hip units (the
+                      contracted FP64 sweeps), whose error against the plain
+                      FP64 form is measured, not bit-level: products feeding a
+                      sum become FMAs;
   -Xclang -target-feature -Xclang -packed-fp32-ops
                       no packed FP32 instructions.
 """
@@ -40,7 +42,7 @@ def sources():
 
 
 def _flags(src: str) -> list[str]:
-    if src.endswith("_f32.hip"):
+    if src.endswith("_f32.hip") or src.endswith("_f64fast.hip"):
         return ["-ffp-contract=fast" if f == "-ffp-contract=off" else f for f in FLAGS]
     return FLAGS
 

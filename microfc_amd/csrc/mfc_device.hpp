@@ -1,15 +1,28 @@
 // mfc_device.hpp -- per-cell and per-face device arithmetic of the RHS.
 //
 // Every routine restates the reference's arithmetic in its evaluation order
-// (the library is built with -ffp-contract=off, so each multiply and add is
-// rounded on its own, like the Fortran statements).  R is the arithmetic type
-// of the WENO/HLLC bulk: double for the plain path, float for the FP32-bulk
-// form.  Conversion to primitive variables always runs in double.
+// (the plain units are built with -ffp-contract=off, so each multiply and add
+// is rounded on its own, like the Fortran statements).  R is the arithmetic
+// type of the WENO/HLLC bulk: double for the FP64 forms, float for the
+// FP32-bulk form.  Conversion to primitive variables always runs in double.
+//
+// The arithmetic lives in an inline namespace named by the translation unit's
+// form (MFC_FORM), so units built with different flags or with the
+// contracted FP64 form (MFC_FAST_F64 = 1: FMA contraction, divisions as a
+// multiply by a Newton-refined reciprocal, the WENO weights normalised by one
+// reciprocal of their sum) never share a kernel or function symbol.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+
+#ifndef MFC_FORM
+#define MFC_FORM ieee
+#endif
+#ifndef MFC_FAST_F64
+#define MFC_FAST_F64 0
+#endif
 
 namespace mfc {
 
@@ -31,10 +44,37 @@ struct Eos {
     double pi_infs[kNumFluidsMax];
 };
 
-// Division in the bulk precision: IEEE correctly rounded in FP64 (the plain
-// path must reproduce the reference bit for bit); in the FP32 bulk a
-// reciprocal-multiply (about 1 ulp), which is what that form trades for speed.
-__device__ __forceinline__ double bdiv(double a, double b) { return a / b; }
+// WENO coefficient planes of the 27-entry table: poly_coef_cbL(r,q) at
+// 2r+q, cbR at 6+2r+q, d_cbL(r) at 12+r, d_cbR(r) at 15+r, beta_coef(r,q) at
+// 18+3r+q.
+enum { kCbL = 0, kCbR = 6, kDL = 12, kDR = 15, kBeta = 18, kNCoef = 27 };
+
+inline namespace MFC_FORM {
+
+constexpr bool kFastF64 = MFC_FAST_F64 != 0;
+
+// Reciprocal of an FP64 value to within about one ulp: v_rcp_f64 refined by
+// two Newton steps (the first half of the compiler's division sequence,
+// without the residual correction that makes a quotient correctly rounded).
+// Valid where b and 1/b are normal; 0 and infinities give inf and 0.
+__device__ __forceinline__ double frcp(double b) {
+    const double r = __builtin_amdgcn_rcp(b);
+    const double e0 = __builtin_fma(-b, r, 1.0);
+    const double r1 = __builtin_fma(r, e0, r);
+    const double e1 = __builtin_fma(-b, r1, 1.0);
+    return __builtin_fma(r1, e1, r1);
+}
+
+// Division in the bulk precision: IEEE correctly rounded in the plain FP64
+// form (it reproduces the reference bit for bit); a multiply by frcp (about
+// 1.5 ulp) in the contracted FP64 form; in the FP32 bulk a reciprocal-multiply
+// (about 1 ulp), which is what that form trades for speed.
+__device__ __forceinline__ double bdiv(double a, double b) {
+    if constexpr (kFastF64)
+        return a * frcp(b);
+    else
+        return a / b;
+}
 __device__ __forceinline__ float bdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double bsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float bsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
@@ -62,31 +102,14 @@ __device__ __forceinline__ void cons_to_prim(const double *c, double *p, const E
     rho = max_ref(rho, 1e-16);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        p[Y::MOM + d] = c[Y::MOM + d] / rho;
+        p[Y::MOM + d] = bdiv(c[Y::MOM + d], rho);
         dyn = dyn + 0.5 * c[Y::MOM + d] * p[Y::MOM + d];
     }
-    p[Y::E] = (c[Y::E] - dyn - pinf) / gam;
+    p[Y::E] = bdiv(c[Y::E] - dyn - pinf, gam);
 #pragma unroll
     for (int i = 0; i < NF; ++i) p[Y::ADV + i] = c[Y::ADV + i];
 }
 
-// WENO coefficients of one cell, in the precision of the bulk.  Plane
-// numbering of the 27-entry table: poly_coef_cbL(r,q) at 2r+q, cbR at 6+2r+q,
-// d_cbL(r) at 12+r, d_cbR(r) at 15+r, beta_coef(r,q) at 18+3r+q.
-enum { kCbL = 0, kCbR = 6, kDL = 12, kDR = 15, kBeta = 18, kNCoef = 27 };
-
-// Smoothness indicators of cell j from its stencil (m_weno.fpp:476-506;
-// WENO3 :425-438).  d[] are the undivided differences.
-template <int WO, typename R>
-struct WenoCell {
-    R d[2 * ((WO - 1) / 2)]; // WENO5: d1, d0, dm1, dm2; WENO3: d0, dm1
-    R beta[(WO - 1) / 2 + 1];
-};
-
-// Reconstruction of one variable at one cell (m_weno.fpp:391-540): v[0..WO-1]
-// is the stencil j-polyn..j+polyn, cf the cell's coefficients.  Computes the
-// smoothness indicators once and evaluates the left (face j-1/2) and/or right
-// (face j+1/2) value.
 // FP64 quotient of the WENO weights: the instruction sequence the compiler
 // emits for an IEEE a / b on gfx950 (v_rcp_f64, two Newton steps, the
 // quotient and one residual correction) without its range-scaling
@@ -127,8 +150,11 @@ __device__ __forceinline__ float wdiv(float a, float b) {
 // Reconstruction of one variable at one cell (m_weno.fpp:391-540): v[0..WO-1]
 // is the stencil j-polyn..j+polyn, cf the cell's coefficients.  Computes the
 // smoothness indicators once and evaluates the left (face j-1/2) and/or right
-// (face j+1/2) value.  In FP64 the weight divisions are ndiv (bit-identical
-// to IEEE division in the range stated there).
+// (face j+1/2) value (smoothness indicators m_weno.fpp:476-506, WENO3
+// :425-438).  In the plain FP64 form the weight divisions are ndiv
+// (bit-identical to IEEE division in the range stated there); the contracted
+// form multiplies by the reciprocals of beta_k^2 and of sum(alpha), which
+// both faces share.
 template <int WO, typename R, bool WANT_L, bool WANT_R>
 __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL, R &vR) {
     if constexpr (WO == 1) {
@@ -142,6 +168,20 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
         const R bb0 = b0 * b0, bb1 = b1 * b1;
         const R pL0 = vj + cf[kCbL + 0] * d0, pL1 = vj + cf[kCbL + 2] * dm1;
         const R pR0 = vj + cf[kCbR + 0] * d0, pR1 = vj + cf[kCbR + 2] * dm1;
+        if constexpr (kFastF64 && sizeof(R) == 8) {
+            // alpha_k = d_k / beta_k^2 and omega_k = alpha_k / sum(alpha) as
+            // products with the reciprocals (shared by both faces)
+            const R r0 = frcp(bb0), r1 = frcp(bb1);
+            if constexpr (WANT_L) {
+                const R a0 = cf[kDL + 0] * r0, a1 = cf[kDL + 1] * r1;
+                vL = (a0 * pL0 + a1 * pL1) * frcp(a0 + a1);
+            }
+            if constexpr (WANT_R) {
+                const R a0 = cf[kDR + 0] * r0, a1 = cf[kDR + 1] * r1;
+                vR = (a0 * pR0 + a1 * pR1) * frcp(a0 + a1);
+            }
+            return;
+        }
         auto weights = [&](auto fast) {
             constexpr bool F = decltype(fast)::value;
             if constexpr (WANT_L) {
@@ -175,6 +215,18 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
             pR[0] = vj + cf[kCbR + 0] * d1 + cf[kCbR + 1] * d0;
             pR[1] = vj + cf[kCbR + 2] * d0 + cf[kCbR + 3] * dm1;
             pR[2] = vj + cf[kCbR + 4] * dm1 + cf[kCbR + 5] * dm2;
+        }
+        if constexpr (kFastF64 && sizeof(R) == 8) {
+            const R r0 = frcp(bb0), r1 = frcp(bb1), r2 = frcp(bb2);
+            if constexpr (WANT_L) {
+                const R a0 = cf[kDL + 0] * r0, a1 = cf[kDL + 1] * r1, a2 = cf[kDL + 2] * r2;
+                vL = (a0 * pL[0] + a1 * pL[1] + a2 * pL[2]) * frcp(a0 + a1 + a2);
+            }
+            if constexpr (WANT_R) {
+                const R a0 = cf[kDR + 0] * r0, a1 = cf[kDR + 1] * r1, a2 = cf[kDR + 2] * r2;
+                vR = (a0 * pR[0] + a1 * pR[1] + a2 * pR[2]) * frcp(a0 + a1 + a2);
+            }
+            return;
         }
         auto weights = [&](auto fast) {
             constexpr bool F = decltype(fast)::value;
@@ -271,4 +323,5 @@ __device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &e
     }
 }
 
+} // inline namespace MFC_FORM
 } // namespace mfc

@@ -14,9 +14,11 @@ struct SweepLaunch {
     int precision; // MFC_PREC_*
 };
 
-// F32: the FP32-bulk and learned forms (their translation units are built
-// with FMA contraction), else the plain FP64 form (built without).
-template <int NF, bool F32>
+// Arithmetic forms, one set of translation units each: the plain FP64 form
+// (built without FMA contraction), the FP32-bulk and learned forms, and the
+// contracted FP64 form (the last two built with contraction).
+enum SweepForm { kFormF64 = 0, kFormF32 = 1, kFormF64Fast = 2 };
+template <int NF, int FORM>
 int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
 
 #define MFC_SWEEP_HELPERS(NF)                                                                           \
@@ -64,14 +66,21 @@ int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
 #define MFC_SWEEP_LAUNCH_F64(NF)                                                                        \
     MFC_SWEEP_HELPERS(NF)                                                                               \
     template <>                                                                                         \
-    int launch_sweep<NF, false>(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {             \
+    int launch_sweep<NF, kFormF64>(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {          \
         return L.precision == 0 ? launch_nf##NF##_r<double>(L, a, st) : -1;                             \
+    }
+
+#define MFC_SWEEP_LAUNCH_F64FAST(NF)                                                                    \
+    MFC_SWEEP_HELPERS(NF)                                                                               \
+    template <>                                                                                         \
+    int launch_sweep<NF, kFormF64Fast>(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {      \
+        return L.precision == 3 ? launch_nf##NF##_r<double>(L, a, st) : -1;                             \
     }
 
 #define MFC_SWEEP_LAUNCH_F32(NF)                                                                        \
     MFC_SWEEP_HELPERS(NF)                                                                               \
     template <>                                                                                         \
-    int launch_sweep<NF, true>(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {              \
+    int launch_sweep<NF, kFormF32>(const SweepLaunch &L, const SweepArgs &a, hipStream_t st) {          \
         if (L.precision == 2) {                                                                         \
             if (L.weno_order != 5) return -1;                                                           \
             switch (L.num_dims) {                                                                       \

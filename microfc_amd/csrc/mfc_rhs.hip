@@ -17,14 +17,19 @@ extern "C" {
 
 namespace mfc {
 const MlpFrags *mlp_frags();
-template <> int launch_sweep<1, false>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<2, false>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<3, false>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<4, false>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<1, true>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<2, true>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<3, true>(const SweepLaunch &, const SweepArgs &, hipStream_t);
-template <> int launch_sweep<4, true>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+#define MFC_DECLARE_SWEEP(NF, FORM) \
+    template <> int launch_sweep<NF, FORM>(const SweepLaunch &, const SweepArgs &, hipStream_t);
+#define MFC_DECLARE_SWEEPS(FORM) \
+    MFC_DECLARE_SWEEP(1, FORM) MFC_DECLARE_SWEEP(2, FORM) MFC_DECLARE_SWEEP(3, FORM) MFC_DECLARE_SWEEP(4, FORM)
+#define MFC_DECLARE_SWEEPS_WIDE(FORM)                                                                 \
+    MFC_DECLARE_SWEEPS(FORM) MFC_DECLARE_SWEEP(5, FORM) MFC_DECLARE_SWEEP(6, FORM) MFC_DECLARE_SWEEP(7, FORM) \
+    MFC_DECLARE_SWEEP(8, FORM) MFC_DECLARE_SWEEP(9, FORM) MFC_DECLARE_SWEEP(10, FORM)
+MFC_DECLARE_SWEEPS_WIDE(kFormF64)
+MFC_DECLARE_SWEEPS(kFormF32)
+MFC_DECLARE_SWEEPS_WIDE(kFormF64Fast)
+#undef MFC_DECLARE_SWEEPS_WIDE
+#undef MFC_DECLARE_SWEEPS
+#undef MFC_DECLARE_SWEEP
 } // namespace mfc
 
 using namespace mfc;
@@ -44,7 +49,8 @@ namespace {
 // mfc_rhs_init with that device current.
 double *g_scratch_dev[kMaxDevices] = {};
 
-constexpr int kNumFluidsBuilt = 4;
+constexpr int kNumFluidsBuilt = 10;   // the FP64 forms (every num_fluids the reference allows)
+constexpr int kNumFluidsBuiltF32 = 4; // the FP32-bulk and learned forms
 
 thread_local std::string g_err;
 
@@ -83,7 +89,7 @@ struct State {
     long nxg = 0, nyg = 0, nzg = 0, ng = 0, nint = 0;
     Dir dir[3];
     hipStream_t own = nullptr, stream = nullptr;
-    double *partials = nullptr; // kReduceBlocks doubles
+    double *partials = nullptr; // 2 * kReduceBlocks doubles
     double *scratch = nullptr;  // this device's `scratch` array (kReduceBlocks doubles)
     double *scratch_state = nullptr; // S*ng doubles, RK3 stage buffer
     bool visc = false;               // any fluid_pp%Re > 0
@@ -180,25 +186,31 @@ int check_params(const mfc_params &p) {
         if (!(p.gammas[i] > 0.0)) return fail(-1, "Unsupported value of fluid_pp(%d)%%gamma", i + 1);
         if (p.pi_infs[i] < 0.0) return fail(-1, "Unsupported value of fluid_pp(%d)%%pi_inf", i + 1);
     }
-    if (p.precision != MFC_PREC_FP64 && p.precision != MFC_PREC_FP32_BULK && p.precision != MFC_PREC_FP32_LEARNED)
+    if (p.precision != MFC_PREC_FP64 && p.precision != MFC_PREC_FP32_BULK && p.precision != MFC_PREC_FP32_LEARNED &&
+        p.precision != MFC_PREC_FP64_FAST)
         return fail(-1, "Unsupported precision");
+    const bool f64 = p.precision == MFC_PREC_FP64 || p.precision == MFC_PREC_FP64_FAST;
+    if (!f64 && p.num_fluids > kNumFluidsBuiltF32)
+        return fail(-1, "num_fluids = %d: the FP32 forms are built for 1..%d fluids", p.num_fluids,
+                    kNumFluidsBuiltF32);
     if (p.precision == MFC_PREC_FP32_LEARNED && p.weno_order != 5)
         return fail(-1, "learned WENO weights need weno_order = 5");
     bool visc = false;
     for (int i = 0; i < p.num_fluids; ++i)
         for (int r = 0; r < 2; ++r) visc = visc || p.Re[i][r] > 0.0;
     if (visc && D > 2) return fail(-1, "viscous terms are 1D and 2D only, as in the reference");
-    if (visc && p.precision != MFC_PREC_FP64) return fail(-1, "viscous terms run in the FP64 form only");
+    if (visc && p.precision != MFC_PREC_FP64) return fail(-1, "viscous terms run in the plain FP64 form only");
     if (visc && p.num_fluids > kViscFluidsMax)
         return fail(-1, "viscous kernels are built for 1..%d fluids", kViscFluidsMax);
     // The FP64 weight divisions (ndiv) equal IEEE division while the squared
     // smoothness indicators stay within [1e-200, 1e100], which weno_eps >=
-    // 1e-100 guarantees from below.
-    if (p.precision == MFC_PREC_FP64 && p.weno_order > 1 && !(p.weno_eps >= 1e-100))
+    // 1e-100 guarantees from below (the contracted form's reciprocals of
+    // beta^2 need the same range to stay normal).
+    if (f64 && p.weno_order > 1 && !(p.weno_eps >= 1e-100))
         return fail(-1, "weno_eps below 1e-100 is not supported in FP64 (the weight division is exact for "
                         "weno_eps >= 1e-100)");
     // In FP32, beta^2 >= weno_eps^2 must stay a normal float and d/beta^2 finite.
-    if (p.precision != MFC_PREC_FP64 && p.weno_order > 1 && !(p.weno_eps >= 1e-18))
+    if (!f64 && p.weno_order > 1 && !(p.weno_eps >= 1e-18))
         return fail(-1, "weno_eps below 1e-18 is not supported in the FP32 forms (its square underflows)");
     return 0;
 }
@@ -449,6 +461,82 @@ __global__ __launch_bounds__(256) void k_icfl(Fields q, Eos eos, Box x, const do
     }
 }
 
+// Stability criteria of one cell (m_data_output.fpp:214-263) from its
+// primitive values, for viscous cases: the mixture Reynolds numbers of
+// s_convert_species_to_mixture_variables_acc (m_variables_conversion.fpp:
+// 210-223), VCFL = maxval(dt/Re)/min(dx, dy)**2 and
+// Rc = min(dx(|u|+c), dy(|v|+c))/maxval(1/Re) (1D: dx alone).
+template <int NF, int D>
+__device__ __forceinline__ void visc_criteria(const double *pr, const Eos &eos, const ViscParams &vp, double dxi,
+                                              double dyj, double dt, double &vcfl, double &rc) {
+    using Y = Sys<NF, D>;
+    double rho = 0.0, gam = 0.0, pinf = 0.0, Re[2];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        rho = rho + pr[f];
+        gam = gam + pr[Y::ADV + f] * eos.gammas[f];
+        pinf = pinf + pr[Y::ADV + f] * eos.pi_infs[f];
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        double v = -1e6; // dflt_real
+        if (vp.size[r] > 0) v = 0.0;
+        for (int q = 0; q < vp.size[r]; ++q) v = pr[Y::ADV + vp.idx[r][q]] / vp.res[r][q] + v;
+        Re[r] = 1.0 / max_ref(v, 1e-16);
+    }
+    const double c = sqrt(((gam + 1.0) * pr[Y::E] + pinf) / (gam * rho));
+    const double dtre = max_ref(dt / Re[0], dt / Re[1]);
+    const double inv = max_ref(1.0 / Re[0], 1.0 / Re[1]);
+    if constexpr (D == 1) {
+        vcfl = dtre / (dxi * dxi);
+        rc = dxi * (fabs(pr[Y::MOM]) + c) / inv;
+    } else {
+        const double h = min_ref(dxi, dyj);
+        vcfl = dtre / (h * h);
+        rc = min_ref(dxi * (fabs(pr[Y::MOM]) + c), dyj * (fabs(pr[Y::MOM + 1]) + c)) / inv;
+    }
+}
+
+// VCFL maximum and Rc minimum over the interior (as the negated maximum of
+// -Rc, so both reduce with max_nan); block results to partials[blockIdx.x]
+// and partials[gridDim.x + blockIdx.x].
+template <int NF, int D>
+__global__ __launch_bounds__(256) void k_visc_criteria(Fields q, Eos eos, ViscParams vp, Box x, const double *dx,
+                                                       const double *dy, double dt, long nint, double *partials) {
+    constexpr int S = Sys<NF, D>::S;
+    double mv = -INFINITY, mr = -INFINITY;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < nint; e += (long)gridDim.x * 256) {
+        const long r = e / (x.m + 1);
+        const int i = (int)(e - r * (x.m + 1));
+        const int j = (int)(r % (x.n + 1));
+        const long o = box_off(x, i, j, 0);
+        double pr[S];
+#pragma unroll
+        for (int v = 0; v < S; ++v) pr[v] = q.v[v][o];
+        double vcfl, rc;
+        visc_criteria<NF, D>(pr, eos, vp, dx[i], D > 1 ? dy[j] : 0.0, dt, vcfl, rc);
+        mv = max_nan(mv, vcfl);
+        mr = max_nan(mr, -rc);
+    }
+    __shared__ double red[2][4];
+    mv = wave_max(mv);
+    mr = wave_max(mr);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = mv;
+        red[1][threadIdx.x >> 6] = mr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red[0][0], b = red[1][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            a = max_nan(a, red[0][w]);
+            b = max_nan(b, red[1][w]);
+        }
+        partials[blockIdx.x] = a;
+        partials[gridDim.x + blockIdx.x] = b;
+    }
+}
+
 // Final pass over the block partials: one block of 1024 threads (eight
 // partials per thread at 8192 blocks).
 __global__ __launch_bounds__(1024) void k_reduce_max(double *partials, int n) {
@@ -530,6 +618,12 @@ int dispatch_nf_d(F &&f) {
     MFC_CASE(2, 1) MFC_CASE(2, 2) MFC_CASE(2, 3)
     MFC_CASE(3, 1) MFC_CASE(3, 2) MFC_CASE(3, 3)
     MFC_CASE(4, 1) MFC_CASE(4, 2) MFC_CASE(4, 3)
+    MFC_CASE(5, 1) MFC_CASE(5, 2) MFC_CASE(5, 3)
+    MFC_CASE(6, 1) MFC_CASE(6, 2) MFC_CASE(6, 3)
+    MFC_CASE(7, 1) MFC_CASE(7, 2) MFC_CASE(7, 3)
+    MFC_CASE(8, 1) MFC_CASE(8, 2) MFC_CASE(8, 3)
+    MFC_CASE(9, 1) MFC_CASE(9, 2) MFC_CASE(9, 3)
+    MFC_CASE(10, 1) MFC_CASE(10, 2) MFC_CASE(10, 3)
 #undef MFC_CASE
     return fail(-1, "no kernels for num_fluids=%d, num_dims=%d", nf, D);
 }
@@ -600,7 +694,8 @@ SweepArgs sweep_args(int d) {
                         : G.prm.precision == MFC_PREC_FP32_BULK
                             ? (uni ? t_cell_parallel<false, float, true>() : t_cell_parallel<false, float, false>())
                             : (uni ? t_cell_parallel<false, double, true>() : t_cell_parallel<false, double, false>());
-        const int xc = !cp ? kTX : G.prm.precision == MFC_PREC_FP64 ? tc_cols<double>(d) : tc_cols<float>(d);
+        const bool f64 = G.prm.precision == MFC_PREC_FP64 || G.prm.precision == MFC_PREC_FP64_FAST;
+        const int xc = !cp ? kTX : f64 ? tc_cols<double>(d) : tc_cols<float>(d);
         a.ntile = (G.prm.m + 1 + xc - 1) / xc;
         a.chunk = G.prm.precision == MFC_PREC_FP32_LEARNED ? t_tile_cells<true, float, true>(d)
                   : G.prm.precision == MFC_PREC_FP32_BULK
@@ -621,12 +716,27 @@ int sweep(int d, SweepArgs &a) {
     }
     TimedScope ts(kKSweepX + d);
     int rc;
-    const bool f32 = G.prm.precision != MFC_PREC_FP64;
+    const int form = G.prm.precision == MFC_PREC_FP64        ? kFormF64
+                     : G.prm.precision == MFC_PREC_FP64_FAST ? kFormF64Fast
+                                                             : kFormF32;
+    auto go = [&](auto nf) -> int {
+        constexpr int NF = decltype(nf)::value;
+        if (form == kFormF64) return launch_sweep<NF, kFormF64>(L, a, G.stream);
+        if (form == kFormF64Fast) return launch_sweep<NF, kFormF64Fast>(L, a, G.stream);
+        if constexpr (NF <= kNumFluidsBuiltF32) return launch_sweep<NF, kFormF32>(L, a, G.stream);
+        return -1;
+    };
     switch (G.prm.num_fluids) {
-    case 1: rc = f32 ? launch_sweep<1, true>(L, a, G.stream) : launch_sweep<1, false>(L, a, G.stream); break;
-    case 2: rc = f32 ? launch_sweep<2, true>(L, a, G.stream) : launch_sweep<2, false>(L, a, G.stream); break;
-    case 3: rc = f32 ? launch_sweep<3, true>(L, a, G.stream) : launch_sweep<3, false>(L, a, G.stream); break;
-    case 4: rc = f32 ? launch_sweep<4, true>(L, a, G.stream) : launch_sweep<4, false>(L, a, G.stream); break;
+    case 1: rc = go(std::integral_constant<int, 1>{}); break;
+    case 2: rc = go(std::integral_constant<int, 2>{}); break;
+    case 3: rc = go(std::integral_constant<int, 3>{}); break;
+    case 4: rc = go(std::integral_constant<int, 4>{}); break;
+    case 5: rc = go(std::integral_constant<int, 5>{}); break;
+    case 6: rc = go(std::integral_constant<int, 6>{}); break;
+    case 7: rc = go(std::integral_constant<int, 7>{}); break;
+    case 8: rc = go(std::integral_constant<int, 8>{}); break;
+    case 9: rc = go(std::integral_constant<int, 9>{}); break;
+    case 10: rc = go(std::integral_constant<int, 10>{}); break;
     default: rc = -1;
     }
     if (rc != 0) return fail(-3, "sweep launch failed (dir %d): %s", d, hipGetErrorString(hipGetLastError()));
@@ -707,7 +817,12 @@ int mfc_rhs_init(const mfc_params *prm) {
     G.D = 1 + (prm->n > 0) + (prm->p > 0);
     G.S = 2 * prm->num_fluids + G.D + 1;
     G.polyn = (prm->weno_order - 1) / 2;
-    G.b = G.polyn + 2;
+    // buff_size (m_global_parameters.fpp:355-359): viscous cases keep
+    // 2*polyn + 2 ghost layers, inviscid ones polyn + 2
+    bool visc = false;
+    for (int i = 0; i < prm->num_fluids; ++i)
+        for (int r = 0; r < 2; ++r) visc = visc || prm->Re[i][r] > 0.0;
+    G.b = visc ? 2 * G.polyn + 2 : G.polyn + 2;
     G.nxg = prm->m + 2 * G.b + 1;
     G.nyg = prm->n > 0 ? prm->n + 2 * G.b + 1 : 1;
     G.nzg = prm->p > 0 ? prm->p + 2 * G.b + 1 : 1;
@@ -718,7 +833,7 @@ int mfc_rhs_init(const mfc_params *prm) {
     G.dir[2].s = prm->p;
     HIPCHK(hipStreamCreateWithFlags(&G.own, hipStreamNonBlocking));
     G.stream = G.own;
-    HIPCHK(hipMalloc(&G.partials, sizeof(double) * kReduceBlocks));
+    HIPCHK(hipMalloc(&G.partials, sizeof(double) * 2 * kReduceBlocks));
     {
         int dev = 0;
         HIPCHK(hipGetDevice(&dev));
@@ -898,6 +1013,44 @@ static int icfl_reduce(double *const *q, bool from_cons, double dt, double *out,
 
 int mfc_icfl_max(double *const *q_prim_vf, double dt, double *icfl_max) {
     return icfl_reduce(q_prim_vf, false, dt, icfl_max, G.partials, kReduceBlocks, true);
+}
+
+int mfc_stability_criteria(double *const *q_prim_vf, double dt, double *out) {
+    if (!out) return fail(-1, "NULL output");
+    out[1] = 0.0;
+    out[2] = 0.0;
+    if (int rc = icfl_reduce(q_prim_vf, false, dt, &out[0], G.partials, kReduceBlocks, true)) return rc;
+    if (!G.visc) return 0;
+    Fields f;
+    if (int rc = fields(q_prim_vf, f, "q_prim_vf")) return rc;
+    const Box x = box();
+    const Eos e = eos();
+    const int nb = (int)std::min<long>(kReduceBlocks, std::max<long>(1, (G.nint + 255) / 256));
+    const double *dy = G.D > 1 ? G.dir[1].ds_dev : nullptr;
+    int rc = 0;
+    if (G.D == 1 || G.D == 2) {
+        rc = dispatch_nf_d([&](auto NF, auto D) -> int {
+            if constexpr (decltype(D)::value <= 2 && decltype(NF)::value <= kViscFluidsMax) {
+                hipLaunchKernelGGL((k_visc_criteria<decltype(NF)::value, decltype(D)::value>), dim3(nb), dim3(256),
+                                   0, G.stream, f, e, G.vp, x, G.dir[0].ds_dev, dy, dt, G.nint, G.partials);
+                HIPCHK(hipGetLastError());
+                return 0;
+            } else {
+                return fail(-1, "viscous criteria are 1D and 2D only");
+            }
+        });
+    }
+    if (rc) return rc;
+    double h[2];
+    for (int k = 0; k < 2; ++k) {
+        hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, G.stream, G.partials + (long)k * nb, nb);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(&h[k], G.partials + (long)k * nb, sizeof(double), hipMemcpyDeviceToHost, G.stream));
+    }
+    HIPCHK(hipStreamSynchronize(G.stream));
+    out[1] = h[0];
+    out[2] = -h[1];
+    return 0;
 }
 
 static int cfl_dt(double *const *q_cons_vf, double cfl, double *dt_out, double *partials, bool timed) {

@@ -73,6 +73,10 @@ struct SweepArgs {
     float ucf32[kNCoef];
 };
 
+// Kernels and their helpers belong to the unit's arithmetic form (see
+// mfc_device.hpp); the argument types above are shared by every form.
+inline namespace MFC_FORM {
+
 __device__ __forceinline__ long ghost_off(const SweepArgs &a, int i, int j, int k) {
     return (long)(i + a.b) + a.nxg * ((long)(j + a.bj) + a.nyg * (long)(k + a.bk));
 }
@@ -148,6 +152,15 @@ __device__ __forceinline__ void finish_load_staged(const SweepArgs &a, long go, 
     }
 }
 
+// x/3 of the third RK3 stage: the IEEE quotient in the plain form, a
+// multiply by the rounded 1/3 in the contracted FP64 form.
+__device__ __forceinline__ double third(double x) {
+    if constexpr (kFastF64)
+        return x * (1.0 / 3.0);
+    else
+        return x / 3.0;
+}
+
 // TVD-RK stage combination of the base, the stage input and rhs
 // (m_time_steppers.fpp:129-362).
 template <int S>
@@ -167,7 +180,7 @@ __device__ __forceinline__ void rk_combine(const SweepArgs &a, const FinishIn<S>
         break;
     default:
 #pragma unroll
-        for (int v = 0; v < S; ++v) o[v] = (f.base[v] + 2.0 * f.in[v] + 2.0 * a.dt * r[v]) / 3.0;
+        for (int v = 0; v < S; ++v) o[v] = third(f.base[v] + 2.0 * f.in[v] + 2.0 * a.dt * r[v]);
         break;
     }
 }
@@ -221,7 +234,7 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, long go, long ro
         break;
     default:
 #pragma unroll
-        for (int v = 0; v < S; ++v) a.qout.v[v][go] = (f.base[v] + 2.0 * f.in[v] + 2.0 * a.dt * r[v]) / 3.0;
+        for (int v = 0; v < S; ++v) a.qout.v[v][go] = third(f.base[v] + 2.0 * f.in[v] + 2.0 * a.dt * r[v]);
         break;
     }
 }
@@ -924,4 +937,5 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
     }
 }
 
+} // inline namespace MFC_FORM
 } // namespace mfc

@@ -1,5 +1,6 @@
 // Sweep kernels for num_fluids = 4, FP32-bulk and learned forms (see
 // mfc_sweep.hpp); built with FMA contraction.
+#define MFC_FORM f32
 #include "mfc_launch.hpp"
 
 namespace mfc {

@@ -1,0 +1,6 @@
+// Sweep kernels for num_fluids = 5, plain FP64 form (see mfc_sweep.hpp).
+#include "mfc_launch.hpp"
+
+namespace mfc {
+MFC_SWEEP_LAUNCH_F64(5)
+} // namespace mfc

@@ -27,10 +27,12 @@ NUM_FLUIDS_MAX = 10
 PREC_FP64 = 0
 PREC_FP32_BULK = 1
 PREC_FP32_LEARNED = 2
+PREC_FP64_FAST = 3
 
 EXPORTED = [
     "mfc_rhs_init", "mfc_rhs_set_grid", "mfc_compute_rhs", "mfc_tvd_rk_step",
-    "mfc_cons_to_prim", "mfc_icfl_max", "mfc_cfl_dt", "mfc_cfl_dt_noalloc", "mfc_sizes",
+    "mfc_cons_to_prim", "mfc_icfl_max", "mfc_stability_criteria", "mfc_cfl_dt", "mfc_cfl_dt_noalloc",
+    "mfc_sizes",
     "mfc_weno_coefficients", "mfc_device_alloc", "mfc_device_free",
     "mfc_memcpy_h2d", "mfc_memcpy_d2h", "mfc_sync", "mfc_stream",
     "mfc_set_stream", "mfc_rhs_finalize", "mfc_last_error",
@@ -83,6 +85,7 @@ def load_library(path: str = LIB_PATH):
     L.mfc_icfl_max.argtypes = [pp, ctypes.c_double, dp]
     L.mfc_cfl_dt.argtypes = [pp, ctypes.c_double, dp]
     L.mfc_cfl_dt_noalloc.argtypes = [pp, ctypes.c_double, dp]
+    L.mfc_stability_criteria.argtypes = [pp, ctypes.c_double, dp]
     L.mfc_sizes.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                             ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
     L.mfc_weno_coefficients.argtypes = [ctypes.c_int, dp]
@@ -141,7 +144,11 @@ class Params:
 
     @property
     def buff_size(self) -> int:
-        return (self.weno_order - 1) // 2 + 2
+        """Ghost layers per side, as the reference sizes them
+        (src/simulation/m_global_parameters.fpp:355-359): 2*polyn + 2 when any
+        fluid has a Reynolds number, else polyn + 2 (polyn = (weno_order-1)/2)."""
+        polyn = (self.weno_order - 1) // 2
+        return 2 * polyn + 2 if self.viscous else polyn + 2
 
     @property
     def ghost_shape(self):
@@ -304,6 +311,15 @@ class RhsModule:
         out = ctypes.c_double()
         _check(self.lib.mfc_icfl_max(self._ptrs(q_prim_vf), dt, ctypes.byref(out)), "mfc_icfl_max")
         return out.value
+
+    def stability_criteria(self, q_prim_vf, dt):
+        """(ICFL max, VCFL max, Rc min) of s_write_run_time_information
+        (src/simulation/m_data_output.fpp:197-263); the last two are 0 when
+        the case is inviscid."""
+        self._enter()
+        out = (ctypes.c_double * 3)()
+        _check(self.lib.mfc_stability_criteria(self._ptrs(q_prim_vf), dt, out), "mfc_stability_criteria")
+        return out[0], out[1], out[2]
 
     def cfl_dt(self, q_cons_vf, cfl):
         self._enter()

@@ -41,7 +41,15 @@ int orc_max_threads(void) {
 
 int orc_num_dims(const orc_case *c) { return 1 + (c->n > 0) + (c->p > 0); }
 int orc_sys_size(const orc_case *c) { return 2 * c->num_fluids + orc_num_dims(c) + 1; }
-int orc_buff_size(const orc_case *c) { return (c->weno_order - 1) / 2 + 2; }
+/* m_global_parameters.fpp:355-359: twice the WENO half-width plus 2 ghost
+ * layers when any fluid has a Reynolds number, else the half-width plus 2. */
+int orc_buff_size(const orc_case *c) {
+    const int polyn = (c->weno_order - 1) / 2;
+    int i, r, visc = 0;
+    for (i = 0; i < c->num_fluids; i++)
+        for (r = 0; r < 2; r++) visc = visc || c->Re[i][r] > 0.0;
+    return visc ? 2 * polyn + 2 : polyn + 2;
+}
 
 typedef struct {
     int m, n, p, b, bj, bk;
@@ -1026,5 +1034,68 @@ int orc_icfl_max(const orc_case *c, const double *const ds[3], const double *q_p
                 else if (!any || icfl > mx) { mx = icfl; any = 1; }
             }
     *icfl_max = nan ? NAN : mx;
+    return 0;
+}
+
+/* VCFL maximum and Rc minimum of a viscous case (m_data_output.fpp:214-263)
+ * with the mixture Reynolds numbers of m_variables_conversion.fpp:210-223:
+ * Re_K(i) = 1/max(sum_j alpha(Re_idx(i,j))/Res(i,j), sgm_eps), dflt_real
+ * before the sum when fluid i has no Reynolds number of that kind.
+ * out[0] = ICFL max, out[1] = VCFL max, out[2] = Rc min (0, 0 when inviscid);
+ * NaN anywhere gives NaN. */
+int orc_stability(const orc_case *c, const double *const ds[3], const double *q_prim, double dt, double out[3]) {
+    geom g;
+    int i, j, f, r, visc = 0;
+    double vmax = 0.0, rmin = 0.0;
+    int any = 0, nan = 0;
+    mkgeom(c, &g);
+    orc_icfl_max(c, ds, q_prim, dt, &out[0]);
+    for (f = 0; f < g.nf; f++)
+        for (r = 0; r < 2; r++) visc = visc || c->Re[f][r] > 0.0;
+    out[1] = 0.0;
+    out[2] = 0.0;
+    if (!visc) return 0;
+    if (g.D > 2) return -1;
+    for (j = 0; j <= g.n; j++)
+        for (i = 0; i <= g.m; i++) {
+            const long o = gidx(&g, i, j, 0);
+            double rho = 0.0, gamma = 0.0, pi_inf = 0.0, Re[2], cc, vcfl, rc, dtre, inv;
+            for (f = 0; f < g.nf; f++) {
+                rho = rho + q_prim[f * g.ng + o];
+                gamma = gamma + q_prim[(g.advxb + f) * g.ng + o] * c->gammas[f];
+                pi_inf = pi_inf + q_prim[(g.advxb + f) * g.ng + o] * c->pi_infs[f];
+            }
+            for (r = 0; r < 2; r++) {
+                int size = 0;
+                double v = -1e6;
+                for (f = 0; f < g.nf; f++) size += c->Re[f][r] > 0.0;
+                if (size > 0) v = 0.0;
+                for (f = 0; f < g.nf; f++)
+                    if (c->Re[f][r] > 0.0) v = q_prim[(g.advxb + f) * g.ng + o] / c->Re[f][r] + v;
+                Re[r] = 1.0 / fmax_ref(v, 1e-16);
+            }
+            cc = (((gamma + 1.0) * q_prim[g.E_idx * g.ng + o] + pi_inf) / (gamma * rho));
+            cc = sqrt(cc);
+            dtre = fmax_ref(dt / Re[0], dt / Re[1]);
+            inv = fmax_ref(1.0 / Re[0], 1.0 / Re[1]);
+            if (g.D == 1) {
+                const double h = ds[0][i + g.b];
+                vcfl = dtre / (h * h);
+                rc = h * (fabs(q_prim[g.momxb * g.ng + o]) + cc) / inv;
+            } else {
+                const double h = fmin_ref(ds[0][i + g.b], ds[1][j + g.b]);
+                vcfl = dtre / (h * h);
+                rc = fmin_ref(ds[0][i + g.b] * (fabs(q_prim[g.momxb * g.ng + o]) + cc),
+                              ds[1][j + g.b] * (fabs(q_prim[(g.momxb + 1) * g.ng + o]) + cc)) / inv;
+            }
+            if (vcfl != vcfl || rc != rc) nan = 1;
+            else if (!any) { vmax = vcfl; rmin = rc; any = 1; }
+            else {
+                if (vcfl > vmax) vmax = vcfl;
+                if (rc < rmin) rmin = rc;
+            }
+        }
+    out[1] = nan ? NAN : vmax;
+    out[2] = nan ? NAN : rmin;
     return 0;
 }

@@ -106,6 +106,7 @@ int orc_tvd_rk_step(const orc_case *c, const double *const cb[3], const double *
  * primitive field.  NaN in any cell gives NaN. */
 int orc_icfl_max(const orc_case *c, const double *const ds[3], const double *q_prim, double dt,
                  double *icfl_max);
+int orc_stability(const orc_case *c, const double *const ds[3], const double *q_prim, double dt, double out[3]);
 
 /* Single-face kernels exposed for unit tests. */
 /* WENO5/3/1 of one variable at one cell from the 5-point stencil v[0..4]

@@ -60,6 +60,7 @@ def lib():
         L.orc_compute_rhs.argtypes = [cp, arr3, arr3, dp, dp, dp, ctypes.c_int]
         L.orc_tvd_rk_step.argtypes = [cp, arr3, arr3, ctypes.c_double, dp, dp, dp, dp, ctypes.c_int]
         L.orc_icfl_max.argtypes = [cp, arr3, dp, ctypes.c_double, dp]
+        L.orc_stability.argtypes = [cp, arr3, dp, ctypes.c_double, dp]
         L.orc_weno_cell.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp, dp, dp]
         L.orc_weno_cell.restype = None
         L.orc_hllc_face.argtypes = [cp, ctypes.c_int, dp, dp, dp, dp]
@@ -110,7 +111,10 @@ class Case:
 
     @property
     def buff_size(self):
-        return (self.weno_order - 1) // 2 + 2
+        """m_global_parameters.fpp:355-359: 2*polyn + 2 ghost layers when any
+        fluid has a Reynolds number, else polyn + 2."""
+        polyn = (self.weno_order - 1) // 2
+        return 2 * polyn + 2 if self.viscous else polyn + 2
 
     @property
     def ghost_shape(self):
@@ -233,6 +237,15 @@ def icfl_max(case: Case, grid, q_prim, dt):
     cbs, dss = _grid_args(grid)
     lib().orc_icfl_max(ctypes.byref(case.c()), dss, _p(q_prim), dt, ctypes.byref(out))
     return out.value
+
+
+def stability(case: Case, grid, q_prim, dt):
+    """(ICFL max, VCFL max, Rc min) of m_data_output.fpp:197-263."""
+    out = np.zeros(3)
+    cbs, dss = _grid_args(grid)
+    if lib().orc_stability(ctypes.byref(case.c()), dss, _p(q_prim), dt, _p(out)) < 0:
+        raise RuntimeError("orc_stability: viscous criteria are 1D and 2D only")
+    return float(out[0]), float(out[1]), float(out[2])
 
 
 def weno_cell(weno_order, weno_eps, coef27, stencil5):

@@ -121,6 +121,30 @@ def multi_fluid_2d(nf: int, N: int = 40, seed: int = 3):
     return prm, grids, ic.ghosted(prm, ic.prim_to_cons(prm, prim))
 
 
+def multi_fluid_3d(nf: int, N: int = 26, seed: int = 4):
+    """nf stiffened-gas fluids in smooth random 3D layers with a pressure
+    pulse and all three velocity components, on a stretched grid with
+    periodic x, reflective y and Riemann-extrapolation z boundaries."""
+    rng = np.random.default_rng(seed)
+    gam = rng.uniform(1.3, 4.4, nf)
+    pinf = np.where(np.arange(nf) % 2 == 0, 0.0, rng.uniform(1e3, 1e5, nf))
+    prm = Params(m=N - 1, n=N, p=N + 1, num_fluids=nf, weno_order=5, gammas=list(1.0 / (gam - 1.0)),
+                 pi_infs=list(gam * pinf / (gam - 1.0)), bc_beg=(-1, -2, -4), bc_end=(-1, -2, -4))
+    grids = stretched(prm, (0.3, 0.4, 0.5), (0.1, 0.9, 2.0))
+    X, Y, Z = ic.mesh(prm, grids)
+    w = np.stack([1.0 + 0.5 * np.sin(2 * np.pi * (k + 1) * X + rng.uniform(0, 6)) *
+                  np.cos(2 * np.pi * (Y + Z) + rng.uniform(0, 6)) for k in range(nf)])
+    alpha = w / w.sum(axis=0)
+    prim = np.zeros(prm.interior_shape)
+    prim[:nf] = alpha * rng.uniform(1.0, 1000.0, nf)[:, None, None, None]
+    prim[nf] = 10.0 * np.sin(2 * np.pi * Y)
+    prim[nf + 1] = 5.0 * np.cos(2 * np.pi * X) * np.sin(np.pi * Y)
+    prim[nf + 2] = 3.0 * np.sin(2 * np.pi * (X + Z))
+    prim[nf + 3] = 1e5 * (1.0 + 0.5 * np.exp(-((X - 0.4) ** 2 + (Y - 0.6) ** 2 + (Z - 0.5) ** 2) / 0.01))
+    prim[nf + 4:] = alpha
+    return prm, grids, ic.ghosted(prm, ic.prim_to_cons(prm, prim))
+
+
 def stretched(case, amps, phases):
     """Non-uniform grids s_cb(xi) = f(xi) - f(0), f(xi) = xi + a sin(2 pi xi + phi)/(2 pi),
     per direction, ghosted by the case's BCs (case: an oracle Case or Params)."""
@@ -211,9 +235,11 @@ def small_cases():
     prm, g, q = ic.shock_droplet_3d(N=20, weno_order=1)
     prm.time_stepper = 1
     out.append(("droplet3d_w1_rk1", prm, g, perturbed(prm, q)))
-    for nf in (3, 4):
+    for nf in (3, 4, 6, 10):
         prm, g, q = multi_fluid_2d(nf)
         out.append((f"fluids{nf}_2d", prm, g, q))
+    prm, g, q = multi_fluid_3d(5)
+    out.append(("fluids5_3d_stretched", prm, g, q))
     out.extend(walls_3d_cases())
     return out
 
@@ -244,6 +270,44 @@ def walls_3d_cases():
     prm, g, q = field_3d_case(26, 25, 28, (-2, -2, -2), (-2, -2, -2), grid="stretched")
     out.append(("field3d_all_walls_stretched", prm, g, q))
     return out
+
+
+def reghost(src, dst, grids, q):
+    """The same interior cells and cell boundaries laid out with dst's ghost
+    width (src and dst are oracle Cases or Params of one grid, e.g. a viscous
+    case and its inviscid twin, whose buff_size differs)."""
+    from microfc_amd.rhs import populate_grid_buffers
+    bs, bd = src.buff_size, dst.buff_size
+    new_grids = []
+    for d, s in enumerate(src.extents()):
+        cbi = grids[d][0][bs:bs + s + 2]  # s_cb(-1..s)
+        new_grids.append(populate_grid_buffers(cbi, bd, dst.bc_beg[d], dst.bc_end[d]))
+    js, ks = (bs if src.n > 0 else 0), (bs if src.p > 0 else 0)
+    jd, kd = (bd if dst.n > 0 else 0), (bd if dst.p > 0 else 0)
+    qn = np.zeros(dst.ghost_shape)
+    qn[:, kd:kd + dst.p + 1, jd:jd + dst.n + 1, bd:bd + dst.m + 1] = \
+        q[:, ks:ks + src.p + 1, js:js + src.n + 1, bs:bs + src.m + 1]
+    return new_grids, qn
+
+
+def relayout(prm, grids, q):
+    """Grids and field laid out with prm's current ghost width, whatever
+    width they were built with (the width is read from the array shapes):
+    for cases made inviscid and then given Reynolds numbers."""
+    from microfc_amd.rhs import populate_grid_buffers
+    bd = prm.buff_size
+    new_grids = []
+    for d, s in enumerate(prm.extents()):
+        cb = grids[d][0]
+        bg = (cb.size - (s + 2)) // 2
+        new_grids.append(populate_grid_buffers(cb[bg:bg + s + 2], bd, prm.bc_beg[d], prm.bc_end[d]))
+    bq = (q.shape[-1] - prm.m - 1) // 2
+    jq, kq = (bq if prm.n > 0 else 0), (bq if prm.p > 0 else 0)
+    jd, kd = (bd if prm.n > 0 else 0), (bd if prm.p > 0 else 0)
+    qn = np.zeros(prm.ghost_shape)
+    qn[:, kd:kd + prm.p + 1, jd:jd + prm.n + 1, bd:bd + prm.m + 1] = \
+        q[:, kq:kq + prm.p + 1, jq:jq + prm.n + 1, bq:bq + prm.m + 1]
+    return new_grids, qn
 
 
 def say(msg):

@@ -49,7 +49,8 @@ def _init(**kw):
     (dict(weno_eps=1e-3), "weno_eps"),
     (dict(bc_beg=(-1, -3, -3)), "bc_x"),
     (dict(bc_beg=(-13, -3, -3), bc_end=(-13, -3, -3)), "bc_x"),
-    (dict(num_fluids=5, gammas=[2.5] * 5, pi_infs=[0.0] * 5), "num_fluids"),
+    (dict(num_fluids=0, gammas=[], pi_infs=[]), "Unsupported value of num_fluids"),
+    (dict(num_fluids=5, gammas=[2.5] * 5, pi_infs=[0.0] * 5, precision=1), "FP32 forms are built for 1..4"),
     (dict(gammas=[-1.0]), "gamma"),
     (dict(pi_infs=[-1.0]), "pi_inf"),
     (dict(time_stepper=4), "time_stepper"),

@@ -9,7 +9,7 @@ from microfc_amd import RhsModule, ic
 from microfc_amd.rhs import MfcError, PREC_FP32_BULK
 from oracle import oracle as O
 
-from helpers import assert_parity, multi_fluid_2d, perturbed, say, stretched_grids, to_oracle
+from helpers import assert_parity, multi_fluid_2d, perturbed, relayout, reghost, say, stretched_grids, to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -31,51 +31,48 @@ def with_velocity(prm, grids, q, amp):
     return q
 
 
+def _visc(prm, g, q, Re):
+    """The case with Reynolds numbers Re, grids and field laid out with the
+    viscous ghost width."""
+    prm.Re = Re
+    return (prm, *relayout(prm, g, q))
+
+
 def _cases():
     out = []
-    prm, g, q = ic.sod_1d(m=199)
-    prm.Re = [(1e3, 2e3)]
+    prm, g, q = _visc(*ic.sod_1d(m=199), [(1e3, 2e3)])
     out.append(("visc_sod1d_w5_rk3", prm, g, with_velocity(prm, g, perturbed(prm, q), 0.2)))
     prm, g, q = ic.sod_1d(m=63, weno_order=1)
     prm.time_stepper = 1
     prm.bc_beg = (-2, -3, -3)
     prm.bc_end = (-4, -3, -3)
-    prm.Re = [(50.0, -1.0)]
-    g = ic.uniform_grids(prm)
+    prm, g, q = _visc(prm, ic.uniform_grids(prm), q, [(50.0, -1.0)])
     out.append(("visc_sod1d_w1_rk1_reflect_extrap", prm, g, with_velocity(prm, g, perturbed(prm, q), 0.2)))
-    prm, g, q = ic.shock_bubble_2d(N=48)
-    prm.Re = [(0.05, 0.1), (0.1, 0.2)]
+    prm, g, q = _visc(*ic.shock_bubble_2d(N=48), [(0.05, 0.1), (0.1, 0.2)])
     out.append(("visc_bubble2d_48", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
     prm, g, q = ic.shock_bubble_2d(N=40, weno_order=3)
     prm.time_stepper = 2
     prm.bc_beg = (-1, -2, -3)
     prm.bc_end = (-1, -4, -3)
-    prm.Re = [(0.05, -1.0), (0.1, -1.0)]
-    g = ic.uniform_grids(prm)
+    prm, g, q = _visc(prm, ic.uniform_grids(prm), q, [(0.05, -1.0), (0.1, -1.0)])
     out.append(("visc_bubble2d_w3_rk2_periodic_reflect", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
-    prm, g, q = ic.shock_bubble_2d(N=40)
-    prm.Re = [(-1.0, 0.05), (-1.0, 0.1)]
+    prm, g, q = _visc(*ic.shock_bubble_2d(N=40), [(-1.0, 0.05), (-1.0, 0.1)])
     g = stretched_grids(prm)
     out.append(("visc_bubble2d_stretched_bulk_only", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
-    prm, g, q = ic.shock_bubble_2d(N=40)
-    prm.Re = [(0.05, 0.1), (0.1, -1.0)]
+    prm, g, q = _visc(*ic.shock_bubble_2d(N=40), [(0.05, 0.1), (0.1, -1.0)])
     g = stretched_grids(prm)
     out.append(("visc_bubble2d_stretched_shear_bulk", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
-    prm, g, q = multi_fluid_2d(3, N=32)
-    prm.Re = [(0.3, -1.0), (-1.0, 0.7), (0.9, 1.1)]
+    prm, g, q = _visc(*multi_fluid_2d(3, N=32), [(0.3, -1.0), (-1.0, 0.7), (0.9, 1.1)])
     out.append(("visc_fluids3_2d", prm, g, q))
-    prm, g, q = multi_fluid_2d(4, N=32)
-    prm.Re = [(0.3, 0.5), (0.4, -1.0), (-1.0, 0.8), (0.9, 1.1)]
+    prm, g, q = _visc(*multi_fluid_2d(4, N=32), [(0.3, 0.5), (0.4, -1.0), (-1.0, 0.8), (0.9, 1.1)])
     out.append(("visc_fluids4_2d", prm, g, q))
     prm, g, q = ic.shock_bubble_2d(N=36, weno_order=1)
     prm.time_stepper = 1
     prm.bc_beg = (-4, -1, -3)
     prm.bc_end = (-2, -1, -3)
-    prm.Re = [(0.05, 0.1), (0.1, 0.2)]
-    g = ic.uniform_grids(prm)
+    prm, g, q = _visc(prm, ic.uniform_grids(prm), q, [(0.05, 0.1), (0.1, 0.2)])
     out.append(("visc_bubble2d_w1_rk1_extrap_periodic", prm, g, with_velocity(prm, g, perturbed(prm, q), 20.0)))
-    prm, g, q = ic.shear_layer_2d(N=48)
-    prm.Re = [(200.0, -1.0)]
+    prm, g, q = _visc(*ic.shear_layer_2d(N=48), [(200.0, -1.0)])
     out.append(("visc_shear2d_48", prm, g, perturbed(prm, q)))
     return out
 
@@ -91,7 +88,9 @@ def test_viscous_rhs_parity(gpu, case):
     say(f"[viscous rhs] {name} shape={prm.ghost_shape} Re={prm.Re}")
     oc = to_oracle(prm)
     ref = O.compute_rhs(oc, grids, q.copy())
-    inv = O.compute_rhs(O.Case(**{**oc.__dict__, "Re": None}), grids, q.copy())
+    oi = O.Case(**{**oc.__dict__, "Re": None})
+    gi, qi = reghost(oc, oi, grids, q)
+    inv = O.compute_rhs(oi, gi, qi)
     mod = RhsModule(prm, grids)
     try:
         qd = torch.from_numpy(q.copy()).cuda()
@@ -144,3 +143,25 @@ def test_viscous_rejections(gpu):
     prm.precision = PREC_FP32_BULK
     with pytest.raises(MfcError, match="FP64"):
         RhsModule(prm, g)
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_stability_criteria_parity(gpu, case):
+    """ICFL max, VCFL max and Rc min of the run-time information
+    (m_data_output.fpp:197-263) equal the oracle's bit for bit."""
+    torch = gpu
+    name, prm, grids, q = case
+    say(f"[stability criteria] {name}")
+    oc = to_oracle(prm)
+    prim = O.cons_to_prim(oc, O.bc_fill(oc, q.copy()))
+    mod = RhsModule(prm, grids)
+    try:
+        pd = torch.from_numpy(prim).cuda()
+        for dt in (1e-7, 2e-4):
+            got = mod.stability_criteria(pd, dt)
+            ref = O.stability(oc, grids, prim, dt)
+            say(f"  dt={dt}: icfl {got[0]:.6e} vcfl {got[1]:.6e} Rc {got[2]:.6e}")
+            assert got == ref, (got, ref)
+            assert got[1] > 0 and got[2] > 0
+    finally:
+        mod.finalize()

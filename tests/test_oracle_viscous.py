@@ -16,6 +16,8 @@ import pytest
 
 from oracle import oracle as O
 
+from helpers import reghost
+
 GAMMA = 2.5  # fluid_pp%gamma = 1/(1.4 - 1)
 INTERIOR = np.s_[3:-3]
 
@@ -63,7 +65,8 @@ def _viscous_part(case, vel, alphas=None):
     grid = O.uniform_grid(case)
     q, X, Y = _state(case, grid, vel, alphas)
     inviscid = O.Case(**{**case.__dict__, "Re": None})
-    r = O.compute_rhs(case, grid, q.copy()) - O.compute_rhs(inviscid, grid, q.copy())
+    gi, qi = reghost(case, inviscid, grid, q)
+    r = O.compute_rhs(case, grid, q.copy()) - O.compute_rhs(inviscid, gi, qi)
     return r[:, 0], X, Y
 
 
@@ -207,3 +210,58 @@ def test_shear_wave_decays_at_the_discrete_viscous_rate():
     lam = (1.0 / Re) * (2.0 - 2.0 * np.cos(2 * np.pi * dx)) / dx ** 2
     want = A * np.exp(-lam * dt * nsteps)
     assert np.max(np.abs(amp - want)) / want < 1e-5, (amp, want)
+
+
+def _uniform_prim(case, alphas, rho_i, u, v, p):
+    nf, D = case.num_fluids, case.num_dims
+    prim = np.zeros(case.ghost_shape)
+    for i in range(nf):
+        prim[i] = alphas[i] * rho_i[i]
+        prim[nf + D + 1 + i] = alphas[i]
+    prim[nf] = u
+    if D > 1:
+        prim[nf + 1] = v
+    prim[nf + D] = p
+    return prim
+
+
+def test_stability_criteria_closed_form():
+    """ICFL, VCFL and Rc of a uniform state (m_data_output.fpp:214-263) with
+    the harmonic mixture Reynolds numbers of m_variables_conversion.fpp:
+    210-223, on a grid with dx != dy: ICFL = dt / min(dx/(|u|+c), dy/(|v|+c)),
+    VCFL = max_i(dt/Re_i) / min(dx, dy)^2, Rc = min(dx(|u|+c), dy(|v|+c)) /
+    max_i(1/Re_i), Re_i = 1/sum_f(alpha_f/Re_f,i) (1e16 for a kind no fluid
+    has)."""
+    gam = [2.5, 1.0 / 3.4]
+    pinf = [0.0, 4.4 * 6e8 / 3.4 / 1e4]
+    case = O.Case(m=31, n=15, num_fluids=2, gammas=gam, pi_infs=pinf, Re=[(100.0, 50.0), (200.0, -1.0)])
+    grid = O.uniform_grid(case)
+    dx, dy = 1.0 / 32, 1.0 / 16
+    al, rr, u, v, p = (0.3, 0.7), (1.2, 1000.0), 3.0, -2.0, 4e4
+    prim = _uniform_prim(case, al, rr, u, v, p)
+    dt = 1e-6
+    icfl, vcfl, rc = O.stability(case, grid, prim, dt)
+    rho = sum(a * r for a, r in zip(al, rr))
+    G = sum(a * g for a, g in zip(al, gam))
+    P = sum(a * q for a, q in zip(al, pinf))
+    c = np.sqrt(((G + 1) * p + P) / (G * rho))
+    re1 = 1.0 / (al[0] / 100.0 + al[1] / 200.0)
+    re2 = 1.0 / (al[0] / 50.0)
+    assert icfl == pytest.approx(dt / min(dx / (abs(u) + c), dy / (abs(v) + c)), rel=1e-14)
+    assert vcfl == pytest.approx(max(dt / re1, dt / re2) / min(dx, dy) ** 2, rel=1e-14)
+    assert rc == pytest.approx(min(dx * (abs(u) + c), dy * (abs(v) + c)) / max(1 / re1, 1 / re2), rel=1e-14)
+    # one fluid, shear only: the bulk Reynolds number is 1/sgm_eps
+    one = O.Case(m=31, n=0, num_fluids=1, gammas=[2.5], pi_infs=[0.0], Re=[(100.0, -1.0)])
+    prim1 = _uniform_prim(one, (1.0,), (1.5,), 3.0, 0.0, 4.0)
+    icfl, vcfl, rc = O.stability(one, O.uniform_grid(one), prim1, dt)
+    c1 = np.sqrt(1.4 * 4.0 / 1.5)
+    assert vcfl == pytest.approx(max(dt / 100.0, dt / 1e16) / dx ** 2, rel=1e-14)
+    assert rc == pytest.approx(dx * (3.0 + c1) * 100.0, rel=1e-14)
+    assert icfl == pytest.approx(dt / dx * (3.0 + c1), rel=1e-14)
+    # a NaN pressure makes the sound speed, ICFL and Rc of its cell NaN,
+    # and the extrema NaN; an inviscid case reports zeros
+    prim[4, 0, 7, 9] = np.nan
+    icfl, vcfl, rc = O.stability(case, grid, prim, dt)
+    assert np.isnan(icfl) and np.isnan(rc)
+    inv = O.Case(m=31, n=15, num_fluids=2, gammas=gam, pi_infs=pinf)
+    assert O.stability(inv, O.uniform_grid(inv), _uniform_prim(inv, al, rr, u, v, p), dt)[1:] == (0.0, 0.0)
