@@ -2,11 +2,12 @@
 !! namelist variables (src/simulation/m_start_up.fpp:73-83, defaults as in
 !! src/simulation/m_global_parameters.fpp:229-280) plus the 3D extension
 !! (p, bc_z) and the choice of bulk arithmetic (bulk_precision), the derived
-!! sizes and the device-resident field type.
+!! sizes, and the device allocation of the reference's field types.
 module m_global_parameters
 
     use iso_c_binding
     use m_mfc_abi
+    use m_derived_types
 
     implicit none
 
@@ -14,33 +15,7 @@ module m_global_parameters
     integer, parameter :: num_fluids_max = 10, num_probes_max = 10
     real(kind(0d0)), parameter :: dflt_real = -1d6
     integer, parameter :: dflt_int = -100
-
-    !> Derived types of the namelist (names and components as in
-    !! src/common/m_derived_types.f90:40-43, 107-117).
-    type int_bounds_info
-        integer :: beg
-        integer :: end
-    end type int_bounds_info
-
-    type physical_parameters
-        real(kind(0d0)) :: gamma
-        real(kind(0d0)) :: pi_inf
-        real(kind(0d0)), dimension(2) :: Re
-    end type physical_parameters
-
-    type probe_parameters
-        real(kind(0d0)) :: x
-        real(kind(0d0)) :: y
-    end type probe_parameters
-
-    !> A ghosted field on the device: one allocation of sys_size variables,
-    !! each (m+2b+1) x (n+2bj+1) x (p+2bk+1) doubles with x fastest, and the
-    !! per-variable device pointers handed to the C ABI.
-    type device_field
-        type(c_ptr) :: base = c_null_ptr
-        type(c_ptr), allocatable, dimension(:) :: v
-        integer(c_size_t) :: nvar_elems = 0
-    end type device_field
+    real(kind(0d0)), parameter :: sgm_eps = 1d-16
 
     ! namelist variables
     character(len=path_len) :: case_dir
@@ -100,7 +75,12 @@ contains
         E_idx = num_fluids + num_dims + 1
         adv_beg = E_idx + 1
         sys_size = E_idx + num_fluids
-        buff_size = (weno_order - 1)/2 + 2
+        ! m_global_parameters.fpp:355-359
+        if (any(fluid_pp(1:num_fluids)%Re(1) > 0d0) .or. any(fluid_pp(1:num_fluids)%Re(2) > 0d0)) then
+            buff_size = 2*((weno_order - 1)/2) + 2
+        else
+            buff_size = (weno_order - 1)/2 + 2
+        end if
         bj = 0; bk = 0
         if (n > 0) bj = buff_size
         if (p > 0) bk = buff_size
@@ -141,25 +121,81 @@ contains
         end if
     end function f_field_elems
 
-    subroutine s_allocate_device_field(f, ghosted)
-        type(device_field), intent(inout) :: f
+    !> Bounds of one variable of a ghosted (or interior) field as the
+    !! reference allocates it (m_time_steppers.fpp:67-112): (-b:m+b, -b:n+b)
+    !! and (0:m, 0:n); a 3D field folds its z extent into the second index.
+    subroutine s_field_bounds(ghosted, lb, ub)
         logical, intent(in) :: ghosted
-        integer :: i
-        integer(c_intptr_t) :: addr
-        f%nvar_elems = f_field_elems(ghosted)
-        call s_mfc_check(mfc_device_alloc(8_c_size_t*f%nvar_elems*int(sys_size, c_size_t), f%base), 'mfc_device_alloc')
-        allocate (f%v(sys_size))
-        addr = transfer(f%base, addr)
-        do i = 1, sys_size
-            f%v(i) = transfer(addr + int(8, c_intptr_t)*int(f%nvar_elems, c_intptr_t)*int(i - 1, c_intptr_t), f%v(i))
-        end do
-    end subroutine s_allocate_device_field
+        integer, intent(out) :: lb(2), ub(2)
+        if (ghosted) then
+            lb = [-buff_size, -bj]
+            ub = [m + buff_size, -bj + (n + 2*bj + 1)*(p + 2*bk + 1) - 1]
+        else
+            lb = [0, 0]
+            ub = [m, (n + 1)*(p + 1) - 1]
+        end if
+    end subroutine s_field_bounds
 
-    subroutine s_free_device_field(f)
-        type(device_field), intent(inout) :: f
-        if (c_associated(f%base)) call s_mfc_check(mfc_device_free(f%base), 'mfc_device_free')
-        f%base = c_null_ptr
-        if (allocated(f%v)) deallocate (f%v)
-    end subroutine s_free_device_field
+    !> Allocates the size(f) variables of a field in one device block
+    !! (returned in base) and points f(i)%sf at variable i, with the
+    !! reference's bounds.  Only the library touches the memory.
+    subroutine s_allocate_field(f, ghosted, base)
+        type(scalar_field), dimension(:), intent(inout) :: f
+        logical, intent(in) :: ghosted
+        type(c_ptr), intent(out) :: base
+        real(kind(0d0)), pointer :: tmp(:, :)
+        integer :: lb(2), ub(2), i
+        integer(c_intptr_t) :: addr
+        integer(c_size_t) :: ne
+        type(c_ptr) :: v
+        call s_field_bounds(ghosted, lb, ub)
+        ne = f_field_elems(ghosted)
+        call s_mfc_check(mfc_device_alloc(8_c_size_t*ne*int(size(f), c_size_t), base), 'mfc_device_alloc')
+        addr = transfer(base, addr)
+        do i = 1, size(f)
+            v = transfer(addr + 8_c_intptr_t*int(ne, c_intptr_t)*int(i - 1, c_intptr_t), v)
+            call c_f_pointer(v, tmp, [ub(1) - lb(1) + 1, ub(2) - lb(2) + 1])
+            f(i)%sf(lb(1):, lb(2):) => tmp
+        end do
+    end subroutine s_allocate_field
+
+    subroutine s_free_field(f, base)
+        type(scalar_field), dimension(:), intent(inout) :: f
+        type(c_ptr), intent(inout) :: base
+        integer :: i
+        do i = 1, size(f)
+            nullify (f(i)%sf)
+        end do
+        if (c_associated(base)) call s_mfc_check(mfc_device_free(base), 'mfc_device_free')
+        base = c_null_ptr
+    end subroutine s_free_field
+
+    !> The device pointers of a field's variables, c_loc of each variable's
+    !! first element: the layout the C ABI takes (include/mfc_rhs.h).
+    subroutine s_field_pointers(f, ptrs)
+        type(scalar_field), dimension(:), intent(in) :: f
+        type(c_ptr), dimension(:), intent(out) :: ptrs
+        integer :: i
+        do i = 1, size(f)
+            ptrs(i) = c_loc(f(i)%sf(lbound(f(i)%sf, 1), lbound(f(i)%sf, 2)))
+        end do
+    end subroutine s_field_pointers
+
+    !> Host copy of one variable: all of sf, ghosts included, into the
+    !! first size(v%sf) elements of host (x fastest).
+    subroutine s_variable_to_host(v, host)
+        type(scalar_field), intent(in) :: v
+        real(kind(0d0)), intent(out), target :: host(*)
+        call s_mfc_check(mfc_memcpy_d2h(c_loc(host(1)), c_loc(v%sf(lbound(v%sf, 1), lbound(v%sf, 2))), &
+                                        8_c_size_t*int(size(v%sf), c_size_t)), 'mfc_memcpy_d2h')
+    end subroutine s_variable_to_host
+
+    !> Device copy of one variable from the first size(v%sf) elements of host.
+    subroutine s_variable_to_device(host, v)
+        real(kind(0d0)), intent(in), target :: host(*)
+        type(scalar_field), intent(inout) :: v
+        call s_mfc_check(mfc_memcpy_h2d(c_loc(v%sf(lbound(v%sf, 1), lbound(v%sf, 2))), c_loc(host(1)), &
+                                        8_c_size_t*int(size(v%sf), c_size_t)), 'mfc_memcpy_h2d')
+    end subroutine s_variable_to_device
 
 end module m_global_parameters

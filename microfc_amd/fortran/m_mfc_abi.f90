@@ -10,7 +10,8 @@ module m_mfc_abi
     implicit none
 
     integer, parameter :: mfc_num_fluids_max = 10
-    integer, parameter :: mfc_prec_fp64 = 0, mfc_prec_fp32_bulk = 1, mfc_prec_fp32_learned = 2
+    integer, parameter :: mfc_prec_fp64 = 0, mfc_prec_fp32_bulk = 1, mfc_prec_fp32_learned = 2, &
+                          mfc_prec_fp64_fast = 3
     integer, parameter :: mfc_mlp_nparam = 2128
 
     type, bind(C) :: mfc_params
@@ -64,6 +65,13 @@ module m_mfc_abi
             real(c_double), value :: dt
             real(c_double), intent(out) :: icfl
         end function mfc_icfl_max
+
+        integer(c_int) function mfc_stability_criteria(q_prim, dt, crit) bind(C, name='mfc_stability_criteria')
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: q_prim
+            real(c_double), value :: dt
+            real(c_double), intent(out) :: crit(3)
+        end function mfc_stability_criteria
 
         integer(c_int) function mfc_cfl_dt(q_cons, cfl, dt) bind(C, name='mfc_cfl_dt')
             import :: c_int, c_double, c_ptr

@@ -7,6 +7,7 @@ module m_start_up
 
     use iso_c_binding
     use m_mfc_abi
+    use m_derived_types
     use m_global_parameters
 
     implicit none
@@ -53,10 +54,9 @@ contains
         if (probe_wrt) call s_abort('probe_wrt is not supported by this build. Exiting ...')
         if (parallel_io) call s_abort('parallel_io is not supported by this build. Exiting ...')
         if (cu_mpi) call s_abort('Unsupported value of cu_mpi. Exiting ...')
-        ! Viscous terms run with finite-difference velocity gradients.  The
-        ! reference's weno_Re_flux branch takes cell gradients at cells -3 and
-        ! m+3 (m_viscous.fpp:411-423) from face values it reconstructs only on
-        ! [-2, m+2] (:475-479), so its boundary faces read unset memory.
+        ! Viscous terms run with finite-difference velocity gradients; the
+        ! WENO-reconstructed gradients of weno_Re_flux (m_viscous.fpp:380-600)
+        ! are not built.
         if (weno_Re_flux .and. any(fluid_pp(1:num_fluids)%Re(1) > 0d0 .or. fluid_pp(1:num_fluids)%Re(2) > 0d0)) &
             call s_abort('weno_Re_flux is not supported by this build (use finite-difference gradients). Exiting ...')
         if (n == 0 .and. bc_y%beg == dflt_int) then
@@ -90,13 +90,14 @@ contains
         read (2) a; close (2)
     end subroutine s_read_record
 
-    !> s_read_serial_data_files (m_start_up.fpp:298-390): cell boundaries and
-    !! interior conservative variables into q (host, ghosted), cell widths
-    !! from the boundaries.
-    subroutine s_read_serial_data_files(q)
-        real(kind(0d0)), intent(inout) :: q(-buff_size:, -bj:, -bk:, :)
+    !> s_read_serial_data_files(q_cons_vf) (m_start_up.fpp:298-390): the cell
+    !! boundaries (cell widths from them) and the interior of every
+    !! conservative variable, copied to the device (ghost cells zero until the
+    !! first ghost fill).
+    subroutine s_read_serial_data_files(q_cons_vf)
+        type(scalar_field), dimension(sys_size), intent(INOUT) :: q_cons_vf
         character(len=path_len + 3*name_len) :: fp
-        real(kind(0d0)), allocatable :: tmp(:, :, :)
+        real(kind(0d0)), allocatable :: tmp(:, :, :), h(:, :, :)
         integer :: i
         call s_read_record(trim(f_t_step_dir(t_step_start))//'/x_cb.dat', x_cb(-1:m), m + 2)
         dx(0:m) = x_cb(0:m) - x_cb(-1:m - 1)
@@ -108,14 +109,15 @@ contains
             call s_read_record(trim(f_t_step_dir(t_step_start))//'/z_cb.dat', z_cb(-1:p), p + 2)
             dz(0:p) = z_cb(0:p) - z_cb(-1:p - 1)
         end if
-        allocate (tmp(0:m, 0:n, 0:p))
-        q = 0d0
+        allocate (tmp(0:m, 0:n, 0:p), h(-buff_size:m + buff_size, -bj:n + bj, -bk:p + bk))
+        h = 0d0
         do i = 1, sys_size
             write (fp, '(A,I0,A)') trim(f_t_step_dir(t_step_start))//'/q_cons_vf', i, '.dat'
             call s_read_record(fp, tmp, (m + 1)*(n + 1)*(p + 1))
-            q(0:m, 0:n, 0:p, i) = tmp
+            h(0:m, 0:n, 0:p) = tmp
+            call s_variable_to_device(h, q_cons_vf(i))
         end do
-        deallocate (tmp)
+        deallocate (tmp, h)
     end subroutine s_read_serial_data_files
 
     !> s_populate_grid_variables_buffers (m_start_up.fpp:517-655) for one

@@ -85,4 +85,4 @@ def test_fortran_driver_passes_reynolds_numbers(tmp_path):
     _write_case(tmp_path, **{"fluid_pp(1)%Re(1)": 100.0, "bulk_precision": 1})
     r = subprocess.run([SIM], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
-    assert "mfc_rhs_init" in r.stdout and "viscous terms run in the FP64 form only" in r.stdout
+    assert "mfc_rhs_init" in r.stdout and "viscous terms run in the plain FP64 form only" in r.stdout

@@ -11,7 +11,7 @@ from microfc_amd import case as C
 from microfc_amd import ic
 from oracle import oracle as O
 
-from helpers import assert_parity, perturbed, say, to_oracle
+from helpers import assert_parity, perturbed, reghost, say, to_oracle
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -56,12 +56,91 @@ def test_fortran_driver_bubble_with_run_time_info(gpu, tmp_path):
     _run(tmp_path, prm, grids, q, dt, nsteps, 6, run_time_info=True)
     got = C.read_serial(str(tmp_path), 12, prm)
     assert_parity(got, _oracle_steps(prm, grids, q, dt, nsteps, 12), what="bubble step 12")
-    lines = [ln.split() for ln in open(tmp_path / "run_time.inf").read().splitlines()[1:]]
-    assert len(lines) == nsteps + 1
+    text = open(tmp_path / "run_time.inf").read()
+    header, rows, footer = _run_time_table(text)
+    assert header[0].startswith("Description: Stability information at each time-step")
+    assert header[5].startswith("Date: ")
+    assert len(rows) == nsteps + 1 and [int(r[0]) for r in rows] == list(range(nsteps + 1))
     oc = to_oracle(prm)
     prim = O.cons_to_prim(oc, O.bc_fill(oc, q.copy()))
     icfl0 = O.icfl_max(oc, grids, prim, dt)
-    assert abs(float(lines[0][2]) - icfl0) < 1e-6
+    assert abs(float(rows[0][2]) - icfl0) < 1e-6
+    assert footer[0].startswith("ICFL Max: ") and abs(float(footer[0].split()[2]) - max(float(r[2]) for r in rows)) < 1e-6
+    assert any(ln.startswith("Run-time: ") for ln in footer)
+
+
+def _run_time_table(text):
+    """(header lines, table rows split into fields, footer lines) of a
+    run_time.inf written by s_open/s_write/s_close_run_time_information_file."""
+    lines = text.splitlines()
+    t = next(i for i, ln in enumerate(lines) if ln.startswith("===") and "Time-steps" in ln)
+    f = next(i for i, ln in enumerate(lines) if ln.startswith("-----"))
+    return lines[:t + 1], [ln.split() for ln in lines[t + 1:f]], [ln for ln in lines[f + 1:] if ln.strip()]
+
+
+def test_fortran_compute_rhs_dropin(gpu, tmp_path):
+    """s_compute_rhs(q_cons_vf, q_prim_vf, rhs_vf, t_step) called from Fortran
+    with the reference's type(scalar_field), dimension(sys_size) arguments:
+    rhs and q_prim_vf equal the oracle's bit for bit (2D and 1D)."""
+    for name, (prm, grids, q) in [("bubble2d_48", ic.shock_bubble_2d(N=48)), ("sod1d_100", ic.sod_1d(m=99))]:
+        q = perturbed(prm, q)
+        d = tmp_path / name
+        d.mkdir()
+        case = C.case_from_params(prm, dt=1e-7, t_step_stop=5, t_step_save=5, run_time_info=True)
+        C.write_inp(case, str(d / "simulation.inp"))
+        C.write_serial(str(d), 0, prm, grids, q)
+        say(f"[fortran s_compute_rhs] {name}")
+        r = subprocess.run([os.path.join(ROOT, "microfc_amd", "fortran", "compute_rhs_check")], cwd=str(d),
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        oc = to_oracle(prm)
+        rhs_ref, prim_ref = O.compute_rhs(oc, grids, q, t_step=0, want_prim=True)
+        got = np.fromfile(d / "rhs.dat", dtype="<f8").reshape(prm.interior_shape)
+        assert_parity(got, rhs_ref, tol=0.0, what=f"{name} s_compute_rhs rhs_vf")
+        gp = np.fromfile(d / "q_prim.dat", dtype="<f8").reshape(prm.ghost_shape)
+        assert_parity(gp, prim_ref, tol=0.0, what=f"{name} s_compute_rhs q_prim_vf")
+
+
+def test_fortran_driver_nan_guard(gpu, tmp_path):
+    """A NaN in the initial condition stops the run with "Error" at the first
+    save (p_main.fpp:291-302), after printing the cell's indices."""
+    prm, grids, q = ic.sod_1d(m=99)
+    b = prm.buff_size
+    q[0, 0, 0, b + 37] = np.nan
+    case = C.case_from_params(prm, dt=1e-4, t_step_stop=4, t_step_save=2)
+    C.write_inp(case, str(tmp_path / "simulation.inp"))
+    C.write_serial(str(tmp_path), 0, prm, grids, q)
+    r = subprocess.run([SIM], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    # STOP "Error" ends the program with the message (its exit status is 0
+    # under amdflang, as for the reference built the same way)
+    assert "STOP: Error" in r.stderr, r.stderr[-500:]
+    assert not (tmp_path / "p_all" / "p0" / "2").exists()
+    assert not (tmp_path / "time_data.dat").exists()
+    # the guard's line: j, k, rank, t_step, m, n of the first NaN
+    loc = [ln.split() for ln in r.stdout.splitlines() if len(ln.split()) == 6 and "Time" not in ln]
+    assert loc and loc[-1][1:] == ["0", "0", "2", "99", "0"], r.stdout[-500:]
+    assert 0 <= int(loc[-1][0]) <= 37
+
+
+def test_fortran_driver_viscous_run_time_information(gpu, tmp_path):
+    """Viscous runs write the ICFL, VCFL and Rc columns and their extrema
+    (m_data_output.fpp:136-141, 280-294, 693-697), the values equal to the
+    oracle's criteria of the first stage's primitive field to the printed
+    digits."""
+    prm, grids, q = ic.viscous_shear_2d(N=51)
+    q = perturbed(prm, q)
+    dt, nsteps = 5e-10, 4
+    _run(tmp_path, prm, grids, q, dt, nsteps, 4, run_time_info=True)
+    header, rows, footer = _run_time_table(open(tmp_path / "run_time.inf").read())
+    assert header[-1].startswith("==== Time-steps ====== Time ======= ICFL Max ==== VCFL Max ====== Rc Min")
+    assert len(rows) == nsteps + 1 and all(len(r) == 5 for r in rows)
+    oc = to_oracle(prm)
+    prim = O.cons_to_prim(oc, O.bc_fill(oc, q.copy()))
+    icfl, vcfl, rc = O.stability(oc, grids, prim, dt)
+    assert abs(float(rows[0][2]) - icfl) <= 1e-6 and abs(float(rows[0][3]) - vcfl) <= 1e-6
+    assert abs(float(rows[0][4]) - rc) <= 1e-6 * max(1.0, abs(rc))
+    names = [ln.split(":")[0] for ln in footer]
+    assert names[:3] == ["ICFL Max", "VCFL Max", "Rc Min"]
 
 
 def test_fortran_driver_reference_viscous_example(gpu, tmp_path):
@@ -77,7 +156,9 @@ def test_fortran_driver_reference_viscous_example(gpu, tmp_path):
         got = C.read_serial(str(tmp_path), t, prm)
         assert_parity(got, _oracle_steps(prm, grids, q, dt, nsteps, t), what=f"viscous example step {t}")
     # the viscous terms moved the state: the inviscid oracle differs
-    inv = _oracle_steps(prm.__class__(**{**prm.__dict__, "Re": None}), grids, q, dt, nsteps, 10)
+    prm_inv = prm.__class__(**{**prm.__dict__, "Re": None})
+    gi, qi = reghost(prm, prm_inv, grids, q)
+    inv = _oracle_steps(prm_inv, gi, qi, dt, nsteps, 10)
     ref = _oracle_steps(prm, grids, q, dt, nsteps, 10)
     assert np.max(np.abs(inv[2] - ref[2])) > 1e-6 * np.max(np.abs(ref[2]))
 
