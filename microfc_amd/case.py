@@ -10,6 +10,14 @@
   the format src/pre_process/m_data_output.f90 writes and
   src/simulation/m_start_up.fpp:298-390 reads.  q_cons_vf<i> holds the
   interior sf(0:m, 0:n, 0:p) of variable i, x fastest.
+* ``write_parallel`` / ``read_parallel`` handle the restart files of
+  parallel_io = T: ``<case_dir>/restart_data/lustre_{x_cb,y_cb,z_cb}.dat``
+  (the global cell boundaries, raw little-endian doubles) and
+  ``lustre_<t_step>.dat``, the interior of every variable as raw doubles,
+  variable i at byte offset (m+1)(n+1)(p+1)*8*(i-1), x fastest -- what
+  src/simulation/m_data_output.fpp:477-540 writes and
+  src/simulation/m_start_up.fpp:393-511 reads through MPI-IO (with p+1 = 1 in
+  the reference's 2D).
 """
 from __future__ import annotations
 
@@ -56,8 +64,9 @@ def write_inp(case: dict, path: str) -> str:
 
 
 def case_from_params(prm: Params, dt: float, t_step_stop: int, t_step_save: int, case_dir: str = "'.'",
-                     run_time_info: bool = False, t_step_start: int = 0) -> dict:
-    c = {"case_dir": case_dir, "run_time_info": run_time_info, "m": prm.m, "n": prm.n, "p": prm.p,
+                     run_time_info: bool = False, t_step_start: int = 0, parallel_io: bool = False) -> dict:
+    c = {"case_dir": case_dir, "run_time_info": run_time_info, "parallel_io": parallel_io,
+         "m": prm.m, "n": prm.n, "p": prm.p,
          "dt": dt, "t_step_start": t_step_start, "t_step_stop": t_step_stop, "t_step_save": t_step_save,
          "num_fluids": prm.num_fluids, "time_stepper": prm.time_stepper, "weno_order": prm.weno_order,
          "weno_eps": prm.weno_eps, "bulk_precision": prm.precision}
@@ -114,6 +123,44 @@ def read_serial(case_dir: str, t_step: int, prm: Params) -> np.ndarray:
     out = np.zeros(prm.interior_shape)
     for v in range(prm.sys_size):
         out[v] = read_record(os.path.join(d, f"q_cons_vf{v + 1}.dat")).reshape(prm.interior_shape[1:])
+    return out
+
+
+def restart_dir(case_dir: str) -> str:
+    return os.path.join(case_dir, "restart_data")
+
+
+def write_parallel_grid(case_dir: str, prm: Params, grids) -> None:
+    """lustre_{x,y,z}_cb.dat: s_cb(-1:s) of each direction, raw doubles."""
+    d = restart_dir(case_dir)
+    os.makedirs(d, exist_ok=True)
+    b = prm.buff_size
+    for k, nm in enumerate("xyz"[: prm.num_dims]):
+        s = prm.extents()[k]
+        np.ascontiguousarray(grids[k][0][b: b + s + 2], dtype="<f8").tofile(os.path.join(d, f"lustre_{nm}_cb.dat"))
+
+
+def write_parallel(case_dir: str, t_step: int, prm: Params, q_ghosted) -> None:
+    """lustre_<t_step>.dat: the interior of every variable, one block each."""
+    d = restart_dir(case_dir)
+    os.makedirs(d, exist_ok=True)
+    np.ascontiguousarray(prm.interior(q_ghosted), dtype="<f8").tofile(os.path.join(d, f"lustre_{t_step}.dat"))
+
+
+def read_parallel(case_dir: str, t_step: int, prm: Params) -> np.ndarray:
+    """Interior conservative field of shape (S, p+1, n+1, m+1) from lustre_<t_step>.dat."""
+    path = os.path.join(restart_dir(case_dir), f"lustre_{t_step}.dat")
+    raw = np.fromfile(path, dtype="<f8")
+    if raw.size != int(np.prod(prm.interior_shape)):
+        raise ValueError(f"{path}: {raw.size} doubles, expected {int(np.prod(prm.interior_shape))}")
+    return raw.reshape(prm.interior_shape).copy()
+
+
+def read_parallel_grid(case_dir: str, prm: Params):
+    """s_cb(-1:s) of each direction from lustre_{x,y,z}_cb.dat."""
+    out = []
+    for k, nm in enumerate("xyz"[: prm.num_dims]):
+        out.append(np.fromfile(os.path.join(restart_dir(case_dir), f"lustre_{nm}_cb.dat"), dtype="<f8"))
     return out
 
 

@@ -30,7 +30,7 @@ program compute_rhs_check
     call s_allocate_field(q_cons_vf, .true., cons_base)
     call s_allocate_field(q_prim_vf, .true., prim_base)
     call s_allocate_field(rhs_vf, .false., rhs_base)
-    call s_read_serial_data_files(q_cons_vf)
+    call s_read_data_files(q_cons_vf)
     call s_populate_grid_variables_buffers()
     call s_initialize_weno_module()
 

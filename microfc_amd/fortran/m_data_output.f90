@@ -2,7 +2,8 @@
 !! run_time.inf with the reference's header, one line of stability criteria
 !! per step (ICFL, and VCFL and Rc in viscous runs) and the closing summary;
 !! the abort when ICFL is NaN or above 1 (:296-305); the serial writer of
-!! <case_dir>/p_all/p0/<t_step>/ (:316-381) and time_data.dat.
+!! <case_dir>/p_all/p0/<t_step>/ (:316-381), the parallel_io writer of
+!! restart_data/lustre_<t_step>.dat (:477-540) and time_data.dat.
 module m_data_output
 
     use iso_c_binding
@@ -14,8 +15,8 @@ module m_data_output
     implicit none
 
     private; public :: s_initialize_data_output_module, s_open_run_time_information_file, &
-        s_write_run_time_information, s_write_serial_data_files, s_close_run_time_information_file, &
-        s_write_time_data, f_viscous
+        s_write_run_time_information, s_write_serial_data_files, s_write_parallel_data_files, &
+        s_write_data_files, s_close_run_time_information_file, s_write_time_data, f_viscous
 
     !> Stability criteria extrema over all the time steps (:74-76).
     real(kind(0d0)) :: icfl_max, vcfl_max, Rc_min
@@ -150,6 +151,39 @@ contains
         end do
         deallocate (h)
     end subroutine s_write_serial_data_files
+
+    !> s_write_parallel_data_files(q_cons_vf, t_step) (:477-540) on one rank:
+    !! restart_data/lustre_<t_step>.dat with the interior of every variable as
+    !! raw doubles, variable i at byte offset (m+1)(n+1)(p+1)*8*(i-1).
+    subroutine s_write_parallel_data_files(q_cons_vf, t_step)
+        type(scalar_field), dimension(sys_size), intent(IN) :: q_cons_vf
+        integer, intent(IN) :: t_step
+        character(len=path_len + 2*name_len) :: fp
+        real(kind(0d0)), allocatable :: h(:, :, :)
+        integer :: i
+        call execute_command_line('mkdir -p '//trim(case_dir)//'/restart_data')
+        write (fp, '(A,I0,A)') trim(case_dir)//'/restart_data/lustre_', t_step, '.dat'
+        open (2, FILE=trim(fp), FORM='unformatted', ACCESS='stream', STATUS='replace')
+        allocate (h(-buff_size:m + buff_size, -bj:n + bj, -bk:p + bk))
+        do i = 1, sys_size
+            call s_variable_to_host(q_cons_vf(i), h)
+            write (2) h(0:m, 0:n, 0:p)
+        end do
+        close (2)
+        deallocate (h)
+    end subroutine s_write_parallel_data_files
+
+    !> The writer parallel_io selects (the reference's s_write_data_files
+    !! procedure pointer).
+    subroutine s_write_data_files(q_cons_vf, t_step)
+        type(scalar_field), dimension(sys_size), intent(IN) :: q_cons_vf
+        integer, intent(IN) :: t_step
+        if (parallel_io) then
+            call s_write_parallel_data_files(q_cons_vf, t_step)
+        else
+            call s_write_serial_data_files(q_cons_vf, t_step)
+        end if
+    end subroutine s_write_data_files
 
     !> time_data.dat as p_main.fpp:255-270 appends it.
     subroutine s_write_time_data(time_final)

@@ -52,7 +52,6 @@ contains
             call s_abort('Unsupported combination of values of t_step_start, t_step_stop and t_step_save. Exiting ...')
         if (num_fluids < 1 .or. num_fluids > num_fluids_max) call s_abort('Unsupported value of num_fluids. Exiting ...')
         if (probe_wrt) call s_abort('probe_wrt is not supported by this build. Exiting ...')
-        if (parallel_io) call s_abort('parallel_io is not supported by this build. Exiting ...')
         if (cu_mpi) call s_abort('Unsupported value of cu_mpi. Exiting ...')
         ! Viscous terms run with finite-difference velocity gradients; the
         ! WENO-reconstructed gradients of weno_Re_flux (m_viscous.fpp:380-600)
@@ -119,6 +118,66 @@ contains
         end do
         deallocate (tmp, h)
     end subroutine s_read_serial_data_files
+
+    !> s_read_parallel_data_files(q_cons_vf) (m_start_up.fpp:393-511) on one
+    !! rank: restart_data/lustre_{x,y,z}_cb.dat hold the cell boundaries and
+    !! lustre_<t_step_start>.dat the interior of every variable, raw doubles,
+    !! variable i at byte offset (m+1)(n+1)(p+1)*8*(i-1) (what MPI-IO's file
+    !! views lay down for a single rank; the reference's 2D has p+1 = 1).
+    subroutine s_read_parallel_data_files(q_cons_vf)
+        type(scalar_field), dimension(sys_size), intent(INOUT) :: q_cons_vf
+        real(kind(0d0)), allocatable :: tmp(:, :, :), h(:, :, :)
+        character(len=path_len + 2*name_len) :: fp
+        integer(kind=8) :: nint
+        integer :: i
+        call s_read_raw(trim(case_dir)//'/restart_data/lustre_x_cb.dat', x_cb(-1:m), m + 2, 1_8)
+        dx(0:m) = x_cb(0:m) - x_cb(-1:m - 1)
+        if (n > 0) then
+            call s_read_raw(trim(case_dir)//'/restart_data/lustre_y_cb.dat', y_cb(-1:n), n + 2, 1_8)
+            dy(0:n) = y_cb(0:n) - y_cb(-1:n - 1)
+        end if
+        if (p > 0) then
+            call s_read_raw(trim(case_dir)//'/restart_data/lustre_z_cb.dat', z_cb(-1:p), p + 2, 1_8)
+            dz(0:p) = z_cb(0:p) - z_cb(-1:p - 1)
+        end if
+        nint = int(m + 1, 8)*int(n + 1, 8)*int(p + 1, 8)
+        write (fp, '(A,I0,A)') trim(case_dir)//'/restart_data/lustre_', t_step_start, '.dat'
+        allocate (tmp(0:m, 0:n, 0:p), h(-buff_size:m + buff_size, -bj:n + bj, -bk:p + bk))
+        h = 0d0
+        do i = 1, sys_size
+            call s_read_raw(fp, tmp, int(nint), 1_8 + 8_8*nint*int(i - 1, 8))
+            h(0:m, 0:n, 0:p) = tmp
+            call s_variable_to_device(h, q_cons_vf(i))
+        end do
+        deallocate (tmp, h)
+    end subroutine s_read_parallel_data_files
+
+    !> count raw doubles from byte position pos (1-based) of a stream file.
+    subroutine s_read_raw(path, a, count, pos)
+        character(len=*), intent(in) :: path
+        integer, intent(in) :: count
+        real(kind(0d0)), intent(out) :: a(count)
+        integer(kind=8), intent(in) :: pos
+        logical :: file_exist
+        integer :: ios
+        inquire (FILE=trim(path), EXIST=file_exist)
+        if (.not. file_exist) call s_abort('File '//trim(path)//' is missing. Exiting...')
+        open (2, FILE=trim(path), FORM='unformatted', ACCESS='stream', ACTION='read', STATUS='old')
+        read (2, POS=pos, iostat=ios) a
+        close (2)
+        if (ios /= 0) call s_abort('File '//trim(path)//' is too short. Exiting...')
+    end subroutine s_read_raw
+
+    !> The reader parallel_io selects (the reference's s_read_data_files
+    !! procedure pointer, m_start_up.fpp:1010-1016).
+    subroutine s_read_data_files(q_cons_vf)
+        type(scalar_field), dimension(sys_size), intent(INOUT) :: q_cons_vf
+        if (parallel_io) then
+            call s_read_parallel_data_files(q_cons_vf)
+        else
+            call s_read_serial_data_files(q_cons_vf)
+        end if
+    end subroutine s_read_data_files
 
     !> s_populate_grid_variables_buffers (m_start_up.fpp:517-655) for one
     !! direction with s cells, boundary codes bcb/bce.

@@ -28,7 +28,7 @@ program p_main
 
     call s_initialize_rhs_module()
     call s_initialize_time_steppers_module()
-    call s_read_serial_data_files(q_cons_ts(1)%vf)
+    call s_read_data_files(q_cons_ts(1)%vf)
     call s_populate_grid_variables_buffers()
     call s_initialize_weno_module()
 
@@ -63,7 +63,7 @@ program p_main
         end if
         if (mod(t_step - t_step_start, t_step_save) == 0 .or. t_step == t_step_stop) then
             call s_check_for_nan(t_step)
-            call s_write_serial_data_files(q_cons_ts(1)%vf, t_step)
+            call s_write_data_files(q_cons_ts(1)%vf, t_step)
         end if
     end do
 

@@ -138,14 +138,27 @@ def shock_bubble_2d(N: int = 2048, weno_order: int = 5, alpha_min: float = 1e-6,
     return prm, grids, ghosted(prm, prim_to_cons(prm, prim))
 
 
-def shock_droplet_3d(N: int = 512, weno_order: int = 5, alpha_min: float = 1e-6, precision: int = 0):
+def shock_droplet_3d(N: int = 512, weno_order: int = 5, alpha_min: float = 1e-6, precision: int = 0,
+                     window=None, shock_x: float = 0.2):
     """Water droplet (R = 0.1, centre of [0,1]^3) in air (rho 1.2, p 1e5) hit
-    by a Mach 2.4 air shock at x = 0.2 (Rankine-Hugoniot post-shock state)."""
+    by a Mach 2.4 air shock at x = shock_x (Rankine-Hugoniot post-shock state
+    of the air, imposed on every fluid behind the shock).
+    window = ((i0, j0, k0), (mx, my, mz)) makes only that box of the N^3 grid,
+    as sub_box would cut it out of the whole case (same cell boundaries,
+    extrapolation at the cut faces)."""
     gw, piw = WATER
     ga, pia = AIR
     prm = Params(m=N - 1, n=N - 1, p=N - 1, num_fluids=2, weno_order=weno_order, gammas=[gw, ga],
                  pi_infs=[piw, pia], precision=precision)
     grids = uniform_grids(prm)
+    if window is not None:
+        (lo, shape) = window
+        full = prm
+        prm = Params(m=shape[0] - 1, n=shape[1] - 1, p=shape[2] - 1, num_fluids=2, weno_order=weno_order,
+                     gammas=[gw, ga], pi_infs=[piw, pia], precision=precision)
+        b = full.buff_size
+        grids = [populate_grid_buffers(grids[d][0][lo[d] + b:lo[d] + b + shape[d] + 1], prm.buff_size,
+                                       prm.bc_beg[d], prm.bc_end[d]) for d in range(3)]
     X, Y, Z = mesh(prm, grids)
     r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2)
     eta = _smooth_indicator(r, 0.1, 1.0 / N)
@@ -156,7 +169,7 @@ def shock_droplet_3d(N: int = 512, weno_order: int = 5, alpha_min: float = 1e-6,
     p2 = p1 * (1.0 + 2.0 * g / (g + 1.0) * (M * M - 1.0))
     rho2 = rho1 * (g + 1.0) * M * M / ((g - 1.0) * M * M + 2.0)
     u2 = M * c1 * (1.0 - rho1 / rho2)
-    shocked = X < 0.2
+    shocked = X < shock_x
     prim = np.zeros(prm.interior_shape)
     prim[0] = a_wat * 1000.0
     prim[1] = a_air * np.where(shocked, rho2, rho1)
@@ -195,30 +208,40 @@ def shear_layer_2d(N: int = 4096, weno_order: int = 5, seed: int = 0, modes: int
     return prm, grids, ghosted(prm, prim_to_cons(prm, prim))
 
 
-def viscous_shear_2d(N: int = 2048, weno_order: int = 5, Re: float = 1e-4):
-    """The reference's viscous case (examples/2D_viscous/case.py) at N^2 cells:
-    [-0.5, 0.5]^2, two water-like stiffened gases (gamma 4.4, p_inf 6e8) with
-    shear and volume Reynolds numbers Re; the lower half holds both fluids at
-    alpha = 0.5, alpha_rho = 1000 each, u = +500; the upper half (patch 2,
-    centred at y = 0.25, length 0.5) only fluid 2 at alpha_rho = 1, u = -500;
-    p = 1e6 everywhere.  Periodic in x; extrapolation in y (the example's bc
-    -6 fills ghosts by extrapolation too).  The example's weno_Re_flux = T is
-    replaced by finite-difference gradients."""
+def viscous_shear_case(N: int = 2048, weno_order: int = 5, Re: float = 1e-4) -> dict:
+    """The reference's examples/2D_viscous/case.py as a case dictionary at
+    N^2 cells: [-0.5, 0.5]^2, two water-like stiffened gases (gamma 4.4,
+    p_inf 6e8) with shear and volume Reynolds numbers Re; patch 1 fills the
+    domain with both fluids at alpha = 0.5, alpha_rho = 1000 each, u = +500;
+    patch 2 (centred at y = 0.25, length 0.5, allowed to overwrite patch 1)
+    holds fluid 2 alone at alpha_rho = 1, u = -500; p = 1e6 everywhere.
+    Periodic in x; bc -6 in y (extrapolation in the ghost fill).  The
+    example's weno_Re_flux = T is left out: the viscous terms use
+    finite-difference gradients."""
     g, pi = eos_inputs(4.4, 6.0e8)
-    prm = Params(m=N - 1, n=N - 1, num_fluids=2, weno_order=weno_order, gammas=[g, g], pi_infs=[pi, pi],
-                 bc_beg=(-1, -6, -3), bc_end=(-1, -6, -3), Re=[(Re, Re), (Re, Re)])
-    grids = uniform_grids(prm, lo=(-0.5, -0.5, 0.0), hi=(0.5, 0.5, 1.0))
-    X, Y, _ = mesh(prm, grids)
-    top = (Y >= 0.0) & (Y <= 0.5)  # patch 2: |y - 0.25| <= 0.25
-    prim = np.zeros(prm.interior_shape)
-    prim[0] = np.where(top, 0.0, 1000.0)
-    prim[1] = np.where(top, 1.0, 1000.0)
-    prim[2] = np.where(top, -500.0, 500.0)
-    prim[3] = 0.0
-    prim[4] = 1.0e6
-    prim[5] = np.where(top, 0.0, 0.5)
-    prim[6] = np.where(top, 1.0, 0.5)
-    return prm, grids, ghosted(prm, prim_to_cons(prm, prim))
+    c = {"m": N - 1, "n": N - 1, "x_domain%beg": -0.5, "x_domain%end": 0.5, "y_domain%beg": -0.5,
+         "y_domain%end": 0.5, "num_fluids": 2, "num_patches": 2, "weno_order": weno_order,
+         "bc_x%beg": -1, "bc_x%end": -1, "bc_y%beg": -6, "bc_y%end": -6}
+    patches = [dict(geometry=3, x_centroid=0.0, y_centroid=0.0, length_x=1.0, length_y=1.0,
+                    **{"vel(1)": 500.0, "vel(2)": 0.0, "pres": 1e6, "alpha_rho(1)": 1000.0,
+                       "alpha_rho(2)": 1000.0, "alpha(1)": 0.5, "alpha(2)": 0.5}),
+               dict(geometry=3, x_centroid=0.0, y_centroid=0.25, length_x=1.0, length_y=0.5,
+                    **{"alter_patch(1)": "T", "vel(1)": -500.0, "vel(2)": 0.0, "pres": 1e6,
+                       "alpha_rho(1)": 0.0, "alpha_rho(2)": 1.0, "alpha(1)": 0.0, "alpha(2)": 1.0})]
+    for k, p in enumerate(patches, start=1):
+        for key, v in p.items():
+            c[f"patch_icpp({k})%{key}"] = v
+    for f in (1, 2):
+        c[f"fluid_pp({f})%gamma"], c[f"fluid_pp({f})%pi_inf"] = g, pi
+        c[f"fluid_pp({f})%Re(1)"] = c[f"fluid_pp({f})%Re(2)"] = Re
+    return c
+
+
+def viscous_shear_2d(N: int = 2048, weno_order: int = 5, Re: float = 1e-4):
+    """The reference's viscous example at N^2 cells (viscous_shear_case),
+    laid out by the pre-processor's patch rules."""
+    from .pre_process import PreProcess
+    return PreProcess(viscous_shear_case(N, weno_order, Re)).state()
 
 
 def viscous_dt(prm: Params, grids, q, safety: float = 0.2) -> float:
@@ -316,3 +339,41 @@ def bubble_cloud_3d(N: int = 768, nbubbles: int = 64, seed: int = 0, weno_order:
         prim[7] = a_air
         q[:, b + k0:b + k1, b:b + N, b:b + N] = prim_to_cons(prm, prim)
     return prm, grids, q
+
+
+def sub_box(prm: Params, grids, q, lo, shape):
+    """A window of a case as a case of its own: interior cells lo[d] ..
+    lo[d]+shape[d]-1 in each direction, the same cell boundaries (so the same
+    cell widths and WENO coefficients), the same fluids and form, and the
+    window's conservative values; a face the window cuts through the domain
+    gets an extrapolation boundary (-3), a face on the domain's boundary keeps
+    its boundary condition."""
+    import copy
+    D = prm.num_dims
+    sub = copy.deepcopy(prm)
+    ext = [prm.m, prm.n, prm.p]
+    bcb, bce = list(prm.bc_beg), list(prm.bc_end)
+    new_ext = [0, 0, 0]
+    new_grids = []
+    for d in range(D):
+        l0, s = lo[d], shape[d]
+        if l0 < 0 or s < 1 or l0 + s > ext[d] + 1:
+            raise ValueError(f"window {l0}+{s} outside 0..{ext[d]} in direction {d}")
+        new_ext[d] = s - 1
+        if l0 > 0 or bcb[d] == -1 and s < ext[d] + 1:
+            bcb[d] = -3
+        if l0 + s < ext[d] + 1 or bce[d] == -1 and s < ext[d] + 1:
+            bce[d] = -3
+        cb = grids[d][0]
+        b = prm.buff_size
+        cbi = cb[l0 + b:l0 + b + s + 1]  # s_cb(l0-1 .. l0+s-1)
+        new_grids.append(cbi)
+    sub.m, sub.n, sub.p = new_ext
+    sub.bc_beg, sub.bc_end = tuple(bcb), tuple(bce)
+    new_grids = [populate_grid_buffers(new_grids[d], sub.buff_size, bcb[d], bce[d]) for d in range(D)]
+    qi = prm.interior(q)
+    sl = [slice(None)] + [slice(None)] * 3
+    idx = [2, 1, 0]  # array axes of x, y, z
+    for d in range(D):
+        sl[1 + idx[d]] = slice(lo[d], lo[d] + shape[d])
+    return sub, new_grids, ghosted(sub, qi[tuple(sl)])

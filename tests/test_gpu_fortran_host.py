@@ -215,3 +215,43 @@ def test_fortran_driver_learned_needs_weights(gpu, tmp_path):
     C.write_serial(str(tmp_path), 0, prm, grids, q)
     r = subprocess.run([SIM], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "mlp_weights" in r.stdout
+
+
+def test_fortran_driver_parallel_io(gpu, tmp_path):
+    """parallel_io = T reads restart_data/lustre_{x_cb,y_cb,<t>}.dat and
+    writes lustre_<t>.dat (m_start_up.fpp:393-511, m_data_output.fpp:477-540):
+    the saved states equal the oracle's steps bit for bit, and the run from
+    a saved restart file continues exactly as the unbroken run."""
+    prm, grids, q = ic.shock_bubble_2d(N=48)
+    q = perturbed(prm, q)
+    dt, nsteps = 1e-7, 8
+    case = C.case_from_params(prm, dt=dt, t_step_stop=nsteps, t_step_save=4, parallel_io=True)
+    C.write_inp(case, str(tmp_path / "simulation.inp"))
+    C.write_parallel_grid(str(tmp_path), prm, grids)
+    C.write_parallel(str(tmp_path), 0, prm, q)
+    say("[fortran host] parallel_io")
+    r = subprocess.run([SIM], cwd=str(tmp_path), capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert not (tmp_path / "p_all").exists()
+    for t in (4, 8):
+        got = C.read_parallel(str(tmp_path), t, prm)
+        assert_parity(got, _oracle_steps(prm, grids, q, dt, nsteps, t), tol=0.0, what=f"parallel_io step {t}")
+    cb = C.read_parallel_grid(str(tmp_path), prm)
+    assert np.array_equal(cb[0], grids[0][0][prm.buff_size:prm.buff_size + prm.m + 2])
+    # restart from the step-4 file: the oracle continued from that state with
+    # the dt sequence a run starting at t_step 4 uses
+    r2 = tmp_path / "restart"
+    r2.mkdir()
+    case2 = dict(case, t_step_start=4)
+    C.write_inp(case2, str(r2 / "simulation.inp"))
+    C.write_parallel_grid(str(r2), prm, grids)
+    (r2 / "restart_data" / "lustre_4.dat").write_bytes((tmp_path / "restart_data" / "lustre_4.dat").read_bytes())
+    r = subprocess.run([SIM], cwd=str(r2), capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    oc = to_oracle(prm)
+    qo = q.copy()
+    prm.interior(qo)[...] = C.read_parallel(str(tmp_path), 4, prm)
+    q2 = np.zeros_like(qo)
+    for t, d in enumerate(C.dt_sequence(dt, 4, nsteps)):
+        O.tvd_rk_step(oc, grids, d, qo, q2, t_step=4 + t)
+    assert_parity(C.read_parallel(str(r2), 8, prm), prm.interior(qo), tol=0.0, what="parallel_io restart step 8")
