@@ -1,14 +1,20 @@
 """Throughput benchmark of the MI355X RHS + TVD-RK3 path (one GPU).
 
-Workload (default): the 2D shock-bubble, two fluids (water + air bubble),
-2048 x 2048 cells, WENO5 + HLLC + RK3, plain FP64, dt from the CFL 0.4
-reduction at t = 0 and then held fixed (the reference's dt is an input).
+Workload (default): the 3D shock-droplet, two fluids (a water droplet of
+radius 0.1 in air hit by a Mach 2.4 air shock), 512^3 cells, WENO5 + HLLC +
+TVD-RK3 in FP64 -- the contracted FP64 form (FMA contraction, reciprocal-
+multiply division; within 1e-12 relative of the FP64 oracle per step), with
+the plain, bit-exact FP64 form timed beside it.  dt comes from the CFL 0.4
+reduction at t = 0 and is then held fixed (the reference's dt is an input).
 A step is one full TVD-RK3 time step (three RHS evaluations with the stage
 updates) over the whole grid; inputs are resident in HBM before timing.
 
-Prints one JSON line with the throughput (cell-updates/s per RK3 step), the
-roofline of the dominant kernel (HIP events on the library's stream) and the
-CPU baseline (the FP64 oracle, one core, one RK3 step of the same grid).
+Prints one JSON line: throughput (cell-updates/s per RK3 step), the roofline
+of the dominant kernel on the compulsory FP64 state traffic (64*S bytes per
+cell per step, a third of it per stage and launch; launch times from HIP
+events on the library's stream), every kernel's figures, and the CPU
+baseline (the FP64 oracle on the host's cores, one RK3 step of a slab of the
+same case).
 """
 import argparse
 import json
@@ -30,6 +36,14 @@ FP64_PEAK_TFLOPS = 78.6
 # profiles/sweep_kernels_shock_bubble_2d_2048_fp64.md).  Other workloads
 # report null.
 PROFILED = {("shock_bubble_2d", 2048, "fp64", "sweep_y"): {"traffic": 678.5e6 + 236.6e6, "valu_busy": 0.79}}
+
+DEFAULT_PRECISION = "fp64_fast"
+FORMS = {"fp64": "plain FP64: the reference's evaluation order, no FMA contraction, IEEE division (bit-exact "
+                 "against the FP64 oracle)",
+         "fp64_fast": "contracted FP64: FMA contraction, reciprocal-multiply division, WENO weights normalised "
+                      "by one reciprocal (within 1e-12 relative of the FP64 oracle per step)",
+         "fp32_bulk": "FP32 WENO/HLLC, FP64 cons->prim, flux differences and RK stages",
+         "fp32_learned": "FP32 bulk with learned WENO5 weights (BF16 MFMA)"}
 
 
 def sweep_bytes_per_step(prm):
@@ -72,49 +86,16 @@ def sweep_bytes_per_step(prm):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--case", default="shock_bubble_2d",
-                    choices=["shock_bubble_2d", "shock_droplet_3d", "shear_layer_2d", "bubble_cloud_3d", "sod_1d",
-                             "viscous_shear_2d"])
-    ap.add_argument("--N", type=int, default=0, help="cells per direction (default: the case's size)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_bulk", "fp32_learned"])
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-aux", action="store_true",
-                    help="skip the auxiliary-kernel block (cons->prim, CFL reduction, ghost fill, RK1 update)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a 2 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the multi-GPU halo exchange is designed but "
-                                   "not built, so --gpus must be 1"}))
-        sys.exit(2)
-
-    import numpy as np
-    import torch
-
-    from microfc_amd import PREC_FP32_BULK, PREC_FP32_LEARNED, PREC_FP64, RhsModule, ic
-
-    prec = {"fp64": PREC_FP64, "fp32_bulk": PREC_FP32_BULK, "fp32_learned": PREC_FP32_LEARNED}[args.precision]
-    if prec == PREC_FP32_LEARNED:
-        # throughput does not depend on the weights; a network with a non-zero
-        # output layer exercises the softmax and the per-face fallback
-        from microfc_amd import learned
-        p = learned.init_params(0)
-        p[learned.OFF["W3"]:learned.OFF["W3"] + 512] = np.random.default_rng(1).normal(0, 0.1, 512)
-        learned.Mlp(p)
+def make_case(args, prec, ic):
+    """(params, grids, q, N, workload name) of the benchmark case."""
     if args.case == "shock_bubble_2d":
         N = args.N or 2048
         prm, grids, q = ic.shock_bubble_2d(N=N, precision=prec)
         workload = f"2D shock-bubble water/air {N}x{N}, WENO5+HLLC+RK3"
     elif args.case == "shock_droplet_3d":
-        N = args.N or 256
+        N = args.N or 512
         prm, grids, q = ic.shock_droplet_3d(N=N, precision=prec)
-        workload = f"3D shock-droplet water/air {N}^3, WENO5+HLLC+RK3"
+        workload = f"3D shock-droplet water/air {N}^3 (two fluids, Mach 2.4 air shock), WENO5+HLLC+RK3"
     elif args.case == "bubble_cloud_3d":
         N = args.N or 768
         prm, grids, q = ic.bubble_cloud_3d(N=N, precision=prec)
@@ -134,27 +115,26 @@ def main():
         prm, grids, q = ic.sod_1d(m=N - 1)
         prm.precision = prec
         workload = f"1D Sod {N} cells, WENO5+HLLC+RK3"
-    ncell = (prm.m + 1) * (prm.n + 1) * (prm.p + 1)
+    return prm, grids, q, N, workload
 
-    if args.copy_calibration:
-        src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+
+def time_steps(RhsModule, torch, ic, prm, grids, q, args, dt=None):
+    """Warm-up, then exactly args.steps timed TVD-RK3 steps with the state
+    resident in HBM, bracketed by device synchronisations; per-kernel HIP
+    event timing on the library's stream.  Returns (seconds, timing, dt,
+    module) with the module still initialised."""
     mod = RhsModule(prm, grids)
     q1 = torch.from_numpy(q).cuda()
     q2 = mod.new_field()
     rhs = mod.new_field(ghosted=False)
-    cfl = 0.4
-    dt = mod.cfl_dt(q1, cfl)
-    if prm.viscous:
-        dt = min(dt, ic.viscous_dt(prm, grids, q))
+    if dt is None:
+        dt = mod.cfl_dt(q1, 0.4)
+        if prm.viscous:
+            dt = min(dt, ic.viscous_dt(prm, grids, q))
     for t in range(args.warmup):
         mod.s_tvd_rk(t, (q1, q2), None, rhs, dt)
     mod.sync()
     torch.cuda.synchronize()
-
     mod.timing(enable=True, reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -162,24 +142,80 @@ def main():
         mod.s_tvd_rk(args.warmup + t, (q1, q2), None, rhs, dt)
     mod.sync()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    sec = time.perf_counter() - t0
     timing = mod.timing(enable=False)
-    sec = t1 - t0
+    assert torch.isfinite(q1).all().item(), "non-finite state after the timed steps"
+    return sec, timing, dt, mod, (q1, q2, rhs)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--case", default="shock_droplet_3d",
+                    choices=["shock_bubble_2d", "shock_droplet_3d", "shear_layer_2d", "bubble_cloud_3d", "sod_1d",
+                             "viscous_shear_2d"])
+    ap.add_argument("--N", type=int, default=0, help="cells per direction (default: the case's size)")
+    ap.add_argument("--precision", default=DEFAULT_PRECISION,
+                    choices=["fp64", "fp64_fast", "fp32_bulk", "fp32_learned"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-aux", action="store_true",
+                    help="skip the auxiliary-kernel block (cons->prim, CFL reduction, ghost fill, RK1 update)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip timing the plain (bit-exact) FP64 form beside the contracted one")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 2 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the multi-GPU halo exchange is designed but "
+                                   "not built, so --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from microfc_amd import PREC_FP32_BULK, PREC_FP32_LEARNED, PREC_FP64, PREC_FP64_FAST, RhsModule, ic
+
+    prec = {"fp64": PREC_FP64, "fp64_fast": PREC_FP64_FAST, "fp32_bulk": PREC_FP32_BULK,
+            "fp32_learned": PREC_FP32_LEARNED}[args.precision]
+    if prec == PREC_FP32_LEARNED:
+        # throughput does not depend on the weights; a network with a non-zero
+        # output layer exercises the softmax and the per-face fallback
+        from microfc_amd import learned
+        p = learned.init_params(0)
+        p[learned.OFF["W3"]:learned.OFF["W3"] + 512] = np.random.default_rng(1).normal(0, 0.1, 512)
+        learned.Mlp(p)
+    prm, grids, q, N, workload = make_case(args, prec, ic)
+    ncell = (prm.m + 1) * (prm.n + 1) * (prm.p + 1)
+    print(f"[bench] {workload}, {args.precision}: {args.warmup} warm-up + {args.steps} timed steps", file=sys.stderr,
+          flush=True)
+
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    sec, timing, dt, mod, (q1, q2, rhs) = time_steps(RhsModule, torch, ic, prm, grids, q, args)
     ms_per_step = 1e3 * sec / args.steps
     value = ncell * args.steps / sec
 
-    assert torch.isfinite(q1).all().item(), "non-finite state after the timed steps"
-
-    # roofline of the dominant kernel
-    per_step = sweep_bytes_per_step(prm)
-    dom = max(per_step, key=lambda k: timing[k][0])
-    dom_ms, dom_n = timing[dom]
-    dom_bytes_per_launch = per_step[dom] * ncell / (timing[dom][1] / args.steps)
-    dom_avg_s = dom_ms / 1e3 / dom_n
-    achieved = dom_bytes_per_launch / dom_avg_s / 1e9
-    prof = PROFILED.get((args.case, N, args.precision, dom), {})
     S, D = prm.sys_size, prm.num_dims
+    # Algorithmic HBM bytes: 64*S per cell per RK3 step (the compulsory FP64
+    # state traffic of the three stages: 16*S, 24*S, 24*S), i.e. 64*S/3 per
+    # cell per stage.  Each kernel launch of a stage covers every cell once,
+    # so a launch is charged one stage's bytes of all cells.
     step_bytes = 64 * S * ncell
+    stage_bytes = step_bytes / 3
+    per_step = sweep_bytes_per_step(prm)
+    launched = {k: v for k, v in timing.items() if v[1] and k in per_step}
+    dom = max(launched, key=lambda k: launched[k][0])
+    dom_ms, dom_n = launched[dom]
+    dom_avg_s = dom_ms / 1e3 / dom_n
+    achieved = stage_bytes / dom_avg_s / 1e9
+    prof = PROFILED.get((args.case, N, args.precision, dom), {})
     f_alg = 3 * (D * (85 * S + 180 + 4 * S + 10) + 20 + 4 * S)
     out = {
         "metric": "cell-updates/s per RK3 step",
@@ -192,31 +228,37 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {PREC_FP64: "f64", PREC_FP32_BULK: "f32 bulk / f64 accumulate",
+        "dtype": {PREC_FP64: "f64", PREC_FP64_FAST: "f64", PREC_FP32_BULK: "f32 bulk / f64 accumulate",
                   PREC_FP32_LEARNED: "f32 bulk + bf16 MFMA weights / f64 accumulate"}[prec],
         "data": "synthetic initial condition of the named case (generated, not read)",
-        "config": {"workload": workload, "cells": ncell, "sys_size": S, "cfl": cfl, "dt": dt,
-                   "precision": args.precision, "time_stepper": "TVD-RK3"},
+        "config": {"workload": workload, "cells": ncell, "sys_size": S, "cfl": 0.4, "dt": dt,
+                   "precision": args.precision, "form": FORMS[args.precision], "time_stepper": "TVD-RK3"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("traffic"),
-                     "valu_busy_profiled": prof.get("valu_busy"),
-                     "algorithmic_bytes_per_launch": dom_bytes_per_launch,
-                     "avg_launch_ms": dom_avg_s * 1e3},
+                     "algorithmic_bytes_per_launch": stage_bytes, "avg_launch_ms": dom_avg_s * 1e3,
+                     "limiter": prof.get("limiter"), "valu_busy_profiled": prof.get("valu_busy"),
+                     "profile": prof.get("profile")},
         "step_hbm": {"algorithmic_bytes_per_cell_step": 64 * S, "achieved_GBps": step_bytes * args.steps / sec / 1e9,
                      "frac": step_bytes * args.steps / sec / 1e9 / HBM_PEAK_GBS},
-        "fp64_valu": {"nominal_flops_per_cell_step": f_alg, "achieved_TFLOPs": f_alg * value / 1e12,
-                      "peak_TFLOPs": FP64_PEAK_TFLOPS, "frac": f_alg * value / 1e12 / FP64_PEAK_TFLOPS},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in timing.items() if v[1]},
     }
-    # every kernel of the step against HBM peak on its algorithmic bytes
+    if not prm.viscous:
+        out["fp64_valu"] = {"nominal_flops_per_cell_step": f_alg, "achieved_TFLOPs": f_alg * value / 1e12,
+                            "peak_TFLOPs": FP64_PEAK_TFLOPS, "frac": f_alg * value / 1e12 / FP64_PEAK_TFLOPS}
+    # every kernel of the step: one stage's algorithmic bytes per launch, and
+    # the bytes the fused design must move (stencil input, rhs partials, RK
+    # base and output) beside it
     kr = {}
     for k, bps in per_step.items():
         ms, n = timing[k]
         if n:
             per_launch = bps * ncell / (n / args.steps)
-            gbs = per_launch / (ms / 1e3 / n) / 1e9
-            kr[k] = {"avg_launch_ms": ms / n, "algorithmic_bytes_per_launch": per_launch, "achieved_GBps": gbs,
-                     "frac": gbs / HBM_PEAK_GBS, "launches_per_step": n / args.steps}
+            kr[k] = {"avg_launch_ms": ms / n, "launches_per_step": n / args.steps,
+                     "algorithmic_bytes_per_launch": stage_bytes,
+                     "achieved_GBps": stage_bytes / (ms / 1e3 / n) / 1e9,
+                     "frac": stage_bytes / (ms / 1e3 / n) / 1e9 / HBM_PEAK_GBS,
+                     "design_bytes_per_launch": per_launch,
+                     "design_GBps": per_launch / (ms / 1e3 / n) / 1e9}
     out["kernels_roofline"] = kr
     if not args.no_aux:
         out["aux_kernels"] = aux_kernels(mod, prm, grids, q1, dt, args, RhsModule, torch)
@@ -225,8 +267,20 @@ def main():
     del q1, q2, rhs
     torch.cuda.empty_cache()
 
+    if prec == PREC_FP64_FAST and not args.no_plain:
+        # the plain FP64 form (bit-exact against the oracle) on the same case
+        prm.precision = PREC_FP64
+        psec, ptiming, _, pmod, bufs = time_steps(RhsModule, torch, ic, prm, grids, q, args, dt=dt)
+        pmod.finalize()
+        del bufs
+        torch.cuda.empty_cache()
+        prm.precision = prec
+        out["plain_fp64"] = {"value": ncell * args.steps / psec, "ms_per_step": 1e3 * psec / args.steps,
+                             "kernels_ms_per_step": {k: v[0] / args.steps for k, v in ptiming.items() if v[1]},
+                             "form": FORMS["fp64"]}
+
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args, prm, grids, q, np)
+        out["cpu_baseline"] = cpu_baseline(prm, grids, q, np, ic)
     print(json.dumps(out))
 
 
@@ -282,39 +336,61 @@ def aux_kernels(mod, prm, grids, q1, dt, args, RhsModule, torch, calls=5):
     return out
 
 
-def cpu_baseline(args, prm, grids, q, np):
-    """The FP64 oracle on the host: one TVD-RK3 step of the same grid (a
-    bounded sample of the workload), same dt rule, on every host thread the
-    run may use (OpenMP over pencils, OMP_NUM_THREADS) and on one core."""
-    import ctypes
-
-    from oracle import oracle as O
+def _cpu_step_seconds(O, L, prm, grids, q, np, threads):
     oc = O.Case(m=prm.m, n=prm.n, p=prm.p, num_fluids=prm.num_fluids, weno_order=prm.weno_order,
                 time_stepper=prm.time_stepper, weno_eps=prm.weno_eps, gammas=list(prm.gammas),
                 pi_infs=list(prm.pi_infs), bc_beg=prm.bc_beg, bc_end=prm.bc_end,
                 Re=[tuple(r) for r in prm.Re] if prm.Re else None)
+    L.orc_set_threads(threads)
+    q1 = q.copy()
+    q2 = np.zeros_like(q1)
+    t0 = time.perf_counter()
+    O.tvd_rk_step(oc, grids, 1e-9, q1, q2, t_step=0)
+    return time.perf_counter() - t0
+
+
+def _sample(prm, grids, q, ic, cells):
+    """The benchmark case itself when it has at most `cells` cells, else a
+    slab of it through the domain centre (whole extent in x and y, as many
+    z (3D) or y (2D) layers as fit), cut faces extrapolated."""
     ncell = (prm.m + 1) * (prm.n + 1) * (prm.p + 1)
-    if ncell > 20_000_000:
-        return {"value": None, "unit": "cell-updates/s", "cores": None, "kind": "port",
-                "sample": "skipped: grid too large for a bounded CPU sample"}
+    if ncell <= cells:
+        return prm, grids, q, "the whole grid"
+    D = prm.num_dims
+    ext = [prm.m + 1, prm.n + 1, prm.p + 1][:D]
+    layer = 1
+    for e in ext[:-1]:
+        layer *= e
+    k = max(5 * prm.weno_order, cells // layer)
+    lo = [0] * D
+    shape = list(ext)
+    lo[-1] = (ext[-1] - k) // 2
+    shape[-1] = k
+    sub = ic.sub_box(prm, grids, q, lo, shape)
+    axis = "zy"[3 - D] if D > 1 else "x"
+    return sub + (f"a {'x'.join(map(str, shape))} slab of it ({axis} = {lo[-1]}..{lo[-1] + k - 1})",)
+
+
+def cpu_baseline(prm, grids, q, np, ic):
+    """The FP64 oracle on the host: one TVD-RK3 step (same dt rule) of a
+    bounded sample of the same case, on every host thread the run may use
+    (OpenMP over pencils, OMP_NUM_THREADS) and on one core."""
+    import ctypes
+
+    from oracle import oracle as O
     L = O.lib()
     L.orc_set_threads.argtypes = [ctypes.c_int]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-
-    def one_step(n):
-        L.orc_set_threads(n)
-        q1 = q.copy()
-        q2 = np.zeros_like(q1)
-        t0 = time.perf_counter()
-        O.tvd_rk_step(oc, grids, 1e-9, q1, q2, t_step=0)
-        return time.perf_counter() - t0
-
-    sec_all = one_step(threads)
-    sec_one = one_step(1)
-    return {"value": ncell / sec_all, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"one TVD-RK3 step of the same {ncell}-cell grid on {threads} host threads "
-                      f"({sec_all:.2f} s) and on one core ({sec_one:.1f} s)",
-            "single_core": {"value": ncell / sec_one, "cores": 1}}
+    pa, ga, qa, what_a = _sample(prm, grids, q, ic, 4_400_000)
+    sec_all = _cpu_step_seconds(O, L, pa, ga, qa, np, threads)
+    pb, gb, qb, what_b = _sample(prm, grids, q, ic, 600_000)
+    sec_one = _cpu_step_seconds(O, L, pb, gb, qb, np, 1)
+    na = (pa.m + 1) * (pa.n + 1) * (pa.p + 1)
+    nb = (pb.m + 1) * (pb.n + 1) * (pb.p + 1)
+    return {"value": na / sec_all, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"one TVD-RK3 step of {what_a} ({na} cells) on {threads} host threads ({sec_all:.2f} s); "
+                      f"one core: {what_b} ({nb} cells, {sec_one:.2f} s)",
+            "single_core": {"value": nb / sec_one, "cores": 1}}
 
 
 if __name__ == "__main__":

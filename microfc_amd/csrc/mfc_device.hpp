@@ -10,7 +10,8 @@
 // form (MFC_FORM), so units built with different flags or with the
 // contracted FP64 form (MFC_FAST_F64 = 1: FMA contraction, divisions as a
 // multiply by a Newton-refined reciprocal, the WENO weights normalised by one
-// reciprocal of their sum) never share a kernel or function symbol.
+// reciprocal of their sum, square roots from v_rsq_f64 refined) never share
+// a kernel or function symbol.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -65,6 +66,31 @@ __device__ __forceinline__ double frcp(double b) {
     return __builtin_fma(r1, e1, r1);
 }
 
+// Reciprocal to within about 20 ulp (measured on gfx950: v_rcp_f64 alone is
+// within 5e-8 relative, one Newton step brings it to 2.2e-15): the
+// contracted form's 1/beta_k^2, whose error only moves the weights of the
+// candidate polynomials, i.e. a face value by that error times the spread
+// of the candidates (the normalisation 1/sum(alpha), which scales the whole
+// value, takes frcp).
+__device__ __forceinline__ double frcp1(double b) {
+    const double r = __builtin_amdgcn_rcp(b);
+    return __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+}
+
+// Square root to within about one ulp from v_rsq_f64 (5e-8 relative): one
+// Newton step on the reciprocal square root, then one on the root itself.
+// 0 gives 0 and a negative argument NaN, like sqrt; no range scaling, so
+// arguments below 1e-290 are not handled (the squared sound speeds it takes
+// are far above that).
+__device__ __forceinline__ double fsqrt(double x) {
+    const double y0 = __builtin_amdgcn_rsq(x);
+    const double y1 = __builtin_fma(y0, __builtin_fma(-0.5 * x * y0, y0, 0.5), y0);
+    const double g = x * y1;
+    const double r = __builtin_fma(-g, g, x);
+    const double root = __builtin_fma(0.5 * y1, r, g);
+    return x == 0.0 ? x : root;
+}
+
 // Division in the bulk precision: IEEE correctly rounded in the plain FP64
 // form (it reproduces the reference bit for bit); a multiply by frcp (about
 // 1.5 ulp) in the contracted FP64 form; in the FP32 bulk a reciprocal-multiply
@@ -76,7 +102,12 @@ __device__ __forceinline__ double bdiv(double a, double b) {
         return a / b;
 }
 __device__ __forceinline__ float bdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double bsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ double bsqrt(double a) {
+    if constexpr (kFastF64)
+        return fsqrt(a);
+    else
+        return sqrt(a);
+}
 __device__ __forceinline__ float bsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 
 template <typename T>
@@ -171,7 +202,7 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
         if constexpr (kFastF64 && sizeof(R) == 8) {
             // alpha_k = d_k / beta_k^2 and omega_k = alpha_k / sum(alpha) as
             // products with the reciprocals (shared by both faces)
-            const R r0 = frcp(bb0), r1 = frcp(bb1);
+            const R r0 = frcp1(bb0), r1 = frcp1(bb1);
             if constexpr (WANT_L) {
                 const R a0 = cf[kDL + 0] * r0, a1 = cf[kDL + 1] * r1;
                 vL = (a0 * pL0 + a1 * pL1) * frcp(a0 + a1);
@@ -217,7 +248,7 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
             pR[2] = vj + cf[kCbR + 4] * dm1 + cf[kCbR + 5] * dm2;
         }
         if constexpr (kFastF64 && sizeof(R) == 8) {
-            const R r0 = frcp(bb0), r1 = frcp(bb1), r2 = frcp(bb2);
+            const R r0 = frcp1(bb0), r1 = frcp1(bb1), r2 = frcp1(bb2);
             if constexpr (WANT_L) {
                 const R a0 = cf[kDL + 0] * r0, a1 = cf[kDL + 1] * r1, a2 = cf[kDL + 2] * r2;
                 vL = (a0 * pL[0] + a1 * pL[1] + a2 * pL[2]) * frcp(a0 + a1 + a2);

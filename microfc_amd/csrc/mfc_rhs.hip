@@ -782,13 +782,21 @@ int viscous(int d, const Fields &qin, const Fields &rhs, const SweepArgs &sa) {
 // Viscous runs use the viscous kernels for each direction instead, with the
 // same finish semantics.
 int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, const Fields *qout, double dt) {
-    for (int d = 0; d < G.D; ++d) {
+    // The plain forms add the directions in the reference's order x, y, z
+    // (m_rhs.fpp:469-656; it fixes the rounding).  The contracted FP64 form
+    // runs them z, y, x: the sum is the same up to rounding, and the RK stage
+    // (base, rhs and stage output traffic) then rides on the x sweep, whose
+    // loads are contiguous rows, instead of on the z sweep's plane-strided
+    // tiles.
+    const bool reverse = G.prm.precision == MFC_PREC_FP64_FAST && !G.visc;
+    for (int k = 0; k < G.D; ++k) {
+        const int d = reverse ? G.D - 1 - k : k;
         SweepArgs a = sweep_args(d);
         a.qin = qin;
         a.rhs = rhs;
-        a.accumulate = d > 0;
+        a.accumulate = k > 0;
         a.dt = dt;
-        const bool last = d == G.D - 1;
+        const bool last = k == G.D - 1;
         a.rk = last ? rk : kRkNone;
         if (a.rk != kRkNone) {
             a.qbase = *qbase;

@@ -33,6 +33,29 @@ def rel_linf(a, b):
     return np.array(out)
 
 
+def state_linf(prm, a, b, q):
+    """Per-variable L-inf error of a state a against b on the scales of the
+    state q the step started from: max|alpha_i rho_i| for the partial
+    densities, max(rho) * max(c) for every momentum component (a momentum
+    that is zero in q, as in a fluid at rest, is measured on the momentum a
+    sound wave would carry), max|E| for the energy and 1 for the volume
+    fractions."""
+    nf, D = prm.num_fluids, prm.num_dims
+    qi = np.asarray(prm.interior(q), dtype=np.float64)
+    rho = sum(qi[i] for i in range(nf))
+    gam = sum(qi[nf + D + 1 + i] * prm.gammas[i] for i in range(nf))
+    pinf = sum(qi[nf + D + 1 + i] * prm.pi_infs[i] for i in range(nf))
+    dyn = sum(0.5 * qi[nf + d] ** 2 for d in range(D)) / rho
+    pres = (qi[nf + D] - dyn - pinf) / gam
+    c = np.sqrt(((gam + 1.0) * pres + pinf) / (gam * rho))
+    mom = float(np.max(rho)) * float(np.max(c))
+    den = [float(np.max(np.abs(qi[i]))) for i in range(nf)] + [mom] * D + [float(np.max(np.abs(qi[nf + D])))] + \
+        [1.0] * nf
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.array([np.max(np.abs(a[v] - b[v])) / den[v] for v in range(a.shape[0])])
+
+
 def scaled_linf(prm, a, b):
     """Per-variable L-inf error of a against b, normalised by a physical
     scale: max|b_v| for partial densities and energy, the largest momentum
