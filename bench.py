@@ -29,13 +29,20 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 # Measured HBM bytes (read + written) per launch of the dominant kernel, from
-# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script's
-# default command, scaled by a 2 GiB device copy (FETCH_SIZE x2.0, WRITE_SIZE
-# x1.0), and its VALU-busy fraction from SQ_ACTIVE_INST_VALU / GRBM_GUI_ACTIVE
-# (profiles/hbm_traffic_shock_bubble_2d_2048_fp64.md,
-# profiles/sweep_kernels_shock_bubble_2d_2048_fp64.md).  Other workloads
-# report null.
-PROFILED = {("shock_bubble_2d", 2048, "fp64", "sweep_y"): {"traffic": 678.5e6 + 236.6e6, "valu_busy": 0.79}}
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script with the
+# same case and form, scaled by a 2 GiB device copy (FETCH_SIZE x2.0,
+# WRITE_SIZE x1.0) and averaged over the kernel's launches in the timed steps,
+# and its VALU-busy fraction (SQ_ACTIVE_INST_VALU * 4 / (1024 CU-SIMDs *
+# GRBM_GUI_ACTIVE / 8)), with the profile summary they come from.  Other
+# workloads report null.
+PROFILED = {
+    ("shock_droplet_3d", 512, "fp64_fast", "sweep_x"): {
+        "traffic": 29.06e9 + 8.80e9, "valu_busy": 0.75, "limiter": "VALU issue (FP64), not HBM",
+        "profile": "profiles/sweep_kernels_shock_droplet_3d_512_fp64_fast.md"},
+    ("shock_bubble_2d", 2048, "fp64", "sweep_y"): {
+        "traffic": 678.5e6 + 236.6e6, "valu_busy": 0.79, "limiter": "VALU issue (FP64), not HBM",
+        "profile": "profiles/sweep_kernels_shock_bubble_2d_2048_fp64.md"},
+}
 
 DEFAULT_PRECISION = "fp64_fast"
 FORMS = {"fp64": "plain FP64: the reference's evaluation order, no FMA contraction, IEEE division (bit-exact "
@@ -44,6 +51,14 @@ FORMS = {"fp64": "plain FP64: the reference's evaluation order, no FMA contracti
                       "by one reciprocal (within 1e-12 relative of the FP64 oracle per step)",
          "fp32_bulk": "FP32 WENO/HLLC, FP64 cons->prim, flux differences and RK stages",
          "fp32_learned": "FP32 bulk with learned WENO5 weights (BF16 MFMA)"}
+
+
+def sweep_order(prm):
+    """The library's sweep order: x, y, z as the reference adds them; the
+    contracted FP64 form of an inviscid case runs z, y, x, so that the RK
+    stage is fused into the x sweep."""
+    names = ["sweep_x", "sweep_y", "sweep_z"][:prm.num_dims]
+    return names[::-1] if prm.precision == 3 and not prm.viscous else names
 
 
 def sweep_bytes_per_step(prm):
@@ -55,15 +70,23 @@ def sweep_bytes_per_step(prm):
     reads the stencil input and writes a face record (HLLC flux, contact
     velocity, viscous flux: S+2+D doubles), the finish reads it and the
     volume fractions, and reads/writes rhs like a sweep (or, in the last
-    direction, reads the RK base and stage input and writes the output)."""
+    direction, reads the RK base and stage input and writes the output).
+    With weno_Re_flux the velocity-gradient planes and their face averages
+    add their passes."""
     S, D, NF = prm.sys_size, prm.num_dims, prm.num_fluids
     out = {}
-    names = ["sweep_x", "sweep_y", "sweep_z"][:D]
+    names = sweep_order(prm)
     if prm.viscous:
         total = 0
         for d in range(D):
             first, last = d == 0, d == D - 1
             per = S + 2 * (S + 2 + D) + NF + (0 if first else S) + (3 * S if last else S)
+            if prm.weno_Re_flux:
+                # the gradient pass of direction d reads the densities and
+                # momenta and writes D gradient planes; the face-average pass
+                # reads the D*D planes and writes D*D face values, which the
+                # face kernel reads
+                per += (NF + 2 * D) + 3 * D * D
             total += 3 * 8 * per
         out["viscous"] = total
         return out
@@ -102,10 +125,11 @@ def make_case(args, prec, ic):
         workload = f"3D shock-bubble cloud water/air {N}^3 (64 air bubbles), WENO5+HLLC+RK3"
     elif args.case == "viscous_shear_2d":
         N = args.N or 2048
-        prm, grids, q = ic.viscous_shear_2d(N=N)
+        prm, grids, q = ic.viscous_shear_2d(N=N, weno_Re_flux=not args.fd_gradients)
         prm.precision = prec
+        grad = "finite-difference" if args.fd_gradients else "WENO-reconstructed (weno_Re_flux)"
         workload = (f"2D viscous shear layer (the reference's 2D_viscous case) {N}x{N}, two fluids, Re 1e-4, "
-                    "WENO5+HLLC+viscous+RK3")
+                    f"WENO5+HLLC+viscous ({grad} velocity gradients)+RK3")
     elif args.case == "shear_layer_2d":
         N = args.N or 4096
         prm, grids, q = ic.shear_layer_2d(N=N, precision=prec)
@@ -157,8 +181,12 @@ def main():
                     choices=["shock_bubble_2d", "shock_droplet_3d", "shear_layer_2d", "bubble_cloud_3d", "sod_1d",
                              "viscous_shear_2d"])
     ap.add_argument("--N", type=int, default=0, help="cells per direction (default: the case's size)")
-    ap.add_argument("--precision", default=DEFAULT_PRECISION,
-                    choices=["fp64", "fp64_fast", "fp32_bulk", "fp32_learned"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp64_fast", "fp32_bulk", "fp32_learned"],
+                    help=f"arithmetic form (default {DEFAULT_PRECISION}; fp64 for the viscous case, whose "
+                         "terms run in the plain form only)")
+    ap.add_argument("--fd-gradients", action="store_true",
+                    help="viscous case: finite-difference velocity gradients instead of the example's "
+                         "weno_Re_flux")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the auxiliary-kernel block (cons->prim, CFL reduction, ghost fill, RK1 update)")
@@ -167,6 +195,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "fp64" if args.case == "viscous_shear_2d" else DEFAULT_PRECISION
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU halo exchange is designed but "
@@ -238,8 +268,13 @@ def main():
                      "algorithmic_bytes_per_launch": stage_bytes, "avg_launch_ms": dom_avg_s * 1e3,
                      "limiter": prof.get("limiter"), "valu_busy_profiled": prof.get("valu_busy"),
                      "profile": prof.get("profile")},
+        # the step's compulsory state traffic, and the bytes this design's
+        # kernels must move (the sweep_bytes_per_step model: rhs partials, RK
+        # base, and in viscous runs the face records and finish passes)
         "step_hbm": {"algorithmic_bytes_per_cell_step": 64 * S, "achieved_GBps": step_bytes * args.steps / sec / 1e9,
-                     "frac": step_bytes * args.steps / sec / 1e9 / HBM_PEAK_GBS},
+                     "frac": step_bytes * args.steps / sec / 1e9 / HBM_PEAK_GBS,
+                     "design_bytes_per_cell_step": sum(per_step.values()),
+                     "design_GBps": sum(per_step.values()) * ncell * args.steps / sec / 1e9},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in timing.items() if v[1]},
     }
     if not prm.viscous:

@@ -67,9 +67,14 @@ typedef struct mfc_params {
      * (the reference's default dflt_real = -1e6, or a zeroed struct) leaves
      * fluid i out of that viscous term (m_global_parameters.fpp:314-338); any
      * positive entry turns the viscous terms on (m_viscous.fpp,
-     * m_riemann_solvers.fpp:683-902) with finite-difference velocity
-     * gradients (weno_Re_flux = F), in 1D and 2D and in MFC_PREC_FP64. */
+     * m_riemann_solvers.fpp:683-902), in 1D and 2D and in MFC_PREC_FP64. */
     double Re[MFC_NUM_FLUIDS_MAX][2];
+    /* weno_Re_flux (m_global_parameters.fpp:103): nonzero takes the viscous
+     * velocity gradients from WENO face values by the divergence theorem and
+     * reconstructs them to the faces with WENO (m_viscous.fpp:200-214,
+     * :372-591, m_rhs.fpp:514-528); zero uses finite differences
+     * (m_viscous.fpp:216-355).  Ignored by inviscid cases. */
+    int weno_Re_flux;
 } mfc_params;
 
 /* Replaces the module set-up chain s_initialize_global_parameters_module,

@@ -51,6 +51,14 @@ def _fmt(val):
     return str(val)
 
 
+def simulation_inputs(case: dict) -> dict:
+    """The simulation's keys of a whole case dictionary (the domain, patches,
+    stretching and output-format keys of the other programs dropped), as the
+    toolchain writes each program's .inp from one dictionary
+    (toolchain/mfc/run/input.py:14-32)."""
+    return {k: v for k, v in case.items() if k in SIMULATION_KEYS}
+
+
 def write_inp(case: dict, path: str) -> str:
     lines = []
     for key, val in case.items():
@@ -70,6 +78,8 @@ def case_from_params(prm: Params, dt: float, t_step_stop: int, t_step_save: int,
          "dt": dt, "t_step_start": t_step_start, "t_step_stop": t_step_stop, "t_step_save": t_step_save,
          "num_fluids": prm.num_fluids, "time_stepper": prm.time_stepper, "weno_order": prm.weno_order,
          "weno_eps": prm.weno_eps, "bulk_precision": prm.precision}
+    if prm.viscous:
+        c["weno_Re_flux"] = bool(prm.weno_Re_flux)
     for d, nm in enumerate("xyz"[: prm.num_dims]):
         c[f"bc_{nm}%beg"] = prm.bc_beg[d]
         c[f"bc_{nm}%end"] = prm.bc_end[d]

@@ -76,6 +76,7 @@ struct Dir {
     double *inv_ds = nullptr; // device, s+1
     double *ds_dev = nullptr; // device, ds(0..s)
     double *cc_dev = nullptr; // device, ghosted cell centres s_cc(-b..s+b)
+    double *inv_dsg = nullptr; // device, 1/ds(i) for i = -b..s+b (weno_Re_flux gradients)
     std::vector<double> coef_h, ds_h;
     bool ready = false;
     bool uniform = false;  // every reconstructed cell has the same coefficients
@@ -96,6 +97,9 @@ struct State {
     ViscParams vp{};
     double *visc_fs = nullptr;       // (S+2+D) x faces of one direction: HLLC flux, contact velocity, viscous flux
     long visc_nface = 0;
+    bool wrf = false;                // viscous with weno_Re_flux: WENO-reconstructed velocity gradients
+    double *visc_grad = nullptr;     // D*D planes of ng: cell-centred velocity gradients (weno_Re_flux)
+    double *visc_fgrad = nullptr;    // D*D planes of visc_nface: their face averages (weno_Re_flux)
 } G;
 
 // ---------------------------------------------------------------------------
@@ -771,6 +775,10 @@ int viscous(int d, const Fields &qin, const Fields &rhs, const SweepArgs &sa) {
     v.bc_beg = bcs[d][0];
     v.bc_end = bcs[d][1];
     v.vp = G.vp;
+    v.grad = G.visc_grad;
+    v.fgrad = G.visc_fgrad;
+    v.ng = G.ng;
+    v.wrf = G.wrf ? 1 : 0;
     TimedScope ts(kKViscous);
     if (launch_viscous(G.prm.num_fluids, G.D, d, G.prm.weno_order, v, sa, G.stream))
         return fail(-3, "viscous launch failed (dir %d): %s", d, hipGetErrorString(hipGetLastError()));
@@ -789,6 +797,31 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
     // loads are contiguous rows, instead of on the z sweep's plane-strided
     // tiles.
     const bool reverse = G.prm.precision == MFC_PREC_FP64_FAST && !G.visc;
+    if (G.wrf) {
+        // cell-centred velocity gradients of the stage input (s_get_viscous,
+        // m_rhs.fpp:456-464), before the directions
+        for (int e = 0; e < G.D; ++e) {
+            ViscGradArgs g;
+            memset(&g, 0, sizeof g);
+            g.q = qin;
+            g.grad = G.visc_grad;
+            g.coef = G.dir[e].coef;
+            g.ncell = G.dir[e].s + 2 * G.b + 1;
+            g.inv_dsg = G.dir[e].inv_dsg;
+            g.eps = G.prm.weno_eps;
+            g.nxg = G.nxg;
+            g.ng = G.ng;
+            g.m = G.prm.m;
+            g.n = G.prm.n;
+            g.b = G.b;
+            g.bj = G.prm.n > 0 ? G.b : 0;
+            g.s = G.dir[e].s;
+            TimedScope ts(kKViscous);
+            if (launch_visc_grad(G.prm.num_fluids, G.D, e, G.prm.weno_order, g, G.stream))
+                return fail(-3, "viscous gradient launch failed (dir %d): %s", e,
+                            hipGetErrorString(hipGetLastError()));
+        }
+    }
     for (int k = 0; k < G.D; ++k) {
         const int d = reverse ? G.D - 1 - k : k;
         SweepArgs a = sweep_args(d);
@@ -865,6 +898,11 @@ int mfc_rhs_init(const mfc_params *prm) {
     if (G.visc) {
         G.visc_nface = std::max((long)(prm->m + 2) * (prm->n + 1), (long)(prm->m + 1) * (prm->n + 2));
         HIPCHK(hipMalloc(&G.visc_fs, sizeof(double) * (G.S + 2 + G.D) * G.visc_nface));
+        G.wrf = prm->weno_Re_flux != 0;
+        if (G.wrf) {
+            HIPCHK(hipMalloc(&G.visc_grad, sizeof(double) * G.D * G.D * G.ng));
+            HIPCHK(hipMalloc(&G.visc_fgrad, sizeof(double) * G.D * G.D * G.visc_nface));
+        }
     }
     G.init = true;
     return 0;
@@ -890,8 +928,9 @@ int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds) {
                 D.uniform = std::memcmp(&v0, &D.coef_h[(long)q * nc + (j + b)], sizeof(double)) == 0;
         }
     }
-    std::vector<double> inv(s + 1);
+    std::vector<double> inv(s + 1), invg(s + 2 * b + 1);
     for (int i = 0; i <= s; ++i) inv[i] = 1.0 / ds[i + b];
+    for (int i = -b; i <= s + b; ++i) invg[i + b] = 1.0 / ds[i + b]; // 1d0/dx(j), m_viscous.fpp:386-393
     // cell centres with ghosts (m_start_up.fpp:356-359, :551, :582)
     std::vector<double> cc(s + 2 * b + 1);
     for (int i = 0; i <= s; ++i) cc[i + b] = s_cb[i - 1 + 1 + b] + ds[i + b] / 2.0;
@@ -902,6 +941,8 @@ int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds) {
     if (D.inv_ds) HIPCHK(hipFree(D.inv_ds));
     if (D.ds_dev) HIPCHK(hipFree(D.ds_dev));
     if (D.cc_dev) HIPCHK(hipFree(D.cc_dev));
+    if (D.inv_dsg) HIPCHK(hipFree(D.inv_dsg));
+    D.inv_dsg = nullptr;
     D.coef = nullptr;
     D.inv_ds = nullptr;
     D.ds_dev = nullptr;
@@ -914,6 +955,10 @@ int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds) {
     HIPCHK(hipMemcpy(D.ds_dev, ds + b, sizeof(double) * (s + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(D.coef, D.coef_h.data(), sizeof(double) * D.coef_h.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(D.inv_ds, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+    if (G.wrf) {
+        HIPCHK(hipMalloc(&D.inv_dsg, sizeof(double) * invg.size()));
+        HIPCHK(hipMemcpy(D.inv_dsg, invg.data(), sizeof(double) * invg.size(), hipMemcpyHostToDevice));
+    }
     D.ready = true;
     return 0;
 }
@@ -1174,11 +1219,14 @@ int mfc_rhs_finalize(void) {
         if (G.dir[d].inv_ds) (void)hipFree(G.dir[d].inv_ds);
         if (G.dir[d].ds_dev) (void)hipFree(G.dir[d].ds_dev);
         if (G.dir[d].cc_dev) (void)hipFree(G.dir[d].cc_dev);
+        if (G.dir[d].inv_dsg) (void)hipFree(G.dir[d].inv_dsg);
         G.dir[d] = Dir();
     }
     if (G.partials) (void)hipFree(G.partials);
     if (G.scratch_state) (void)hipFree(G.scratch_state);
     if (G.visc_fs) (void)hipFree(G.visc_fs);
+    if (G.visc_grad) (void)hipFree(G.visc_grad);
+    if (G.visc_fgrad) (void)hipFree(G.visc_fgrad);
     if (G.own) (void)hipStreamDestroy(G.own);
     G = State();
     return 0;

@@ -65,16 +65,13 @@ __device__ __forceinline__ void cell_coef(const ViscArgs &a, int c, double *cf) 
 }
 
 // Viscous source flux of one face (m_riemann_solvers.fpp:683-902) from its
-// two face states, star velocity and gradients.  The reference passes
-// dqR_prim_d*_n as the solver's left gradients and dqL_prim_d*_n as its right
-// ones (m_rhs.fpp:545-556); both sides of a face evaluate the same
-// difference, so each average below is of two equal terms, kept as written.
-// gxn[d]: normal-direction one-sided difference of velocity component d
-// across the face; gt[d]: the four-point average of the transverse
-// differences (2D).
+// two face states, star velocity and the face averages of the velocity
+// gradients: ax[c] = dvel_avg_dx(c), ay[c] = dvel_avg_dy(c), each
+// 5e-1*(solver-left + solver-right value) as the reference forms it (the
+// solver's left gradients are the caller's dqR_prim_d*_n, m_rhs.fpp:545-556).
 template <int NF, int D, int DIR>
-__device__ __forceinline__ void face_flux(const ViscArgs &a, const double *ql, const double *qr, const double *gxn,
-                                          const double *gt, double *flux, double &ustar, double *fs) {
+__device__ __forceinline__ void face_flux(const ViscArgs &a, const double *ql, const double *qr, const double *ax,
+                                          const double *ay, double *flux, double &ustar, double *fs) {
     double vsrc[D];
     hllc_face<NF, D, DIR, double, true>(ql, qr, a.eos, flux, ustar, vsrc);
     double ra[2];
@@ -82,27 +79,21 @@ __device__ __forceinline__ void face_flux(const ViscArgs &a, const double *ql, c
 #pragma unroll
     for (int v = 0; v <= D; ++v) fs[v] = 0.0;
     if constexpr (DIR == 0) {
-        // gxn = d(u, v)/dx across the face, gt = d(u, v)/dy averaged
         if (a.vp.size[0] > 0) {
-            const double dvx = 5e-1 * (gxn[0] + gxn[0]);
-            const double t11 = (4.0 / 3.0) * dvx / ra[0];
+            const double t11 = (4.0 / 3.0) * ax[0] / ra[0];
             fs[0] = fs[0] - t11;
             fs[D] = fs[D] - vsrc[0] * t11;
         }
         if (a.vp.size[1] > 0) {
-            const double dvx = 5e-1 * (gxn[0] + gxn[0]);
-            const double t11 = dvx / ra[1];
+            const double t11 = ax[0] / ra[1];
             fs[0] = fs[0] - t11;
             fs[D] = fs[D] - vsrc[0] * t11;
         }
         if constexpr (D == 2) {
             if (a.vp.size[0] > 0) {
-                double dvy[2], tau[2];
-#pragma unroll
-                for (int d = 0; d < 2; ++d) dvy[d] = 5e-1 * (gt[d] + gt[d]);
-                const double dvx2 = 5e-1 * (gxn[1] + gxn[1]);
-                tau[0] = -((2.0 / 3.0) * dvy[1] / ra[0]);
-                tau[1] = (dvy[0] + dvx2) / ra[0];
+                double tau[2];
+                tau[0] = -((2.0 / 3.0) * ay[1] / ra[0]);
+                tau[1] = (ay[0] + ax[1]) / ra[0];
 #pragma unroll
                 for (int d = 0; d < 2; ++d) {
                     fs[d] = fs[d] - tau[d];
@@ -110,23 +101,16 @@ __device__ __forceinline__ void face_flux(const ViscArgs &a, const double *ql, c
                 }
             }
             if (a.vp.size[1] > 0) {
-                const double dvy2 = 5e-1 * (gt[1] + gt[1]);
-                const double t11 = dvy2 / ra[1];
+                const double t11 = ay[1] / ra[1];
                 fs[0] = fs[0] - t11;
                 fs[D] = fs[D] - vsrc[0] * t11;
             }
         }
     } else {
-        // gxn = d(u, v)/dy across the face, gt = d(u, v)/dx averaged
         if (a.vp.size[0] > 0) {
-            double dvx[2], dvy[2], tau[2];
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                dvx[d] = 5e-1 * (gt[d] + gt[d]);
-                dvy[d] = 5e-1 * (gxn[d] + gxn[d]);
-            }
-            tau[0] = (dvy[0] + dvx[1]) / ra[0];
-            tau[1] = (4.0 * dvy[1] - 2.0 * dvx[0]) / (3.0 * ra[0]);
+            double tau[2];
+            tau[0] = (ay[0] + ax[1]) / ra[0];
+            tau[1] = (4.0 * ay[1] - 2.0 * ax[0]) / (3.0 * ra[0]);
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
                 fs[d] = fs[d] - tau[d];
@@ -134,12 +118,34 @@ __device__ __forceinline__ void face_flux(const ViscArgs &a, const double *ql, c
             }
         }
         if (a.vp.size[1] > 0) {
-            const double dvx1 = 5e-1 * (gt[0] + gt[0]);
-            const double dvy2 = 5e-1 * (gxn[1] + gxn[1]);
-            const double t22 = (dvx1 + dvy2) / ra[1];
+            const double t22 = (ax[0] + ay[1]) / ra[1];
             fs[1] = fs[1] - t22;
             fs[D] = fs[D] - vsrc[1] * t22;
         }
+    }
+}
+
+// Face averages of finite-difference gradients: both sides of a face hold
+// the same difference (dqR_prim_d*_n of the left cell and dqL_prim_d*_n of
+// the right one, m_viscous.fpp:231-336), so each average is 5e-1*(g + g).
+// gn: differences across the face, gt: the transverse four-point averages.
+template <int D, int DIR>
+__device__ __forceinline__ void fd_averages(const double *gn, const double *gt, double *ax, double *ay) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const double n2 = 5e-1 * (gn[c] + gn[c]), t2 = 5e-1 * (gt[c] + gt[c]);
+        ax[c] = DIR == 0 ? n2 : t2;
+        ay[c] = DIR == 0 ? t2 : n2;
+    }
+}
+
+// The face averages precomputed by k_visc_face_grad (weno_Re_flux).
+template <int D>
+__device__ __forceinline__ void weno_averages(const ViscArgs &a, long t, double *ax, double *ay) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        ax[c] = c < D ? a.fgrad[(long)c * a.nface + t] : 0.0;
+        ay[c] = c < D && D == 2 ? a.fgrad[(long)(D + c) * a.nface + t] : 0.0;
     }
 }
 
@@ -156,6 +162,100 @@ __device__ __forceinline__ void store_face(const ViscArgs &a, long t, const doub
     for (int v = 0; v <= D; ++v) a.fs[(long)(S + 1 + v) * a.nface + t] = fs[v];
 }
 
+// Gradient planes of direction E (weno_Re_flux), one thread per ghosted
+// cell: each velocity component reconstructed along E with WENO over cells
+// [-b+polyn, s+b-polyn] (s_reconstruct_cell_boundary_values_visc,
+// m_viscous.fpp:400-469), then d u_c / d x_E = 1/ds * (vR - vL)
+// (s_apply_scalar_divergence_theorem, :372-398) over [-b+1, s+b-1].  Cells
+// where the reference reads face values it never wrote, or never writes the
+// gradient, get NaN: their reconstructions reach no face flux the RHS uses.
+template <int NF, int D, int E, int WO>
+__global__ __launch_bounds__(256) void k_visc_grad(ViscGradArgs g) {
+    using Y = Sys<NF, D>;
+    constexpr int P = (WO - 1) / 2;
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= g.ng) return;
+    const int i = (int)(o % g.nxg) - g.b;
+    const int j = (int)(o / g.nxg) - g.bj;
+    const int je = E == 0 ? i : j;
+    const bool set = je >= -g.b + P && je <= g.s + g.b - P && je >= -g.b + 1 && je <= g.s + g.b - 1;
+    const long step = E == 0 ? 1 : g.nxg;
+    double cf[kNCoef];
+    if (set) {
+#pragma unroll
+        for (int q = 0; q < kNCoef; ++q) cf[q] = g.coef[(long)q * g.ncell + (je + g.b)];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        double val = __builtin_nan("");
+        if (set) {
+            double st[WO];
+#pragma unroll
+            for (int r = 0; r < WO; ++r) {
+                const long oc = o + (r - P) * step;
+                double rho = 0.0;
+#pragma unroll
+                for (int f = 0; f < NF; ++f) rho = rho + g.q.v[f][oc];
+                rho = max_ref(rho, 1e-16);
+                st[r] = g.q.v[Y::MOM + c][oc] / rho;
+            }
+            double vl, vr;
+            weno_cell<WO, double, true, true>(st, cf, g.eps, vl, vr);
+            val = g.inv_dsg[je + g.b] * (vr - vl);
+        }
+        g.grad[(long)(E * D + c) * g.ng + o] = val;
+    }
+}
+
+// Face averages of the gradients for the faces of direction DIR
+// (weno_Re_flux), one thread per face: each gradient plane reconstructed
+// along DIR (s_reconstruct_cell_boundary_values_visc_deriv,
+// m_viscous.fpp:471-591) at the upper face of the left cell (dqR_prim_d*_n)
+// and the lower face of the right cell (dqL_prim_d*_n), the bc = -4 copies
+// of s_populate_riemann_states_variables_buffers (m_riemann_solvers.fpp:
+// 488-608, with the caller's L/R swap), then 5e-1*(R + L) as
+// s_compute_viscous_source_flux averages them (:683-902).
+template <int NF, int D, int DIR, int WO>
+__global__ __launch_bounds__(256) void k_visc_face_grad(ViscArgs a) {
+    constexpr int P = (WO - 1) / 2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nface) return;
+    int x, y, f, s;
+    if constexpr (DIR == 0) {
+        f = (int)(t % (a.m + 2)) - 1;
+        y = (int)(t / (a.m + 2));
+        x = f;
+        s = a.m;
+    } else {
+        x = (int)(t % (a.m + 1));
+        f = (int)(t / (a.m + 1)) - 1;
+        y = f;
+        s = a.n;
+    }
+    const long step = DIR == 0 ? 1 : a.nxg;
+    const long o0 = (long)(x + a.b) + a.nxg * (long)(y + a.bj); // cell f of the face's pair
+    double cf0[kNCoef], cf1[kNCoef];
+#pragma unroll
+    for (int q = 0; q < kNCoef; ++q) {
+        cf0[q] = a.coef[(long)q * a.ncell + (f + a.b)];
+        cf1[q] = a.coef[(long)q * a.ncell + (f + 1 + a.b)];
+    }
+#pragma unroll
+    for (int pl = 0; pl < D * D; ++pl) {
+        const double *G = a.grad + (long)pl * a.ng;
+        double st[WO], dummy, R, L;
+#pragma unroll
+        for (int r = 0; r < WO; ++r) st[r] = G[o0 + (r - P) * step];
+        weno_cell<WO, double, false, true>(st, cf0, a.eps, dummy, R);
+#pragma unroll
+        for (int r = 0; r < WO; ++r) st[r] = G[o0 + (r + 1 - P) * step];
+        weno_cell<WO, double, true, false>(st, cf1, a.eps, L, dummy);
+        if (f == -1 && a.bc_beg == -4) R = L;
+        if (f == s && a.bc_end == -4) L = R;
+        a.fgrad[(long)pl * a.nface + t] = 5e-1 * (R + L);
+    }
+}
+
 // x faces: one wave per 63 faces f0 .. f0+62 of row y (f0 = -1 + 63*tile).
 // Lane l converts stencil cells into LDS once, then reconstructs both faces
 // of cell c = f0 + l (the smoothness indicators shared, as in the
@@ -163,7 +263,7 @@ __device__ __forceinline__ void store_face(const ViscArgs &a, long t, const doub
 // value of cell c+1, from the next lane.  No register cap: the kernel takes
 // 230 VGPRs (2 waves per SIMD) for two fluids in 2D; capped at 4 or 3 waves
 // it spills and is slower.
-template <int NF, int D, int WO, bool UNI>
+template <int NF, int D, int WO, bool UNI, bool WRF>
 __global__ __launch_bounds__(64) void k_visc_face_x(ViscArgs a, int ntile) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
@@ -202,7 +302,7 @@ __global__ __launch_bounds__(64) void k_visc_face_x(ViscArgs a, int ntile) {
     // velocities of rows y-1 and y+1 of this lane's cell, then of the next
     // lane's (2D transverse gradients)
     double gt[2] = {0.0, 0.0};
-    if constexpr (D == 2) {
+    if constexpr (D == 2 && !WRF) {
         double um[2], up[2], um1[2], up1[2];
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
@@ -234,14 +334,21 @@ __global__ __launch_bounds__(64) void k_visc_face_x(ViscArgs a, int ntile) {
 #pragma unroll
         for (int v = 0; v < S; ++v) qr[v] = ql[v];
     }
-    // one-sided x differences across the face (dqR_prim_dx_n(1) at x)
-    double gxn[2] = {0.0, 0.0};
-    const double dxf = a.ccx[f + 1 + a.b] - a.ccx[f + a.b];
+    const long t = (long)y * (a.m + 2) + (f + 1);
+    double ax[2], ay[2];
+    if constexpr (WRF) {
+        weno_averages<D>(a, t, ax, ay);
+    } else {
+        // one-sided x differences across the face (dqR_prim_dx_n(1) at x)
+        double gxn[2] = {0.0, 0.0};
+        const double dxf = a.ccx[f + 1 + a.b] - a.ccx[f + a.b];
 #pragma unroll
-    for (int d = 0; d < D; ++d) gxn[d] = (sp[Y::MOM + d][lane + P + 1] - sp[Y::MOM + d][lane + P]) / dxf;
+        for (int d = 0; d < D; ++d) gxn[d] = (sp[Y::MOM + d][lane + P + 1] - sp[Y::MOM + d][lane + P]) / dxf;
+        fd_averages<D, 0>(gxn, gt, ax, ay);
+    }
     double flux[S], ustar, fs[D + 1];
-    face_flux<NF, D, 0>(a, ql, qr, gxn, gt, flux, ustar, fs);
-    store_face<NF, D>(a, (long)y * (a.m + 2) + (f + 1), flux, ustar, fs);
+    face_flux<NF, D, 0>(a, ql, qr, ax, ay, flux, ustar, fs);
+    store_face<NF, D>(a, t, flux, ustar, fs);
 }
 
 // y faces (2D): thread = x column, marching along y over the L faces
@@ -252,7 +359,7 @@ __global__ __launch_bounds__(64) void k_visc_face_x(ViscArgs a, int ntile) {
 // per-face WENO, HLLC and viscous arithmetic (the window in an LDS ring
 // leaves the count unchanged); capped at 2, 3 or 4 waves it spills (0.1-0.9
 // KB per lane) and is slower.
-template <int NF, int WO, int L, bool UNI>
+template <int NF, int WO, int L, bool UNI, bool WRF>
 __global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
     constexpr int D = 2;
     using Y = Sys<NF, D>;
@@ -285,12 +392,14 @@ __global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
         weno_cell<WO, double, false, true>(st, cf, a.eps, dummy, ql[v]);
     }
     // row f0: own velocity and the x neighbours'
-    double uo[2], uw[2], ue[2];
+    double uo[2], uw[2] = {0.0, 0.0}, ue[2] = {0.0, 0.0};
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
         uo[d] = w[P][Y::MOM + d];
-        uw[d] = cell_vel<NF, D>(a, x - 1, min(f0, hi), d);
-        ue[d] = cell_vel<NF, D>(a, x + 1, min(f0, hi), d);
+        if constexpr (!WRF) {
+            uw[d] = cell_vel<NF, D>(a, x - 1, min(f0, hi), d);
+            ue[d] = cell_vel<NF, D>(a, x + 1, min(f0, hi), d);
+        }
     }
 #pragma unroll 1
     for (int k = 0; k < L; ++k) {
@@ -310,12 +419,14 @@ __global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
             for (int r = 0; r < W; ++r) st[r] = w[r][v];
             weno_cell<WO, double, true, true>(st, cf, a.eps, qr[v], qn[v]);
         }
-        double no[2], nw[2], ne[2];
+        double no[2], nw[2] = {0.0, 0.0}, ne[2] = {0.0, 0.0};
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
             no[d] = w[P][Y::MOM + d];
-            nw[d] = cell_vel<NF, D>(a, x - 1, min(f + 1, hi), d);
-            ne[d] = cell_vel<NF, D>(a, x + 1, min(f + 1, hi), d);
+            if constexpr (!WRF) {
+                nw[d] = cell_vel<NF, D>(a, x - 1, min(f + 1, hi), d);
+                ne[d] = cell_vel<NF, D>(a, x + 1, min(f + 1, hi), d);
+            }
         }
         if (f <= a.n) {
             double l2[S], r2[S];
@@ -324,20 +435,27 @@ __global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
                 l2[v] = (f == -1 && a.bc_beg == -4) ? qr[v] : ql[v];
                 r2[v] = (f == a.n && a.bc_end == -4) ? ql[v] : qr[v];
             }
-            const double dyf = a.ccy[f + 1 + a.b] - a.ccy[f + a.b];
-            double gxn[2], gt[2];
+            const long t = (long)(f + 1) * (a.m + 1) + x;
+            double ax[2], ay[2];
+            if constexpr (WRF) {
+                weno_averages<D>(a, t, ax, ay);
+            } else {
+                const double dyf = a.ccy[f + 1 + a.b] - a.ccy[f + a.b];
+                double gxn[2], gt[2];
 #pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                gxn[d] = (no[d] - uo[d]) / dyf;
-                // four-point average of the one-sided x differences of rows
-                // f+1 and f (dqR_prim_dx_n(2) at f, m_viscous.fpp:285-315)
-                const double sum = (no[d] - nw[d]) / dxm + (ne[d] - no[d]) / dxp + (uo[d] - uw[d]) / dxm +
-                                   (ue[d] - uo[d]) / dxp;
-                gt[d] = 25e-2 * sum;
+                for (int d = 0; d < 2; ++d) {
+                    gxn[d] = (no[d] - uo[d]) / dyf;
+                    // four-point average of the one-sided x differences of rows
+                    // f+1 and f (dqR_prim_dx_n(2) at f, m_viscous.fpp:285-315)
+                    const double sum = (no[d] - nw[d]) / dxm + (ne[d] - no[d]) / dxp + (uo[d] - uw[d]) / dxm +
+                                       (ue[d] - uo[d]) / dxp;
+                    gt[d] = 25e-2 * sum;
+                }
+                fd_averages<D, 1>(gxn, gt, ax, ay);
             }
             double flux[S], ustar, fs[D + 1];
-            face_flux<NF, D, 1>(a, l2, r2, gxn, gt, flux, ustar, fs);
-            store_face<NF, D>(a, (long)(f + 1) * (a.m + 1) + x, flux, ustar, fs);
+            face_flux<NF, D, 1>(a, l2, r2, ax, ay, flux, ustar, fs);
+            store_face<NF, D>(a, t, flux, ustar, fs);
         }
 #pragma unroll
         for (int v = 0; v < S; ++v) ql[v] = qn[v];
@@ -406,24 +524,31 @@ __global__ __launch_bounds__(256) void k_visc_finish(ViscArgs a, SweepArgs s, lo
 
 constexpr int kYChunk = 8; // y faces per thread of k_visc_face_y (16 and 32: 1 % and 4 % slower at 2048^2)
 
-template <int NF, int D, int DIR, int WO, bool UNI>
+template <int NF, int D, int DIR, int WO, bool UNI, bool WRF>
 void launch_face_u(const ViscArgs &a, hipStream_t st) {
     if constexpr (DIR == 0) {
         const int ntile = (a.m + 2 + 62) / 63;
-        hipLaunchKernelGGL((k_visc_face_x<NF, D, WO, UNI>), dim3((unsigned)((long)ntile * (a.n + 1))), dim3(64), 0,
-                           st, a, ntile);
+        hipLaunchKernelGGL((k_visc_face_x<NF, D, WO, UNI, WRF>), dim3((unsigned)((long)ntile * (a.n + 1))), dim3(64),
+                           0, st, a, ntile);
     } else {
         const int nxb = (a.m + 1 + 255) / 256;
         const int nchunk = (a.n + 2 + kYChunk - 1) / kYChunk;
-        hipLaunchKernelGGL((k_visc_face_y<NF, WO, kYChunk, UNI>), dim3((unsigned)((long)nxb * nchunk)), dim3(256), 0,
-                           st, a, nxb);
+        hipLaunchKernelGGL((k_visc_face_y<NF, WO, kYChunk, UNI, WRF>), dim3((unsigned)((long)nxb * nchunk)),
+                           dim3(256), 0, st, a, nxb);
     }
 }
 
 template <int NF, int D, int DIR, int WO>
 void launch_face(const ViscArgs &a, hipStream_t st) {
-    if (a.uniform) launch_face_u<NF, D, DIR, WO, true>(a, st);
-    else launch_face_u<NF, D, DIR, WO, false>(a, st);
+    if (a.wrf) {
+        hipLaunchKernelGGL((k_visc_face_grad<NF, D, DIR, WO>), dim3((unsigned)((a.nface + 255) / 256)), dim3(256), 0,
+                           st, a);
+        if (a.uniform) launch_face_u<NF, D, DIR, WO, true, true>(a, st);
+        else launch_face_u<NF, D, DIR, WO, false, true>(a, st);
+    } else {
+        if (a.uniform) launch_face_u<NF, D, DIR, WO, true, false>(a, st);
+        else launch_face_u<NF, D, DIR, WO, false, false>(a, st);
+    }
 }
 
 template <int NF, int D, int DIR>
@@ -449,7 +574,37 @@ int launch_nf(int D, int dir, int weno_order, const ViscArgs &a, const SweepArgs
     return -1;
 }
 
+template <int NF, int D, int E>
+int launch_grad_e(int weno_order, const ViscGradArgs &g, hipStream_t st) {
+    const dim3 grid((unsigned)((g.ng + 255) / 256));
+    switch (weno_order) {
+    case 1: hipLaunchKernelGGL((k_visc_grad<NF, D, E, 1>), grid, dim3(256), 0, st, g); break;
+    case 3: hipLaunchKernelGGL((k_visc_grad<NF, D, E, 3>), grid, dim3(256), 0, st, g); break;
+    case 5: hipLaunchKernelGGL((k_visc_grad<NF, D, E, 5>), grid, dim3(256), 0, st, g); break;
+    default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int NF>
+int launch_grad_nf(int D, int e, int weno_order, const ViscGradArgs &g, hipStream_t st) {
+    if (D == 1 && e == 0) return launch_grad_e<NF, 1, 0>(weno_order, g, st);
+    if (D == 2 && e == 0) return launch_grad_e<NF, 2, 0>(weno_order, g, st);
+    if (D == 2 && e == 1) return launch_grad_e<NF, 2, 1>(weno_order, g, st);
+    return -1;
+}
+
 } // namespace
+
+int launch_visc_grad(int nf, int D, int e, int weno_order, const ViscGradArgs &g, hipStream_t st) {
+    switch (nf) {
+    case 1: return launch_grad_nf<1>(D, e, weno_order, g, st);
+    case 2: return launch_grad_nf<2>(D, e, weno_order, g, st);
+    case 3: return launch_grad_nf<3>(D, e, weno_order, g, st);
+    case 4: return launch_grad_nf<4>(D, e, weno_order, g, st);
+    default: return -1;
+    }
+}
 
 int launch_viscous(int nf, int D, int dir, int weno_order, const ViscArgs &a, const SweepArgs &sa,
                    hipStream_t st) {

@@ -109,6 +109,7 @@ contains
             prm%Re(:, i) = fluid_pp(i)%Re(:)
         end do
         prm%precision = bulk_precision
+        prm%weno_Re_flux = merge(1, 0, weno_Re_flux)
     end function f_mfc_params
 
     !> Number of doubles per variable of a ghosted (or interior) field.

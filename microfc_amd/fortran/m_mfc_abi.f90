@@ -26,6 +26,7 @@ module m_mfc_abi
         real(c_double) :: pi_infs(mfc_num_fluids_max)
         integer(c_int) :: precision
         real(c_double) :: Re(2, mfc_num_fluids_max)
+        integer(c_int) :: weno_Re_flux
     end type mfc_params
 
     interface

@@ -53,11 +53,6 @@ contains
         if (num_fluids < 1 .or. num_fluids > num_fluids_max) call s_abort('Unsupported value of num_fluids. Exiting ...')
         if (probe_wrt) call s_abort('probe_wrt is not supported by this build. Exiting ...')
         if (cu_mpi) call s_abort('Unsupported value of cu_mpi. Exiting ...')
-        ! Viscous terms run with finite-difference velocity gradients; the
-        ! WENO-reconstructed gradients of weno_Re_flux (m_viscous.fpp:380-600)
-        ! are not built.
-        if (weno_Re_flux .and. any(fluid_pp(1:num_fluids)%Re(1) > 0d0 .or. fluid_pp(1:num_fluids)%Re(2) > 0d0)) &
-            call s_abort('weno_Re_flux is not supported by this build (use finite-difference gradients). Exiting ...')
         if (n == 0 .and. bc_y%beg == dflt_int) then
             bc_y%beg = -3; bc_y%end = -3
         end if

@@ -208,20 +208,26 @@ def shear_layer_2d(N: int = 4096, weno_order: int = 5, seed: int = 0, modes: int
     return prm, grids, ghosted(prm, prim_to_cons(prm, prim))
 
 
-def viscous_shear_case(N: int = 2048, weno_order: int = 5, Re: float = 1e-4) -> dict:
+def viscous_shear_case(N: int = 2048, weno_order: int = 5, Re: float = 1e-4, weno_Re_flux: bool = True) -> dict:
     """The reference's examples/2D_viscous/case.py as a case dictionary at
-    N^2 cells: [-0.5, 0.5]^2, two water-like stiffened gases (gamma 4.4,
-    p_inf 6e8) with shear and volume Reynolds numbers Re; patch 1 fills the
-    domain with both fluids at alpha = 0.5, alpha_rho = 1000 each, u = +500;
-    patch 2 (centred at y = 0.25, length 0.5, allowed to overwrite patch 1)
-    holds fluid 2 alone at alpha_rho = 1, u = -500; p = 1e6 everywhere.
-    Periodic in x; bc -6 in y (extrapolation in the ghost fill).  The
-    example's weno_Re_flux = T is left out: the viscous terms use
-    finite-difference gradients."""
+    N^2 cells (N = 51 is the example itself): [-0.5, 0.5]^2, two water-like
+    stiffened gases (gamma 4.4, p_inf 6e8) with shear and volume Reynolds
+    numbers Re; patch 1 fills the domain with both fluids at alpha = 0.5,
+    alpha_rho = 1000 each, u = +500; patch 2 (centred at y = 0.25, length
+    0.5, allowed to overwrite patch 1) holds fluid 2 alone at alpha_rho = 1,
+    u = -500; p = 1e6 everywhere.  Periodic in x; bc -6 in y (extrapolation
+    in the ghost fill).  The example's logistics and algorithm keys come
+    along: dt 5e-10, 10 RK3 steps saved every step, run_time_info,
+    parallel_io and weno_Re_flux (WENO-reconstructed velocity gradients;
+    False selects finite differences)."""
     g, pi = eos_inputs(4.4, 6.0e8)
-    c = {"m": N - 1, "n": N - 1, "x_domain%beg": -0.5, "x_domain%end": 0.5, "y_domain%beg": -0.5,
-         "y_domain%end": 0.5, "num_fluids": 2, "num_patches": 2, "weno_order": weno_order,
-         "bc_x%beg": -1, "bc_x%end": -1, "bc_y%beg": -6, "bc_y%end": -6}
+    c = {"case_dir": "'.'", "run_time_info": "T",
+         "x_domain%beg": -0.5, "x_domain%end": 0.5, "y_domain%beg": -0.5, "y_domain%end": 0.5,
+         "m": N - 1, "n": N - 1, "dt": 5e-10, "t_step_start": 0, "t_step_stop": 10, "t_step_save": 1,
+         "num_patches": 2, "num_fluids": 2, "time_stepper": 3, "weno_order": weno_order, "weno_eps": 1e-16,
+         "weno_Re_flux": "T" if weno_Re_flux else "F",
+         "bc_x%beg": -1, "bc_x%end": -1, "bc_y%beg": -6, "bc_y%end": -6,
+         "format": 1, "precision": 2, "prim_vars_wrt": "T", "parallel_io": "T"}
     patches = [dict(geometry=3, x_centroid=0.0, y_centroid=0.0, length_x=1.0, length_y=1.0,
                     **{"vel(1)": 500.0, "vel(2)": 0.0, "pres": 1e6, "alpha_rho(1)": 1000.0,
                        "alpha_rho(2)": 1000.0, "alpha(1)": 0.5, "alpha(2)": 0.5}),
@@ -237,11 +243,11 @@ def viscous_shear_case(N: int = 2048, weno_order: int = 5, Re: float = 1e-4) -> 
     return c
 
 
-def viscous_shear_2d(N: int = 2048, weno_order: int = 5, Re: float = 1e-4):
+def viscous_shear_2d(N: int = 2048, weno_order: int = 5, Re: float = 1e-4, weno_Re_flux: bool = True):
     """The reference's viscous example at N^2 cells (viscous_shear_case),
     laid out by the pre-processor's patch rules."""
     from .pre_process import PreProcess
-    return PreProcess(viscous_shear_case(N, weno_order, Re)).state()
+    return PreProcess(viscous_shear_case(N, weno_order, Re, weno_Re_flux)).state()
 
 
 def viscous_dt(prm: Params, grids, q, safety: float = 0.2) -> float:

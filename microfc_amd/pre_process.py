@@ -244,7 +244,8 @@ class PreProcess:
             Re.append(tuple(float(g(f"fluid_pp({f + 1})%Re({r + 1})", -1.0)) for r in range(2)))
         prm = Params(m=self.m, n=self.n, num_fluids=self.num_fluids, weno_order=int(g("weno_order", 5)),
                      gammas=self.gammas, pi_infs=self.pi_infs, bc_beg=bcb, bc_end=bce,
-                     Re=Re if any(r > 0 for pair in Re for r in pair) else None)
+                     Re=Re if any(r > 0 for pair in Re for r in pair) else None,
+                     weno_Re_flux=_logical(g("weno_Re_flux", "F")))
         return prm
 
     def grids(self):

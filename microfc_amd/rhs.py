@@ -61,6 +61,7 @@ class CParams(ctypes.Structure):
         ("pi_infs", ctypes.c_double * NUM_FLUIDS_MAX),
         ("precision", ctypes.c_int),
         ("Re", (ctypes.c_double * 2) * NUM_FLUIDS_MAX),
+        ("weno_Re_flux", ctypes.c_int),
     ]
 
 
@@ -129,6 +130,9 @@ class Params:
     # fluid_pp(i)%Re(1:2) per fluid; entries <= 0 leave the fluid out, None
     # (or no positive entry) is inviscid
     Re: list = None
+    # viscous velocity gradients: WENO face values by the divergence theorem
+    # (True) or finite differences (False)
+    weno_Re_flux: bool = False
 
     @property
     def viscous(self) -> bool:
@@ -186,6 +190,7 @@ class Params:
         for i in range(NUM_FLUIDS_MAX):
             for r in range(2):
                 c.Re[i][r] = self.Re[i][r] if self.Re and i < len(self.Re) else -1e6
+        c.weno_Re_flux = 1 if self.weno_Re_flux else 0
         return c
 
 

@@ -610,8 +610,9 @@ int orc_bc_fill(const orc_case *c, double *q) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* Viscous terms with finite-difference velocity gradients (weno_Re_flux = F):
- * m_viscous.fpp:159-370 (gradients), m_riemann_solvers.fpp:169-231 (face
+/* Viscous terms: m_viscous.fpp:159-591 (velocity gradients, by finite
+ * differences or, with weno_Re_flux, by the divergence theorem on WENO face
+ * values), m_riemann_solvers.fpp:169-231 (face
  * Reynolds numbers), :683-902 (viscous source flux), m_rhs.fpp:591-603 and
  * :639-651 (its divergence).  1D and 2D, as in the reference.               */
 
@@ -620,10 +621,12 @@ typedef struct {
     int idx[2][ORC_NUM_FLUIDS_MAX];    /* Re_idx, 0-based fluid index */
     double res[2][ORC_NUM_FLUIDS_MAX]; /* Res */
     /* Face velocity gradients, D components of ng doubles each, over the
-     * ghosted grid: Lx1/Rx1 = dqL/dqR_prim_dx_n(1) and Ly2/Ry2 =
-     * dqL/dqR_prim_dy_n(2) (one-sided differences), Lx2/Rx2 = dq*_prim_dx_n(2)
-     * and Ly1/Ry1 = dq*_prim_dy_n(1) (their four-point averages).  Entries the
-     * reference does not compute hold NaN. */
+     * ghosted grid: Lx1/Rx1 = dqL/dqR_prim_dx_n(1), Ly2/Ry2 =
+     * dqL/dqR_prim_dy_n(2), Lx2/Rx2 = dq*_prim_dx_n(2) and Ly1/Ry1 =
+     * dq*_prim_dy_n(1).  By finite differences: one-sided differences for
+     * the first two, their four-point averages for the others; with
+     * weno_Re_flux, WENO face values of the divergence-theorem gradients.
+     * Entries the reference does not compute hold NaN. */
     double *Lx1, *Rx1, *Ly2, *Ry2, *Lx2, *Rx2, *Ly1, *Ry1;
 } visc;
 
@@ -709,6 +712,144 @@ static int visc_gradients(const orc_case *c, const geom *g, const double *qp, co
 #undef U
 #undef AT
     return 0;
+}
+
+/* Stencil of cell j along direction e (0 = x, 1 = y) of plane a (ng doubles
+ * over the ghosted grid) at transverse index t, as pencil() forms it. */
+static void stencil_of(const orc_case *c, const geom *g, const double *a, int e, int j, int t, double *st) {
+    int r;
+    for (r = -2; r <= 2; r++) {
+        if ((c->weno_order == 1 && r != 0) || (c->weno_order == 3 && (r < -1 || r > 1))) {
+            st[r + 2] = 0.0;
+            continue;
+        }
+        st[r + 2] = e == 0 ? a[gidx(g, j + r, t, 0)] : a[gidx(g, t, j + r, 0)];
+    }
+}
+
+/* Velocity gradients with weno_Re_flux = T.
+ * 1. s_get_viscous (m_viscous.fpp:186-214): for each direction e, WENO
+ *    reconstruction along e of the velocity components over cells
+ *    [-b+polyn, s+b-polyn] and every transverse index
+ *    (s_reconstruct_cell_boundary_values_visc, :400-469), then the scalar
+ *    divergence theorem (s_apply_scalar_divergence_theorem, :372-398):
+ *    dq_prim_de_qp(j) = 1/ds(j) * (vR(j) - vL(j)) over [-b+1, s+b-1].
+ * 2. In each direction d of s_compute_rhs (m_rhs.fpp:514-528), WENO
+ *    reconstruction along d of dq_prim_dx_qp and dq_prim_dy_qp
+ *    (s_reconstruct_cell_boundary_values_visc_deriv, m_viscous.fpp:471-591)
+ *    into dqL/dqR_prim_dx_n(d) and dqL/dqR_prim_dy_n(d): L is the value at
+ *    the cell's lower face, R at its upper face.
+ * The theorem's outermost cells -b+1..-b+polyn-1 (and their mirror) read
+ * face values step 1 never writes, and cells -b, s+b are never written; they
+ * hold NaN here.  Their reconstructions reach only cells whose faces the
+ * fluxes at -1/2 .. s+1/2 do not use (a NaN reaching an output would show). */
+static int visc_gradients_weno(const orc_case *c, const geom *g, const double *qp, const double *const cb[3],
+                               const double *const ds[3], visc *w) {
+    const long tot = (long)g->D * g->ng;
+    const int b = g->b, D = g->D, polyn = (c->weno_order - 1) / 2;
+    const int ext[2] = {g->m, g->n};
+    double **arr[8] = {&w->Lx1, &w->Rx1, &w->Ly2, &w->Ry2, &w->Lx2, &w->Rx2, &w->Ly1, &w->Ry1};
+    double *G[2] = {NULL, NULL}, *coef[2] = {NULL, NULL}, *vel = NULL;
+    int a, e, d, i, j, t, rc = 0;
+    long o;
+    for (a = 0; a < (D > 1 ? 8 : 2); a++) {
+        *arr[a] = (double *)malloc(sizeof(double) * tot);
+        if (!*arr[a]) return -2;
+        for (o = 0; o < tot; o++) (*arr[a])[o] = NAN;
+    }
+    vel = (double *)malloc(sizeof(double) * tot);
+    for (e = 0; e < D; e++) {
+        const long nc = ext[e] + 2 * b + 1;
+        G[e] = (double *)malloc(sizeof(double) * tot);
+        coef[e] = (double *)malloc(sizeof(double) * 27 * nc);
+        if (!G[e] || !coef[e]) rc = -2;
+        else {
+            for (o = 0; o < tot; o++) G[e][o] = NAN;
+            orc_weno_coefficients(ext[e], b, c->weno_order, cb[e], coef[e]);
+        }
+    }
+    if (!vel || rc) goto done;
+    /* the velocity components of the primitive field */
+    for (i = 0; i < D; i++)
+        for (o = 0; o < g->ng; o++) vel[i * g->ng + o] = qp[(long)(g->momxb + i) * g->ng + o];
+    /* 1. reconstruction along e and the divergence theorem */
+    for (e = 0; e < D; e++) {
+        const int s = ext[e], tl = D > 1 ? -b : 0, th = D > 1 ? ext[1 - e] + b : 0;
+        const long nc = s + 2 * b + 1;
+        for (i = 0; i < D; i++)
+            for (t = tl; t <= th; t++)
+                for (j = -b + polyn; j <= s + b - polyn; j++) {
+                    double cf[27], st[5], vL, vR;
+                    int q;
+                    if (j < -b + 1 || j > s + b - 1) continue;
+                    for (q = 0; q < 27; q++) cf[q] = coef[e][q * nc + j + b];
+                    stencil_of(c, g, vel + (long)i * g->ng, e, j, t, st);
+                    orc_weno_cell(c->weno_order, c->weno_eps, cf, st, &vL, &vR);
+                    G[e][(long)i * g->ng + (e == 0 ? gidx(g, j, t, 0) : gidx(g, t, j, 0))] =
+                        1.0 / ds[e][j + b] * (vR - vL);
+                }
+    }
+    /* 2. reconstruction of each gradient along each sweep direction d */
+    for (d = 0; d < D; d++) {
+        const int s = ext[d], tl = D > 1 ? -b : 0, th = D > 1 ? ext[1 - d] + b : 0;
+        const long nc = s + 2 * b + 1;
+        for (e = 0; e < D; e++) {
+            /* dq*_prim_dx_n(1) = Lx1/Rx1, dq*_prim_dy_n(1) = Ly1/Ry1,
+             * dq*_prim_dx_n(2) = Lx2/Rx2, dq*_prim_dy_n(2) = Ly2/Ry2 */
+            double *L = d == 0 ? (e == 0 ? w->Lx1 : w->Ly1) : (e == 0 ? w->Lx2 : w->Ly2);
+            double *R = d == 0 ? (e == 0 ? w->Rx1 : w->Ry1) : (e == 0 ? w->Rx2 : w->Ry2);
+            for (i = 0; i < D; i++)
+                for (t = tl; t <= th; t++)
+                    for (j = -b + polyn; j <= s + b - polyn; j++) {
+                        double cf[27], st[5];
+                        const long at = (long)i * g->ng + (d == 0 ? gidx(g, j, t, 0) : gidx(g, t, j, 0));
+                        int q;
+                        for (q = 0; q < 27; q++) cf[q] = coef[d][q * nc + j + b];
+                        stencil_of(c, g, G[e] + (long)i * g->ng, d, j, t, st);
+                        orc_weno_cell(c->weno_order, c->weno_eps, cf, st, &L[at], &R[at]);
+                    }
+        }
+    }
+done:
+    free(vel);
+    for (e = 0; e < 2; e++) { free(G[e]); free(coef[e]); }
+    return rc;
+}
+
+/* The gradient part of s_populate_riemann_states_variables_buffers for
+ * bc = -4 (m_riemann_solvers.fpp:488-508, 524-544, 560-576, 592-608): with
+ * the caller's swap (the solver's dqL_prim_d*_vf are dqR_prim_d*_n), the
+ * upper-face value of cell -1 takes the lower-face value of cell 0 and the
+ * lower-face value of cell s+1 the upper-face value of cell s, over the
+ * interior transverse range.  (With finite-difference gradients both sides
+ * already hold the same value.) */
+static void visc_riemann_buffers(const orc_case *c, const geom *g, visc *w) {
+    int i, t;
+#define AT(A, xx, yy) A[(long)i * g->ng + gidx(g, xx, yy, 0)]
+    for (i = 0; i < g->D; i++) {
+        for (t = 0; t <= g->n; t++) {
+            if (c->bc_beg[0] == -4) {
+                AT(w->Rx1, -1, t) = AT(w->Lx1, 0, t);
+                if (g->D > 1) AT(w->Ry1, -1, t) = AT(w->Ly1, 0, t);
+            }
+            if (c->bc_end[0] == -4) {
+                AT(w->Lx1, g->m + 1, t) = AT(w->Rx1, g->m, t);
+                if (g->D > 1) AT(w->Ly1, g->m + 1, t) = AT(w->Ry1, g->m, t);
+            }
+        }
+        if (g->D == 1) continue;
+        for (t = 0; t <= g->m; t++) {
+            if (c->bc_beg[1] == -4) {
+                AT(w->Rx2, t, -1) = AT(w->Lx2, t, 0);
+                AT(w->Ry2, t, -1) = AT(w->Ly2, t, 0);
+            }
+            if (c->bc_end[1] == -4) {
+                AT(w->Lx2, t, g->n + 1) = AT(w->Rx2, t, g->n);
+                AT(w->Ly2, t, g->n + 1) = AT(w->Ry2, t, g->n);
+            }
+        }
+    }
+#undef AT
 }
 
 /* Harmonic face Reynolds numbers Re_avg(1:2) from the reconstructed volume
@@ -929,7 +1070,10 @@ int orc_compute_rhs(const orc_case *c, const double *const cb[3], const double *
             cc[dir] = (double *)malloc(sizeof(double) * (s + 2 * g.b + 1));
             cell_centres(s, g.b, cb[dir], ds[dir], cc[dir]);
         }
-        if (rc == 0) rc = visc_gradients(c, &g, qp, cc[0], cc[1], &w);
+        if (rc == 0)
+            rc = c->weno_Re_flux ? visc_gradients_weno(c, &g, qp, cb, ds, &w)
+                                 : visc_gradients(c, &g, qp, cc[0], cc[1], &w);
+        if (rc == 0) visc_riemann_buffers(c, &g, &w);
         free(cc[0]);
         free(cc[1]);
         if (rc != 0) { visc_free(&w); free(qc); free(qp); return rc; }

@@ -13,7 +13,7 @@
  *   src/simulation/m_time_steppers.fpp  (TVD RK1/2/3)
  *   src/simulation/m_start_up.fpp       (grid ghost buffers)
  *   src/simulation/m_data_output.fpp    (ICFL)
- *   src/simulation/m_viscous.fpp        (velocity gradients, weno_Re_flux = F)
+ *   src/simulation/m_viscous.fpp        (velocity gradients, weno_Re_flux = F and T)
  * keeping the reference's evaluation order operation by operation.  The 3D
  * case is the dimension-split generalisation (the reference is 1D/2D only).
  *
@@ -45,9 +45,12 @@ typedef struct {
     int bc_beg[3], bc_end[3];    /* bc_x%beg, bc_x%end, bc_y..., bc_z... */
     /* fluid_pp(i)%Re(1:2): shear and volume Reynolds numbers; a value <= 0
      * leaves fluid i out of that viscous term (m_global_parameters.fpp:314-338).
-     * Any positive entry makes the case viscous (1D and 2D only, finite-difference
-     * velocity gradients, i.e. weno_Re_flux = F). */
+     * Any positive entry makes the case viscous (1D and 2D only). */
     double Re[ORC_NUM_FLUIDS_MAX][2];
+    /* weno_Re_flux: velocity gradients from WENO-reconstructed face values
+     * by the divergence theorem, then WENO-reconstructed to the faces
+     * (nonzero), or finite differences (0). */
+    int weno_Re_flux;
 } orc_case;
 
 /* Host threads of the pencil and update loops (OpenMP builds; the results do

@@ -36,6 +36,7 @@ class _Case(ctypes.Structure):
         ("pi_infs", ctypes.c_double * NUM_FLUIDS_MAX),
         ("bc_beg", ctypes.c_int * 3), ("bc_end", ctypes.c_int * 3),
         ("Re", (ctypes.c_double * 2) * NUM_FLUIDS_MAX),
+        ("weno_Re_flux", ctypes.c_int),
     ]
 
 
@@ -96,6 +97,7 @@ class Case:
     bc_beg: tuple = (-3, -3, -3)
     bc_end: tuple = (-3, -3, -3)
     Re: list = None  # per fluid (Re(1), Re(2)); a value <= 0 leaves the fluid out
+    weno_Re_flux: bool = False  # velocity gradients from WENO face values (True) or finite differences
 
     @property
     def viscous(self):
@@ -146,6 +148,7 @@ class Case:
         for i in range(NUM_FLUIDS_MAX):
             for r in range(2):
                 s.Re[i][r] = self.Re[i][r] if self.Re and i < len(self.Re) else -1e6
+        s.weno_Re_flux = 1 if self.weno_Re_flux else 0
         return s
 
 

@@ -19,7 +19,8 @@ def to_oracle(prm: Params) -> O.Case:
     return O.Case(m=prm.m, n=prm.n, p=prm.p, num_fluids=prm.num_fluids, weno_order=prm.weno_order,
                   time_stepper=prm.time_stepper, t_step_stop=prm.t_step_stop, weno_eps=prm.weno_eps,
                   gammas=list(prm.gammas), pi_infs=list(prm.pi_infs), bc_beg=tuple(prm.bc_beg),
-                  bc_end=tuple(prm.bc_end), Re=[tuple(r) for r in prm.Re] if prm.Re else None)
+                  bc_end=tuple(prm.bc_end), Re=[tuple(r) for r in prm.Re] if prm.Re else None,
+                  weno_Re_flux=prm.weno_Re_flux)
 
 
 def rel_linf(a, b):

@@ -69,15 +69,6 @@ def test_fortran_driver_rejects_invalid_case(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.exists(SIM), reason="Fortran host not built")
-def test_fortran_driver_rejects_weno_re_flux(tmp_path):
-    """Viscous cases run with finite-difference gradients; the WENO-gradient
-    branch (weno_Re_flux) is refused with a message."""
-    _write_case(tmp_path, **{"fluid_pp(1)%Re(1)": 100.0, "weno_Re_flux": True})
-    r = subprocess.run([SIM], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
-    assert r.returncode != 0 and "weno_Re_flux" in r.stdout
-
-
-@pytest.mark.skipif(not os.path.exists(SIM), reason="Fortran host not built")
 def test_fortran_driver_passes_reynolds_numbers(tmp_path):
     """fluid_pp%Re reaches mfc_rhs_init through the bind(C) struct: a viscous
     case in the FP32-bulk form is refused by the library's input check (which

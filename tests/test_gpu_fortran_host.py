@@ -9,6 +9,7 @@ import pytest
 
 from microfc_amd import case as C
 from microfc_amd import ic
+from microfc_amd.pre_process import PreProcess
 from oracle import oracle as O
 
 from helpers import assert_parity, perturbed, reghost, say, to_oracle
@@ -144,11 +145,41 @@ def test_fortran_driver_viscous_run_time_information(gpu, tmp_path):
 
 
 def test_fortran_driver_reference_viscous_example(gpu, tmp_path):
-    """examples/2D_viscous/case.py as the reference runs it (m = n = 50,
-    dt = 5e-10, 10 steps, Re 1e-4 for both fluids, periodic x) but with
-    finite-difference gradients: the Fortran host's fluid_pp%Re reach the
-    library, and the run equals the oracle's viscous steps."""
-    prm, grids, q = ic.viscous_shear_2d(N=51)
+    """examples/2D_viscous/case.py run unchanged (m = n = 50, dt = 5e-10, 10
+    RK3 steps saved every step, Re 1e-4 for both fluids, periodic x,
+    weno_Re_flux, run_time_info, parallel_io): the whole dictionary through
+    the pre-processor into restart_data/, its simulation keys into
+    simulation.inp.  Every saved state equals the oracle's steps with
+    WENO-reconstructed velocity gradients bit for bit, and run_time.inf has
+    the viscous columns for every step."""
+    case = ic.viscous_shear_case(N=51)
+    prm, grids, q = PreProcess(case).write(str(tmp_path))
+    assert prm.weno_Re_flux and prm.buff_size == 6
+    C.write_inp(C.simulation_inputs(case), str(tmp_path / "simulation.inp"))
+    say("[fortran host] examples/2D_viscous unchanged")
+    r = subprocess.run([SIM], cwd=str(tmp_path), capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    dt, nsteps = case["dt"], case["t_step_stop"]
+    oc = to_oracle(prm)
+    qo = q.copy()
+    q2 = np.zeros_like(qo)
+    for t, d in enumerate(C.dt_sequence(dt, 0, nsteps)):
+        O.tvd_rk_step(oc, grids, d, qo, q2, t_step=t)
+        assert_parity(C.read_parallel(str(tmp_path), t + 1, prm), prm.interior(qo), tol=0.0,
+                      what=f"2D_viscous step {t + 1}")
+    header, rows, footer = _run_time_table(open(tmp_path / "run_time.inf").read())
+    assert len(rows) == nsteps + 1 and all(len(row) == 5 for row in rows)
+    # the WENO gradients are not the finite differences here
+    fd = prm.__class__(**{**prm.__dict__, "weno_Re_flux": False})
+    assert np.max(np.abs(_oracle_steps(fd, grids, q, dt, nsteps, nsteps)[2] - prm.interior(qo)[2])) > 0.0
+
+
+def test_fortran_driver_viscous_fd_gradients(gpu, tmp_path):
+    """The viscous example with finite-difference gradients (weno_Re_flux =
+    F) on a perturbed state, serial files: the Fortran host's fluid_pp%Re
+    reach the library, and the run equals the oracle's viscous steps."""
+    prm, grids, q = ic.viscous_shear_2d(N=51, weno_Re_flux=False)
+    assert not prm.weno_Re_flux
     q = perturbed(prm, q)
     dt, nsteps = 5e-10, 10
     _run(tmp_path, prm, grids, q, dt, nsteps, 5)

@@ -50,7 +50,7 @@ def test_viscous_shear_matches_the_reference_example():
     alpha_rho 1000, u = 500; upper half fluid 2 only, alpha_rho 1, u = -500;
     p = 1e6; Re(1:2) = 1e-4 for both fluids; periodic x."""
     prm, grids, q = ic.viscous_shear_2d(N=32)
-    assert prm.Re == [(1e-4, 1e-4), (1e-4, 1e-4)] and prm.viscous
+    assert prm.Re == [(1e-4, 1e-4), (1e-4, 1e-4)] and prm.viscous and prm.weno_Re_flux
     assert prm.bc_beg[0] == prm.bc_end[0] == -1
     qi = prm.interior(q)[:, 0]
     low, high = qi[:, 4, 4], qi[:, 28, 4]  # y = -0.36 and 0.39

@@ -22,10 +22,20 @@ GAMMA = 2.5  # fluid_pp%gamma = 1/(1.4 - 1)
 INTERIOR = np.s_[3:-3]
 
 
-def _case(D, N, Re, nf=1, bc=-3):
-    n = N - 1 if D > 1 else 0
+def _case(D, N, Re, nf=1, bc=-3, wrf=False, Ny=None):
+    n = (Ny or N) - 1 if D > 1 else 0
     return O.Case(m=N - 1, n=n, num_fluids=nf, gammas=[GAMMA] * nf, pi_infs=[0.0] * nf,
-                  bc_beg=(bc,) * 3, bc_end=(bc,) * 3, Re=Re)
+                  bc_beg=(bc,) * 3, bc_end=(bc,) * 3, Re=Re, weno_Re_flux=wrf)
+
+
+# Both gradient branches: finite differences (weno_Re_flux = F) and the
+# divergence theorem on WENO5 face values reconstructed to the faces with
+# WENO5 (weno_Re_flux = T).  For velocities polynomial of degree <= 2 the
+# latter is exact too: every 3-cell candidate of WENO5 reproduces a
+# quadratic's face values whatever the weights, so vR - vL over ds is the
+# derivative at the cell centre and its reconstruction (linear data) the
+# derivative at the face.  The 2D cases use unequal cell widths in x and y.
+WRF = pytest.mark.parametrize("wrf", [False, True], ids=["fd_gradients", "weno_re_flux"])
 
 
 def _centres(case, grid):
@@ -70,70 +80,98 @@ def _viscous_part(case, vel, alphas=None):
     return r[:, 0], X, Y
 
 
-def _check(got, want, scale, tol=1e-9):
-    g = got[INTERIOR, INTERIOR] if got.ndim == 2 and got.shape[0] > 1 else got[..., INTERIOR]
-    w = want[INTERIOR, INTERIOR] if np.ndim(want) == 2 and np.shape(want)[0] > 1 else \
-        (want[..., INTERIOR] if np.ndim(want) else want)
+def _inner(wrf):
+    """Cells clear of the extrapolated ghost layers: the face gradients of
+    weno_Re_flux reach 5 cells further (velocity stencil, then gradient
+    stencil) than the finite differences."""
+    return np.s_[6:-6] if wrf else INTERIOR
+
+
+def _check(got, want, scale, tol=1e-9, inner=INTERIOR):
+    g = got[inner, inner] if got.ndim == 2 and got.shape[0] > 1 else got[..., inner]
+    w = want[inner, inner] if np.ndim(want) == 2 and np.shape(want)[0] > 1 else \
+        (want[..., inner] if np.ndim(want) else want)
     err = np.max(np.abs(g - w)) / scale
     assert err < tol, err
 
 
-def test_1d_quadratic_velocity():
+@WRF
+def test_1d_quadratic_velocity(wrf):
     """u = a x^2: momentum (4/3 mu_s + mu_b) u_xx; the energy term adds the
     HLLC face velocity times the stress, close to d_x(u tau_xx)."""
     a, R1, R2 = 0.3, 0.01, 0.02
-    case = _case(1, 64, Re=[(R1, R2)])
+    case = _case(1, 64, Re=[(R1, R2)], wrf=wrf)
     r, X, _ = _viscous_part(case, [lambda x, y: a * x * x])
     mu = 4.0 / 3.0 / R1 + 1.0 / R2
-    _check(r[1], np.full_like(X, mu * 2 * a), mu * 2 * a)
+    _check(r[1], np.full_like(X, mu * 2 * a), mu * 2 * a, inner=_inner(wrf))
     # d_x(u mu u_x) = mu (u_x^2 + u u_xx) = 6 mu a^2 x^2, to the O(dx^2) of the
     # reconstructed face velocity
-    _check(r[2], 6 * mu * a * a * X * X, 6 * mu * a * a, tol=2e-3)
+    _check(r[2], 6 * mu * a * a * X * X, 6 * mu * a * a, tol=2e-3, inner=_inner(wrf))
     assert np.all(r[0] == 0.0) and np.all(r[3] == 0.0)  # mass and volume fraction untouched
 
 
+def test_1d_quartic_velocity_weno_re_flux():
+    """u = a x^4 with weno_Re_flux: the theorem on WENO5 face values gives
+    G_j = u'(x_j) (point values read as cell averages), whose WENO5 face
+    value is q(x_{j+1/2}) with q = u' - h^2/24 u^(3), so the momentum term is
+    mu (q(x_{j+1/2}) - q(x_{j-1/2}))/h = 12 mu a x^2 exactly at the ideal
+    weights; the nonlinear weights add O(h^2) with a small constant.  The
+    average of the wrong face values, 5e-1 (L(j) + R(j+1)), would add
+    12 mu a h^2, some fifty times that."""
+    a, R1, R2, N = 0.3, 0.01, 0.02, 64
+    case = _case(1, N, Re=[(R1, R2)], wrf=True)
+    r, X, _ = _viscous_part(case, [lambda x, y: a * x ** 4])
+    mu = 4.0 / 3.0 / R1 + 1.0 / R2
+    offset = 12 * mu * a / N ** 2
+    _check(r[1], 12 * mu * a * X * X, offset, tol=0.1, inner=_inner(True))
+
+
+@WRF
 @pytest.mark.parametrize("which", ["couette_x", "couette_y"])
-def test_2d_couette_heating(which):
+def test_2d_couette_heating(which, wrf):
     """u = a y (or v = a x): stresses are uniform, so the momentum terms
     vanish and the energy gains the viscous heating mu_s a^2 exactly."""
     a, R1 = 0.7, 0.05
-    case = _case(2, 40, Re=[(R1, -1.0)])
+    case = _case(2, 40, Re=[(R1, -1.0)], wrf=wrf, Ny=28)
     vel = [lambda x, y: a * y, lambda x, y: 0.0 * x] if which == "couette_x" else \
           [lambda x, y: 0.0 * x, lambda x, y: a * x]
     r, X, Y = _viscous_part(case, vel)
     heat = a * a / R1
-    _check(r[1], 0.0, heat)
-    _check(r[2], 0.0, heat)
-    _check(r[3], np.full_like(X, heat), heat)
+    _check(r[1], 0.0, heat, inner=_inner(wrf))
+    _check(r[2], 0.0, heat, inner=_inner(wrf))
+    _check(r[3], np.full_like(X, heat), heat, inner=_inner(wrf))
 
 
-def test_2d_quadratic_shear_and_normal():
+@WRF
+def test_2d_quadratic_shear_and_normal(wrf):
     """u = a y^2 + c x^2, v = e x^2 + f y^2: momentum
     x: mu_s (2a) + (4/3 mu_s + mu_b)(2c);  y: mu_s (2e) + (4/3 mu_s + mu_b)(2f)."""
     a, c, e, f = 0.4, -0.25, 0.15, 0.35
     R1, R2 = 0.02, 0.03
-    case = _case(2, 40, Re=[(R1, R2)])
+    case = _case(2, 40, Re=[(R1, R2)], wrf=wrf, Ny=28)
     r, X, _ = _viscous_part(case, [lambda x, y: a * y * y + c * x * x, lambda x, y: e * x * x + f * y * y])
     ms, mb = 1.0 / R1, 1.0 / R2
     wx = ms * 2 * a + (4.0 / 3.0 * ms + mb) * 2 * c
     wy = ms * 2 * e + (4.0 / 3.0 * ms + mb) * 2 * f
-    _check(r[1], np.full_like(X, wx), abs(wx))
-    _check(r[2], np.full_like(X, wy), abs(wy))
+    _check(r[1], np.full_like(X, wx), abs(wx), inner=_inner(wrf))
+    _check(r[2], np.full_like(X, wy), abs(wy), inner=_inner(wrf))
 
 
-def test_2d_bilinear_cross_terms():
+@WRF
+def test_2d_bilinear_cross_terms(wrf):
     """u = a x y: tau_xx = 4/3 mu_s a y + mu_b a y, tau_xy = mu_s a x,
     tau_yy = -2/3 mu_s a y + mu_b a y; momentum x: 0, y: mu_s a/3 + mu_b a."""
     a, R1, R2 = 0.6, 0.02, 0.05
-    case = _case(2, 40, Re=[(R1, R2)])
+    case = _case(2, 40, Re=[(R1, R2)], wrf=wrf, Ny=28)
     r, X, _ = _viscous_part(case, [lambda x, y: a * x * y, lambda x, y: 0.0 * x])
     ms, mb = 1.0 / R1, 1.0 / R2
     scale = ms * a
-    _check(r[1], np.zeros_like(X), scale)
-    _check(r[2], np.full_like(X, ms * a / 3.0 + mb * a), scale)
+    _check(r[1], np.zeros_like(X), scale, inner=_inner(wrf))
+    _check(r[2], np.full_like(X, ms * a / 3.0 + mb * a), scale, inner=_inner(wrf))
 
 
-def test_two_fluid_reynolds_mixture():
+@WRF
+def test_two_fluid_reynolds_mixture(wrf):
     """Two fluids at alpha = (0.25, 0.75) with different shear Reynolds
     numbers: the face Reynolds number is 1/(0.25/R_1 + 0.75/R_2), so the
     Couette heating is a^2 (0.25/R_1 + 0.75/R_2); a fluid without Re(1)
@@ -141,20 +179,21 @@ def test_two_fluid_reynolds_mixture():
     a = 0.5
     for Re, mu in (([(0.01, -1.0), (0.04, -1.0)], 0.25 / 0.01 + 0.75 / 0.04),
                    ([(0.01, -1.0), (-1.0, -1.0)], 0.25 / 0.01)):
-        case = _case(2, 32, Re=Re, nf=2)
+        case = _case(2, 32, Re=Re, nf=2, wrf=wrf)
         r, X, _ = _viscous_part(case, [lambda x, y: a * y, lambda x, y: 0.0 * x], alphas=[0.25, 0.75])
         heat = a * a * mu
-        _check(r[2 + 2], np.full_like(X, heat), heat)
+        _check(r[2 + 2], np.full_like(X, heat), heat, inner=_inner(wrf))
 
 
-def test_2d_y_uniform_equals_1d():
+@WRF
+def test_2d_y_uniform_equals_1d(wrf):
     """A 2D field that does not vary in y gives the 1D viscous RHS in every
     row, bit for bit (the extra 2D terms add exact zeros; 16 rows make the y
     spacing, and so every y coefficient, exact)."""
     rng = np.random.default_rng(5)
     N = 48
-    c1 = _case(1, N, Re=[(0.02, 0.03)])
-    c2 = _case(2, N, Re=[(0.02, 0.03)])
+    c1 = _case(1, N, Re=[(0.02, 0.03)], wrf=wrf)
+    c2 = _case(2, N, Re=[(0.02, 0.03)], wrf=wrf)
     c2.n = 15
     g1, g2 = O.uniform_grid(c1), O.uniform_grid(c2)
     b = c1.buff_size

@@ -190,9 +190,12 @@ def test_reference_examples_equal_the_benchmark_generators():
     prm_r, g_r, q_r = ic.sod_1d(m=999)
     assert np.array_equal(q, q_r) and np.array_equal(g[0][0], g_r[0][0])
     gam, pinf = 1.0 / (4.4 - 1.0), 4.4 * 6.0e8 / (4.4 - 1.0)
-    visc = {"m": 50, "n": 50, "x_domain%beg": -0.5, "x_domain%end": 0.5, "y_domain%beg": -0.5,
-            "y_domain%end": 0.5, "num_fluids": 2, "num_patches": 2, "weno_order": 5,
+    visc = {"case_dir": "'.'", "run_time_info": "T", "x_domain%beg": -0.5, "x_domain%end": 0.5,
+            "y_domain%beg": -0.5, "y_domain%end": 0.5, "m": 50, "n": 50, "dt": 5e-10, "t_step_start": 0,
+            "t_step_stop": 10, "t_step_save": 1, "num_patches": 2, "num_fluids": 2, "time_stepper": 3,
+            "weno_order": 5, "weno_eps": 1e-16, "weno_Re_flux": "T",
             "bc_x%beg": -1, "bc_x%end": -1, "bc_y%beg": -6, "bc_y%end": -6,
+            "format": 1, "precision": 2, "prim_vars_wrt": "T", "parallel_io": "T",
             "patch_icpp(1)%geometry": 3, "patch_icpp(1)%x_centroid": 0.0, "patch_icpp(1)%y_centroid": 0.0,
             "patch_icpp(1)%length_x": 1.0, "patch_icpp(1)%length_y": 1.0, "patch_icpp(1)%vel(1)": 500.0,
             "patch_icpp(1)%vel(2)": 0.0, "patch_icpp(1)%pres": 1e6, "patch_icpp(1)%alpha_rho(1)": 1000.0,
@@ -206,7 +209,7 @@ def test_reference_examples_equal_the_benchmark_generators():
         visc[f"fluid_pp({f})%gamma"], visc[f"fluid_pp({f})%pi_inf"] = gam, pinf
         visc[f"fluid_pp({f})%Re(1)"] = visc[f"fluid_pp({f})%Re(2)"] = 1e-4
     prm, g, q = PreProcess(visc).state()
-    assert prm.buff_size == 6 and visc == ic.viscous_shear_case(N=51)
+    assert prm.buff_size == 6 and prm.weno_Re_flux and visc == ic.viscous_shear_case(N=51)
     # patch 2 holds the rows whose centre lies in [0, 0.5] (y_cc from the
     # reference's formula -0.5 + 0.5 dy (2j + 1): exact for j = 25, where it
     # is 0), patch 1 the rest
