@@ -412,6 +412,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
     const int k = (int)(row / (a.n + 1));
     const int lane = threadIdx.x;
     const int x0 = tile * TW;
+    // conservative values of this lane's finish cell x0+lane, taken from the
+    // staging loads (forms that load before converting) so the finish does
+    // not read the stage input again
+    constexpr bool kStagedFinish = !(sizeof(R) == 8 && !UNI);
+    double cin[S];
 
     // 1. stage primitive stencil cells x0-1-PN .. x0+63+PN.  FP32 forms and
     // uniform-grid FP64 issue every load before the first conversion waits
@@ -442,6 +447,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
             const long go = ghost_off(a, ok ? x : x0, j, k);
 #pragma unroll
             for (int v = 0; v < S; ++v) cv[it][v] = ok ? a.qin.v[v][go] : 0.0;
+        }
+        // cell x0+lane is staging entry lane+1+PN: loaded by lane
+        // (lane+1+PN) mod 64 in pass 0, or in pass 1 for the last 1+PN lanes
+        static_assert(NIT == 2, "two staging passes");
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            const double send = lane < 1 + PN ? cv[1][v] : cv[0][v];
+            cin[v] = __shfl(send, (lane + 1 + PN) & 63, 64);
         }
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
@@ -558,7 +571,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
             Fp[v] = sf[v][lane + 1];
         }
         FinishIn<S> fin;
-        finish_load<NF, D>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), fin);
+        if constexpr (kStagedFinish)
+            finish_load_staged<NF, D>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), cin, 1, fin);
+        else
+            finish_load<NF, D>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), fin);
         finish_cell<NF, D, R>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), a.inv_ds[i], Fm, Fp,
                               sf[S][lane], sf[S][lane + 1], fin);
     }

@@ -146,9 +146,11 @@ def test_trained_weights_beat_plain_against_finer_run(gpu):
         say(f"    plain FP32   {' '.join(f'{x:.3e}' for x in e32)}  GPU s {res[PREC_FP32_BULK][1]:.3f}")
         say(f"    learned      {' '.join(f'{x:.3e}' for x in el)}  GPU s {res[PREC_FP32_LEARNED][1]:.3f}")
         say(f"    gain over plain FP32 {' '.join(f'{x:.3f}x' for x in e32 / el)}; finer run GPU s {sf:.3f}")
-        # the shear layer's density, transverse momentum and energy errors
-        # all drop; the streamwise momentum is left to the report
-        for v in (0, 2, 3):
+        # every variable the network reconstructs is closer to the finer
+        # run than plain FP32 (measured gains: density 1.10-1.15x,
+        # streamwise momentum 1.03-1.04x, transverse momentum 1.9-2.3x,
+        # energy 1.25-1.29x)
+        for v in range(4):
             assert el[v] < e32[v], (seed, names[v])
     # matched-error view: everything against a 4x finer plain FP64 run,
     # compared on the coarse grid
