@@ -185,22 +185,25 @@ __global__ __launch_bounds__(256) void k_visc_grad(ViscGradArgs g) {
 #pragma unroll
         for (int q = 0; q < kNCoef; ++q) cf[q] = g.coef[(long)q * g.ncell + (je + g.b)];
     }
+    double st[D][WO];
+    if (set) {
+#pragma unroll
+        for (int r = 0; r < WO; ++r) {
+            const long oc = o + (r - P) * step;
+            double rho = 0.0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) rho = rho + g.q.v[f][oc];
+            rho = max_ref(rho, 1e-16);
+#pragma unroll
+            for (int c = 0; c < D; ++c) st[c][r] = g.q.v[Y::MOM + c][oc] / rho;
+        }
+    }
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         double val = __builtin_nan("");
         if (set) {
-            double st[WO];
-#pragma unroll
-            for (int r = 0; r < WO; ++r) {
-                const long oc = o + (r - P) * step;
-                double rho = 0.0;
-#pragma unroll
-                for (int f = 0; f < NF; ++f) rho = rho + g.q.v[f][oc];
-                rho = max_ref(rho, 1e-16);
-                st[r] = g.q.v[Y::MOM + c][oc] / rho;
-            }
             double vl, vr;
-            weno_cell<WO, double, true, true>(st, cf, g.eps, vl, vr);
+            weno_cell<WO, double, true, true>(st[c], cf, g.eps, vl, vr);
             val = g.inv_dsg[je + g.b] * (vr - vl);
         }
         g.grad[(long)(E * D + c) * g.ng + o] = val;
@@ -289,16 +292,25 @@ __global__ __launch_bounds__(64) void k_visc_face_x(ViscArgs a, int ntile) {
     const int c = min(f0 + lane, hi - P); // this lane's cell (clamped past the last face)
     double cf[kNCoef];
     cell_coef<UNI>(a, c, cf);
-    double vl[S], ql[S], qr[S];
-#pragma unroll
+    // one variable at a time (bounded registers); the face states go to LDS
+    // rows only this lane reads back: its upper-face value, and the next
+    // lane's lower-face value
+    __shared__ double sq[2][S][64];
+#pragma unroll 1
     for (int v = 0; v < S; ++v) {
-        double st[WO];
+        double st[WO], vl, vr;
 #pragma unroll
         for (int r = 0; r < WO; ++r) st[r] = sp[v][lane + r];
-        weno_cell<WO, double, true, true>(st, cf, a.eps, vl[v], ql[v]);
+        weno_cell<WO, double, true, true>(st, cf, a.eps, vl, vr);
+        sq[0][v][lane] = vr;
+        sq[1][v][lane] = __shfl_down(vl, 1, 64);
     }
+    double ql[S], qr[S];
 #pragma unroll
-    for (int v = 0; v < S; ++v) qr[v] = __shfl_down(vl[v], 1, 64);
+    for (int v = 0; v < S; ++v) {
+        ql[v] = sq[0][v][lane];
+        qr[v] = sq[1][v][lane];
+    }
     // velocities of rows y-1 and y+1 of this lane's cell, then of the next
     // lane's (2D transverse gradients)
     double gt[2] = {0.0, 0.0};
@@ -468,6 +480,107 @@ __global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
     }
 }
 
+// y faces (2D), cell-parallel on an LDS tile: a 256-thread block covers
+// kYC x-columns by kYK cells along y, the K lanes of a column consecutive
+// in the wave.  The primitive stencil rows of the tile (with one x-halo
+// column on each side, for the transverse velocity differences) are
+// converted once into LDS; thread (column, c) reconstructs both faces of
+// cell f0+c one variable at a time (the WENO of the sweeps), its upper-face
+// value and the next lane's lower-face value go to LDS as the states of
+// face f0+c+1/2, and after a barrier the first K-1 lanes of each column
+// solve their faces (HLLC, face Reynolds numbers, gradients, viscous flux)
+// and store the face records.  Each cell is converted and reconstructed
+// once, as in the marching kernel it replaces, with the per-variable
+// reconstruction bounded to one variable's registers.
+constexpr int kYC = 16; // x-columns per tile
+constexpr int kYK = 16; // cells per column (faces: kYK - 1)
+
+template <int NF, int WO, bool UNI, bool WRF>
+__global__ __launch_bounds__(kYC *kYK) void k_visc_face_yc(ViscArgs a, int nxb) {
+    constexpr int D = 2;
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    constexpr int P = (WO - 1) / 2;
+    constexpr int R = kYK + 2 * P; // stencil rows f0-P .. f0+K-1+P
+    constexpr int XC = kYC + 2;     // columns x0-1 .. x0+C
+    __shared__ double sp[S][R][XC];
+    __shared__ double sL[S][kYK][kYC]; // face f0+c+1/2: left state (upper face of cell f0+c)
+    __shared__ double sR[S][kYK][kYC]; //                right state (lower face of cell f0+c+1)
+    const int tid = threadIdx.x;
+    const int col = tid / kYK, c = tid % kYK;
+    const int x0 = (int)(blockIdx.x % nxb) * kYC;
+    const int f0 = -1 + (kYK - 1) * (int)(blockIdx.x / nxb);
+    const int hx = a.m + a.b, hy = a.n + a.b; // last ghost cells
+    for (int e = tid; e < R * XC; e += kYC * kYK) {
+        const int r = e / XC, xc = e % XC;
+        const long o = goff(a, min(x0 - 1 + xc, hx), min(f0 - P + r, hy));
+        double cv[S], pv[S];
+#pragma unroll
+        for (int v = 0; v < S; ++v) cv[v] = a.q.v[v][o];
+        cons_to_prim<NF, D>(cv, pv, a.eos);
+#pragma unroll
+        for (int v = 0; v < S; ++v) sp[v][r][xc] = pv[v];
+    }
+    __syncthreads();
+    const int x = x0 + col;
+    const int cy = f0 + c; // this lane's cell
+    double cf[kNCoef];
+    cell_coef<UNI>(a, min(cy, hy - P), cf);
+#pragma unroll 1
+    for (int v = 0; v < S; ++v) {
+        double st[WO], vl, vr;
+#pragma unroll
+        for (int r = 0; r < WO; ++r) st[r] = sp[v][c + r][col + 1];
+        weno_cell<WO, double, true, true>(st, cf, a.eps, vl, vr);
+        sL[v][c][col] = vr;
+        // lane c+1 of the same column holds cell f0+c+1 (the last lane's value
+        // is not used)
+        sR[v][c][col] = __shfl_down(vl, 1, 64);
+    }
+    __syncthreads();
+    const int f = cy;
+    if (c == kYK - 1 || x > a.m || f > a.n) return;
+    double ql[S], qr[S];
+#pragma unroll
+    for (int v = 0; v < S; ++v) {
+        ql[v] = sL[v][c][col];
+        qr[v] = sR[v][c][col];
+    }
+    if (f == -1 && a.bc_beg == -4) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) ql[v] = qr[v];
+    }
+    if (f == a.n && a.bc_end == -4) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) qr[v] = ql[v];
+    }
+    const long t = (long)(f + 1) * (a.m + 1) + x;
+    double ax[2], ay[2];
+    if constexpr (WRF) {
+        weno_averages<D>(a, t, ax, ay);
+    } else {
+        // velocities of cells (x-1 .. x+1, f) and (x-1 .. x+1, f+1)
+        const int r0 = c + P, r1 = c + P + 1;
+        const double dyf = a.ccy[f + 1 + a.b] - a.ccy[f + a.b];
+        const double dxm = a.ccx[x + a.b] - a.ccx[x - 1 + a.b], dxp = a.ccx[x + 1 + a.b] - a.ccx[x + a.b];
+        double gxn[2], gt[2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const double uo = sp[Y::MOM + d][r0][col + 1], uw = sp[Y::MOM + d][r0][col], ue = sp[Y::MOM + d][r0][col + 2];
+            const double no = sp[Y::MOM + d][r1][col + 1], nw = sp[Y::MOM + d][r1][col], ne = sp[Y::MOM + d][r1][col + 2];
+            gxn[d] = (no - uo) / dyf;
+            // four-point average of the one-sided x differences of rows f+1
+            // and f (dqR_prim_dx_n(2) at f, m_viscous.fpp:285-315)
+            const double sum = (no - nw) / dxm + (ne - no) / dxp + (uo - uw) / dxm + (ue - uo) / dxp;
+            gt[d] = 25e-2 * sum;
+        }
+        fd_averages<D, 1>(gxn, gt, ax, ay);
+    }
+    double flux[S], ustar, fs[D + 1];
+    face_flux<NF, D, 1>(a, ql, qr, ax, ay, flux, ustar, fs);
+    store_face<NF, D>(a, t, flux, ustar, fs);
+}
+
 // Finish of one direction for every interior cell, in the reference's
 // order (m_rhs.fpp:565-652): the flux divergence (assigned in the first
 // direction, added after), the alpha*div(u) source, the viscous divergence
@@ -531,10 +644,10 @@ void launch_face_u(const ViscArgs &a, hipStream_t st) {
         hipLaunchKernelGGL((k_visc_face_x<NF, D, WO, UNI, WRF>), dim3((unsigned)((long)ntile * (a.n + 1))), dim3(64),
                            0, st, a, ntile);
     } else {
-        const int nxb = (a.m + 1 + 255) / 256;
-        const int nchunk = (a.n + 2 + kYChunk - 1) / kYChunk;
-        hipLaunchKernelGGL((k_visc_face_y<NF, WO, kYChunk, UNI, WRF>), dim3((unsigned)((long)nxb * nchunk)),
-                           dim3(256), 0, st, a, nxb);
+        const int nxb = (a.m + 1 + kYC - 1) / kYC;
+        const int ntile = (a.n + 2 + kYK - 2) / (kYK - 1); // faces -1 .. n in tiles of kYK - 1
+        hipLaunchKernelGGL((k_visc_face_yc<NF, WO, UNI, WRF>), dim3((unsigned)((long)nxb * ntile)),
+                           dim3(kYC * kYK), 0, st, a, nxb);
     }
 }
 
