@@ -35,10 +35,14 @@ FP64_PEAK_TFLOPS = 78.6
 # and its VALU-busy fraction (SQ_ACTIVE_INST_VALU * 4 / (1024 CU-SIMDs *
 # GRBM_GUI_ACTIVE / 8)), with the profile summary they come from.  Other
 # workloads report null.
+_P512 = "profiles/sweep_kernels_shock_droplet_3d_512_fp64_fast.md"
 PROFILED = {
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_x"): {
-        "traffic": 29.06e9 + 8.80e9, "valu_busy": 0.75, "limiter": "VALU issue (FP64), not HBM",
-        "profile": "profiles/sweep_kernels_shock_droplet_3d_512_fp64_fast.md"},
+        "traffic": 23.13e9 + 8.70e9, "valu_busy": 0.79, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
+    ("shock_droplet_3d", 512, "fp64_fast", "sweep_y"): {
+        "traffic": 17.44e9 + 8.59e9, "valu_busy": 0.71, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
+    ("shock_droplet_3d", 512, "fp64_fast", "sweep_z"): {
+        "traffic": 8.99e9 + 8.59e9, "valu_busy": 0.83, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
     ("shock_bubble_2d", 2048, "fp64", "sweep_y"): {
         "traffic": 678.5e6 + 236.6e6, "valu_busy": 0.79, "limiter": "VALU issue (FP64), not HBM",
         "profile": "profiles/sweep_kernels_shock_bubble_2d_2048_fp64.md"},

@@ -1,19 +1,23 @@
 // mfc_viscous.hip -- the RHS and RK stage of viscous runs: inviscid and
-// viscous fluxes with finite-difference velocity gradients.
+// viscous fluxes, with velocity gradients by finite differences or
+// (weno_Re_flux) by the divergence theorem on WENO face values.
 //
 // The viscous flux needs, per face, the HLLC star velocity and the
 // reconstructed volume fractions (m_riemann_solvers.fpp:169-231, :313-323),
 // which the inviscid sweeps keep in registers and LDS only.  Viscous runs
 // therefore use these kernels in place of the sweeps.  The face kernels
 // reconstruct with the sweeps' own weno_cell and hllc_face (hence the same
-// bits), each cell converted to primitive and reconstructed once (both
-// faces): in LDS and lane shuffles along x, in a register window marching
-// along y.  The velocity gradients come from the primitive values and the
-// conservative field.  Per face they store the HLLC flux, the contact
-// velocity and the viscous source flux (m_riemann_solvers.fpp:683-902); a
-// finish kernel per direction applies the flux divergence, the
-// alpha*div(u) source and the viscous divergence in the reference's order
-// (m_rhs.fpp:565-652), and in the last direction the RK stage.
+// bits), each cell converted to primitive once into an LDS tile and
+// reconstructed once (both faces), one variable at a time.  The
+// finite-difference gradients come from the primitive velocities (of the
+// tile, and for the x faces' neighbours in y of the conservative field);
+// with weno_Re_flux, gradient planes and their face averages are computed
+// by two passes before the face kernels.  Per face they store the HLLC flux,
+// the contact velocity and the viscous source flux
+// (m_riemann_solvers.fpp:683-902); a finish kernel per direction applies the
+// flux divergence, the alpha*div(u) source and the viscous divergence in the
+// reference's order (m_rhs.fpp:565-652), and in the last direction the RK
+// stage.
 #include "mfc_viscous.hpp"
 
 namespace mfc {
@@ -363,123 +367,6 @@ __global__ __launch_bounds__(64) void k_visc_face_x(ViscArgs a, int ntile) {
     store_face<NF, D>(a, t, flux, ustar, fs);
 }
 
-// y faces (2D): thread = x column, marching along y over the L faces
-// f0 .. f0+L-1 of its chunk (f0 = -1 + L*chunk) with the primitive stencil
-// in a register window, so each cell is converted and reconstructed once
-// (both faces); x-neighbour velocities come from the conservative field.
-// The kernel holds 256 VGPRs and 72 AGPRs (one wave per SIMD), taken by the
-// per-face WENO, HLLC and viscous arithmetic (the window in an LDS ring
-// leaves the count unchanged); capped at 2, 3 or 4 waves it spills (0.1-0.9
-// KB per lane) and is slower.
-template <int NF, int WO, int L, bool UNI, bool WRF>
-__global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
-    constexpr int D = 2;
-    using Y = Sys<NF, D>;
-    constexpr int S = Y::S;
-    constexpr int P = (WO - 1) / 2;
-    constexpr int W = 2 * P + 1;
-    const int x = (int)(blockIdx.x % nxb) * 256 + threadIdx.x;
-    const int chunk = (int)(blockIdx.x / nxb);
-    if (x > a.m) return;
-    const int f0 = -1 + L * chunk;
-    const int hi = a.n + a.b;
-    auto prim = [&](int cy, double *pv) {
-        const long o = goff(a, x, min(cy, hi));
-        double cv[S];
-#pragma unroll
-        for (int v = 0; v < S; ++v) cv[v] = a.q.v[v][o];
-        cons_to_prim<NF, D>(cv, pv, a.eos);
-    };
-    double w[W][S]; // cells cur-P .. cur+P
-#pragma unroll
-    for (int r = 0; r < W; ++r) prim(f0 - P + r, w[r]);
-    const double dxm = a.ccx[x + a.b] - a.ccx[x - 1 + a.b], dxp = a.ccx[x + 1 + a.b] - a.ccx[x + a.b];
-    double cf[kNCoef], ql[S], dummy;
-    cell_coef<UNI>(a, min(f0, hi - P), cf);
-#pragma unroll
-    for (int v = 0; v < S; ++v) {
-        double st[W];
-#pragma unroll
-        for (int r = 0; r < W; ++r) st[r] = w[r][v];
-        weno_cell<WO, double, false, true>(st, cf, a.eps, dummy, ql[v]);
-    }
-    // row f0: own velocity and the x neighbours'
-    double uo[2], uw[2] = {0.0, 0.0}, ue[2] = {0.0, 0.0};
-#pragma unroll
-    for (int d = 0; d < 2; ++d) {
-        uo[d] = w[P][Y::MOM + d];
-        if constexpr (!WRF) {
-            uw[d] = cell_vel<NF, D>(a, x - 1, min(f0, hi), d);
-            ue[d] = cell_vel<NF, D>(a, x + 1, min(f0, hi), d);
-        }
-    }
-#pragma unroll 1
-    for (int k = 0; k < L; ++k) {
-        const int f = f0 + k;
-        // window to cells f+1-P .. f+1+P
-#pragma unroll
-        for (int r = 0; r + 1 < W; ++r)
-#pragma unroll
-            for (int v = 0; v < S; ++v) w[r][v] = w[r + 1][v];
-        prim(f + 1 + P, w[W - 1]);
-        cell_coef<UNI>(a, min(f + 1, hi - P), cf);
-        double qr[S], qn[S];
-#pragma unroll
-        for (int v = 0; v < S; ++v) {
-            double st[W];
-#pragma unroll
-            for (int r = 0; r < W; ++r) st[r] = w[r][v];
-            weno_cell<WO, double, true, true>(st, cf, a.eps, qr[v], qn[v]);
-        }
-        double no[2], nw[2] = {0.0, 0.0}, ne[2] = {0.0, 0.0};
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-            no[d] = w[P][Y::MOM + d];
-            if constexpr (!WRF) {
-                nw[d] = cell_vel<NF, D>(a, x - 1, min(f + 1, hi), d);
-                ne[d] = cell_vel<NF, D>(a, x + 1, min(f + 1, hi), d);
-            }
-        }
-        if (f <= a.n) {
-            double l2[S], r2[S];
-#pragma unroll
-            for (int v = 0; v < S; ++v) {
-                l2[v] = (f == -1 && a.bc_beg == -4) ? qr[v] : ql[v];
-                r2[v] = (f == a.n && a.bc_end == -4) ? ql[v] : qr[v];
-            }
-            const long t = (long)(f + 1) * (a.m + 1) + x;
-            double ax[2], ay[2];
-            if constexpr (WRF) {
-                weno_averages<D>(a, t, ax, ay);
-            } else {
-                const double dyf = a.ccy[f + 1 + a.b] - a.ccy[f + a.b];
-                double gxn[2], gt[2];
-#pragma unroll
-                for (int d = 0; d < 2; ++d) {
-                    gxn[d] = (no[d] - uo[d]) / dyf;
-                    // four-point average of the one-sided x differences of rows
-                    // f+1 and f (dqR_prim_dx_n(2) at f, m_viscous.fpp:285-315)
-                    const double sum = (no[d] - nw[d]) / dxm + (ne[d] - no[d]) / dxp + (uo[d] - uw[d]) / dxm +
-                                       (ue[d] - uo[d]) / dxp;
-                    gt[d] = 25e-2 * sum;
-                }
-                fd_averages<D, 1>(gxn, gt, ax, ay);
-            }
-            double flux[S], ustar, fs[D + 1];
-            face_flux<NF, D, 1>(a, l2, r2, ax, ay, flux, ustar, fs);
-            store_face<NF, D>(a, t, flux, ustar, fs);
-        }
-#pragma unroll
-        for (int v = 0; v < S; ++v) ql[v] = qn[v];
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-            uo[d] = no[d];
-            uw[d] = nw[d];
-            ue[d] = ne[d];
-        }
-    }
-}
-
 // y faces (2D), cell-parallel on an LDS tile: a 256-thread block covers
 // kYC x-columns by kYK cells along y, the K lanes of a column consecutive
 // in the wave.  The primitive stencil rows of the tile (with one x-halo
@@ -490,8 +377,9 @@ __global__ __launch_bounds__(256) void k_visc_face_y(ViscArgs a, int nxb) {
 // face f0+c+1/2, and after a barrier the first K-1 lanes of each column
 // solve their faces (HLLC, face Reynolds numbers, gradients, viscous flux)
 // and store the face records.  Each cell is converted and reconstructed
-// once, as in the marching kernel it replaces, with the per-variable
-// reconstruction bounded to one variable's registers.
+// once, with the per-variable reconstruction bounded to one variable's
+// registers (a thread marching along y with the stencil window in registers
+// held 256 VGPRs and 72 AGPRs: one wave per SIMD).
 constexpr int kYC = 16; // x-columns per tile
 constexpr int kYK = 16; // cells per column (faces: kYK - 1)
 
@@ -503,9 +391,14 @@ __global__ __launch_bounds__(kYC *kYK) void k_visc_face_yc(ViscArgs a, int nxb) 
     constexpr int P = (WO - 1) / 2;
     constexpr int R = kYK + 2 * P; // stencil rows f0-P .. f0+K-1+P
     constexpr int XC = kYC + 2;     // columns x0-1 .. x0+C
-    __shared__ double sp[S][R][XC];
-    __shared__ double sL[S][kYK][kYC]; // face f0+c+1/2: left state (upper face of cell f0+c)
-    __shared__ double sR[S][kYK][kYC]; //                right state (lower face of cell f0+c+1)
+    // stencil rows of each column contiguous (odd pitch R + 1): the lanes of
+    // a column read consecutive entries
+    __shared__ double sp[S][XC][R + 1];
+    // face f0+c+1/2 of column col: left state (upper face of cell f0+c) and
+    // right state (lower face of cell f0+c+1), c fastest (the lanes of a
+    // column are consecutive: conflict-free LDS access)
+    __shared__ double sL[S][kYC][kYK];
+    __shared__ double sR[S][kYC][kYK];
     const int tid = threadIdx.x;
     const int col = tid / kYK, c = tid % kYK;
     const int x0 = (int)(blockIdx.x % nxb) * kYC;
@@ -519,7 +412,7 @@ __global__ __launch_bounds__(kYC *kYK) void k_visc_face_yc(ViscArgs a, int nxb) 
         for (int v = 0; v < S; ++v) cv[v] = a.q.v[v][o];
         cons_to_prim<NF, D>(cv, pv, a.eos);
 #pragma unroll
-        for (int v = 0; v < S; ++v) sp[v][r][xc] = pv[v];
+        for (int v = 0; v < S; ++v) sp[v][xc][r] = pv[v];
     }
     __syncthreads();
     const int x = x0 + col;
@@ -530,12 +423,12 @@ __global__ __launch_bounds__(kYC *kYK) void k_visc_face_yc(ViscArgs a, int nxb) 
     for (int v = 0; v < S; ++v) {
         double st[WO], vl, vr;
 #pragma unroll
-        for (int r = 0; r < WO; ++r) st[r] = sp[v][c + r][col + 1];
+        for (int r = 0; r < WO; ++r) st[r] = sp[v][col + 1][c + r];
         weno_cell<WO, double, true, true>(st, cf, a.eps, vl, vr);
-        sL[v][c][col] = vr;
+        sL[v][col][c] = vr;
         // lane c+1 of the same column holds cell f0+c+1 (the last lane's value
         // is not used)
-        sR[v][c][col] = __shfl_down(vl, 1, 64);
+        sR[v][col][c] = __shfl_down(vl, 1, 64);
     }
     __syncthreads();
     const int f = cy;
@@ -543,8 +436,8 @@ __global__ __launch_bounds__(kYC *kYK) void k_visc_face_yc(ViscArgs a, int nxb) 
     double ql[S], qr[S];
 #pragma unroll
     for (int v = 0; v < S; ++v) {
-        ql[v] = sL[v][c][col];
-        qr[v] = sR[v][c][col];
+        ql[v] = sL[v][col][c];
+        qr[v] = sR[v][col][c];
     }
     if (f == -1 && a.bc_beg == -4) {
 #pragma unroll
@@ -566,8 +459,8 @@ __global__ __launch_bounds__(kYC *kYK) void k_visc_face_yc(ViscArgs a, int nxb) 
         double gxn[2], gt[2];
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
-            const double uo = sp[Y::MOM + d][r0][col + 1], uw = sp[Y::MOM + d][r0][col], ue = sp[Y::MOM + d][r0][col + 2];
-            const double no = sp[Y::MOM + d][r1][col + 1], nw = sp[Y::MOM + d][r1][col], ne = sp[Y::MOM + d][r1][col + 2];
+            const double uo = sp[Y::MOM + d][col + 1][r0], uw = sp[Y::MOM + d][col][r0], ue = sp[Y::MOM + d][col + 2][r0];
+            const double no = sp[Y::MOM + d][col + 1][r1], nw = sp[Y::MOM + d][col][r1], ne = sp[Y::MOM + d][col + 2][r1];
             gxn[d] = (no - uo) / dyf;
             // four-point average of the one-sided x differences of rows f+1
             // and f (dqR_prim_dx_n(2) at f, m_viscous.fpp:285-315)
@@ -634,8 +527,6 @@ __global__ __launch_bounds__(256) void k_visc_finish(ViscArgs a, SweepArgs s, lo
 #pragma unroll
     for (int v = 0; v < S; ++v) s.qout.v[v][go] = o[v];
 }
-
-constexpr int kYChunk = 8; // y faces per thread of k_visc_face_y (16 and 32: 1 % and 4 % slower at 2048^2)
 
 template <int NF, int D, int DIR, int WO, bool UNI, bool WRF>
 void launch_face_u(const ViscArgs &a, hipStream_t st) {
