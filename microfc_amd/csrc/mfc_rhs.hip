@@ -81,6 +81,7 @@ struct Dir {
     bool ready = false;
     bool uniform = false;  // every reconstructed cell has the same coefficients
     double ucf[kNCoef] = {};
+    bool uniform_wide = false; // the same over every cell the coefficients are set for (-b+polyn..s+b-polyn)
 };
 
 struct State {
@@ -816,6 +817,8 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
             g.b = G.b;
             g.bj = G.prm.n > 0 ? G.b : 0;
             g.s = G.dir[e].s;
+            g.uniform = G.dir[e].uniform_wide ? 1 : 0;
+            for (int q = 0; q < kNCoef; ++q) g.ucf[q] = G.dir[e].ucf[q];
             TimedScope ts(kKViscous);
             if (launch_visc_grad(G.prm.num_fluids, G.D, e, G.prm.weno_order, g, G.stream))
                 return fail(-3, "viscous gradient launch failed (dir %d): %s", e,
@@ -927,6 +930,16 @@ int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds) {
             for (int j = 0; j <= s + 1 && D.uniform; ++j)
                 D.uniform = std::memcmp(&v0, &D.coef_h[(long)q * nc + (j + b)], sizeof(double)) == 0;
         }
+    }
+    // ... and over every cell with coefficients (the weno_Re_flux gradient
+    // pass reconstructs the ghost cells too)
+    {
+        const long nc = s + 2 * b + 1;
+        const int pn = (G.prm.weno_order - 1) / 2;
+        D.uniform_wide = D.uniform;
+        for (int q = 0; q < kNCoef && D.uniform_wide; ++q)
+            for (int j = -b + pn; j <= s + b - pn && D.uniform_wide; ++j)
+                D.uniform_wide = std::memcmp(&D.ucf[q], &D.coef_h[(long)q * nc + (j + b)], sizeof(double)) == 0;
     }
     std::vector<double> inv(s + 1), invg(s + 2 * b + 1);
     for (int i = 0; i <= s; ++i) inv[i] = 1.0 / ds[i + b];

@@ -173,7 +173,7 @@ __device__ __forceinline__ void store_face(const ViscArgs &a, long t, const doub
 // (s_apply_scalar_divergence_theorem, :372-398) over [-b+1, s+b-1].  Cells
 // where the reference reads face values it never wrote, or never writes the
 // gradient, get NaN: their reconstructions reach no face flux the RHS uses.
-template <int NF, int D, int E, int WO>
+template <int NF, int D, int E, int WO, bool UNI>
 __global__ __launch_bounds__(256) void k_visc_grad(ViscGradArgs g) {
     using Y = Sys<NF, D>;
     constexpr int P = (WO - 1) / 2;
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_visc_grad(ViscGradArgs g) {
     double cf[kNCoef];
     if (set) {
 #pragma unroll
-        for (int q = 0; q < kNCoef; ++q) cf[q] = g.coef[(long)q * g.ncell + (je + g.b)];
+        for (int q = 0; q < kNCoef; ++q) cf[q] = UNI ? g.ucf[q] : g.coef[(long)q * g.ncell + (je + g.b)];
     }
     double st[D][WO];
     if (set) {
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) void k_visc_grad(ViscGradArgs g) {
 // of s_populate_riemann_states_variables_buffers (m_riemann_solvers.fpp:
 // 488-608, with the caller's L/R swap), then 5e-1*(R + L) as
 // s_compute_viscous_source_flux averages them (:683-902).
-template <int NF, int D, int DIR, int WO>
+template <int NF, int D, int DIR, int WO, bool UNI>
 __global__ __launch_bounds__(256) void k_visc_face_grad(ViscArgs a) {
     constexpr int P = (WO - 1) / 2;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -242,11 +242,8 @@ __global__ __launch_bounds__(256) void k_visc_face_grad(ViscArgs a) {
     const long step = DIR == 0 ? 1 : a.nxg;
     const long o0 = (long)(x + a.b) + a.nxg * (long)(y + a.bj); // cell f of the face's pair
     double cf0[kNCoef], cf1[kNCoef];
-#pragma unroll
-    for (int q = 0; q < kNCoef; ++q) {
-        cf0[q] = a.coef[(long)q * a.ncell + (f + a.b)];
-        cf1[q] = a.coef[(long)q * a.ncell + (f + 1 + a.b)];
-    }
+    cell_coef<UNI>(a, f, cf0);
+    cell_coef<UNI>(a, f + 1, cf1);
 #pragma unroll
     for (int pl = 0; pl < D * D; ++pl) {
         const double *G = a.grad + (long)pl * a.ng;
@@ -545,10 +542,14 @@ void launch_face_u(const ViscArgs &a, hipStream_t st) {
 template <int NF, int D, int DIR, int WO>
 void launch_face(const ViscArgs &a, hipStream_t st) {
     if (a.wrf) {
-        hipLaunchKernelGGL((k_visc_face_grad<NF, D, DIR, WO>), dim3((unsigned)((a.nface + 255) / 256)), dim3(256), 0,
-                           st, a);
-        if (a.uniform) launch_face_u<NF, D, DIR, WO, true, true>(a, st);
-        else launch_face_u<NF, D, DIR, WO, false, true>(a, st);
+        const dim3 grid((unsigned)((a.nface + 255) / 256));
+        if (a.uniform) {
+            hipLaunchKernelGGL((k_visc_face_grad<NF, D, DIR, WO, true>), grid, dim3(256), 0, st, a);
+            launch_face_u<NF, D, DIR, WO, true, true>(a, st);
+        } else {
+            hipLaunchKernelGGL((k_visc_face_grad<NF, D, DIR, WO, false>), grid, dim3(256), 0, st, a);
+            launch_face_u<NF, D, DIR, WO, false, true>(a, st);
+        }
     } else {
         if (a.uniform) launch_face_u<NF, D, DIR, WO, true, false>(a, st);
         else launch_face_u<NF, D, DIR, WO, false, false>(a, st);
@@ -581,10 +582,17 @@ int launch_nf(int D, int dir, int weno_order, const ViscArgs &a, const SweepArgs
 template <int NF, int D, int E>
 int launch_grad_e(int weno_order, const ViscGradArgs &g, hipStream_t st) {
     const dim3 grid((unsigned)((g.ng + 255) / 256));
+    const bool u = g.uniform != 0;
     switch (weno_order) {
-    case 1: hipLaunchKernelGGL((k_visc_grad<NF, D, E, 1>), grid, dim3(256), 0, st, g); break;
-    case 3: hipLaunchKernelGGL((k_visc_grad<NF, D, E, 3>), grid, dim3(256), 0, st, g); break;
-    case 5: hipLaunchKernelGGL((k_visc_grad<NF, D, E, 5>), grid, dim3(256), 0, st, g); break;
+    case 1: hipLaunchKernelGGL((k_visc_grad<NF, D, E, 1, false>), grid, dim3(256), 0, st, g); break;
+    case 3:
+        if (u) hipLaunchKernelGGL((k_visc_grad<NF, D, E, 3, true>), grid, dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((k_visc_grad<NF, D, E, 3, false>), grid, dim3(256), 0, st, g);
+        break;
+    case 5:
+        if (u) hipLaunchKernelGGL((k_visc_grad<NF, D, E, 5, true>), grid, dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((k_visc_grad<NF, D, E, 5, false>), grid, dim3(256), 0, st, g);
+        break;
     default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -58,6 +58,8 @@ struct ViscGradArgs {
     double eps;
     long nxg, ng;
     int m, n, b, bj, s;     // s: last interior index along e
+    int uniform;            // every cell the pass reconstructs has the coefficients ucf (bit for bit)
+    double ucf[kNCoef];
 };
 
 // Direction dir of a viscous RHS evaluation: the face records, then the
