@@ -705,7 +705,8 @@ SweepArgs sweep_args(int d) {
         a.chunk = G.prm.precision == MFC_PREC_FP32_LEARNED ? t_tile_cells<true, float, true>(d)
                   : G.prm.precision == MFC_PREC_FP32_BULK
                       ? (uni ? t_tile_cells<false, float, true>(d) : t_tile_cells<false, float, false>(d))
-                      : (uni ? t_tile_cells<false, double, true>(d) : t_tile_cells<false, double, false>(d));
+                  : !uni ? t_tile_cells<false, double, false>(d)
+                         : tc_rows_of(true, G.prm.precision == MFC_PREC_FP64_FAST, d) - 2;
         a.nchunk = (s + 1 + a.chunk - 1) / a.chunk;
         const int other = d == 1 ? G.prm.p + 1 : G.prm.n + 1;
         a.nblocks = (long)a.ntile * a.nchunk * other;

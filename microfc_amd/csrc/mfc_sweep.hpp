@@ -73,6 +73,17 @@ struct SweepArgs {
     float ucf32[kNCoef];
 };
 
+// Threads per column of the cell-parallel y/z tile by form.  Measured on
+// MI355X: in the plain FP64 form 32 (512-thread blocks, 30 cells per column)
+// makes the y sweep 2-3 % faster than 16 (14 cells) through less halo
+// staging; in the contracted FP64 form 16 is faster (512^3 y sweep 6.32
+// against 6.83 ms, 2048^2 0.204 against 0.219 ms: with less arithmetic per
+// cell, four 256-thread blocks per CU hide the finish's rhs traffic better
+// than two of 512).  The z sweep, whose stencil rows are planes apart,
+// gains from 32 at 256^3 but loses 6 % at 512^3 (36 planes per block), and
+// in FP32, where the 3D sweeps near the HBM rate, 32 is up to 11 % slower.
+constexpr int tc_rows_of(bool f64, bool fast_f64, int dir) { return f64 && !fast_f64 && dir == 1 ? 32 : 16; }
+
 // Kernels and their helpers belong to the unit's arithmetic form (see
 // mfc_device.hpp); the argument types above are shared by every form.
 inline namespace MFC_FORM {
@@ -597,14 +608,10 @@ constexpr int t_threads() { return kTX * t_faces<ML>(); }
 template <bool ML, typename R, bool UNI>
 constexpr bool t_cell_parallel() { return x_cell_parallel<ML, R, UNI>(); }
 // Threads per column of the cell-parallel tile (a power of two <= 64; the
-// column's lanes are consecutive within a wave).  Measured on MI355X: 32
-// (512-thread blocks, 30 cells per column) makes the FP64 y sweep 2-3 %
-// faster than 16 (14 cells) through less halo staging; the z sweep, whose
-// stencil rows are planes apart, gains at 256^3 but loses 6 % at 512^3
-// (36 planes per block), and in FP32, where the 3D sweeps near the HBM
-// rate, 32 is up to 11 % slower.
+// column's lanes are consecutive within a wave): tc_rows_of (plain and
+// contracted FP64 differ, so the host picks by form too).
 template <typename R>
-constexpr int tc_rows(int dir) { return sizeof(R) == 8 && dir == 1 ? 32 : 16; }
+constexpr int tc_rows(int dir) { return tc_rows_of(sizeof(R) == 8, kFastF64, dir); }
 // x columns per cell-parallel tile: 32 for the FP32 z sweep, whose stencil
 // rows are planes apart (twice the cells per page touched; measured 7.5 %
 // faster at 512^3, 5 % at 256^3; 64 columns and FP64 gain nothing), else 16.
