@@ -83,6 +83,11 @@ struct SweepArgs {
 // gains from 32 at 256^3 but loses 6 % at 512^3 (36 planes per block), and
 // in FP32, where the 3D sweeps near the HBM rate, 32 is up to 11 % slower.
 constexpr int tc_rows_of(bool f64, bool fast_f64, int dir) { return f64 && !fast_f64 && dir == 1 ? 32 : 16; }
+// x columns per cell-parallel tile by form: 32 for the FP32 z sweep, whose
+// stencil rows are planes apart (twice the cells per page touched; measured
+// 7.5 % faster at 512^3, 5 % at 256^3; 64 columns and FP64 gain nothing),
+// else 16.
+constexpr int tc_cols_of(bool f64, bool fast_f64, int dir) { return !f64 && dir == 2 ? 32 : 16; }
 
 // Kernels and their helpers belong to the unit's arithmetic form (see
 // mfc_device.hpp); the argument types above are shared by every form.
@@ -612,11 +617,8 @@ constexpr bool t_cell_parallel() { return x_cell_parallel<ML, R, UNI>(); }
 // contracted FP64 differ, so the host picks by form too).
 template <typename R>
 constexpr int tc_rows(int dir) { return tc_rows_of(sizeof(R) == 8, kFastF64, dir); }
-// x columns per cell-parallel tile: 32 for the FP32 z sweep, whose stencil
-// rows are planes apart (twice the cells per page touched; measured 7.5 %
-// faster at 512^3, 5 % at 256^3; 64 columns and FP64 gain nothing), else 16.
 template <typename R>
-constexpr int tc_cols(int dir) { return sizeof(R) == 4 && dir == 2 ? 32 : kTX; }
+constexpr int tc_cols(int dir) { return tc_cols_of(sizeof(R) == 8, kFastF64, dir); }
 template <bool ML, typename R, bool UNI>
 constexpr int t_tile_cells(int dir) {
     return t_cell_parallel<ML, R, UNI>() ? tc_rows<R>(dir) - 2 : t_faces<ML>() - 1;
