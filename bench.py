@@ -42,7 +42,7 @@ PROFILED = {
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_x"): {
         "traffic": 23.13e9 + 8.70e9, "valu_busy": 0.79, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_y"): {
-        "traffic": 17.44e9 + 8.59e9, "valu_busy": 0.71, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
+        "traffic": 17.44e9 + 8.59e9, "valu_busy": 0.85, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_z"): {
         "traffic": 8.99e9 + 8.59e9, "valu_busy": 0.83, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
     ("shock_bubble_2d", 2048, "fp64", "sweep_y"): {
