@@ -418,8 +418,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
     constexpr int PN = (WO - 1) / 2;
     constexpr int NC = 64 + 2 * PN + 1;
     constexpr int TW = x_tile_cells<ML, R, UNI>();
-    __shared__ R sp[S][NC];
-    __shared__ R sf[S + 1][64];
+    // Cell-parallel plain forms exchange face states and fluxes between
+    // lanes by shuffles: LDS holds only the staged rows (widened to 128 so
+    // the two face states of a variable fit over its row), 8 KB per wave
+    // for FP64 instead of 9 KB of staging and flux rows.
+    constexpr bool kShfl = x_cell_parallel<ML, R, UNI>() && !ML;
+    constexpr int SPW = kShfl && NC < 128 ? 128 : NC;
+    __shared__ R sp[S][SPW];
+    __shared__ R sf[kShfl ? 1 : S + 1][64];
 
     const long bid = xcd_logical_block(blockIdx.x, a.nblocks);
     const int tile = (int)(bid % a.ntile);
@@ -520,12 +526,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
     } else if constexpr (x_cell_parallel<ML, R, UNI>()) {
         // both reconstructions of cell jf (coefficients of a cell past the
         // last face are clamped; those lanes' values are not used), one
-        // variable at a time through LDS to bound the live registers, placed
-        // as in the learned path above: no LDS beyond the staging and flux
-        // rows, so LDS does not cap the occupancy (measured: 15 KB per wave
-        // held the uniform FP64 x sweep to 3 waves per SIMD).
-        R(*const sl)[NC] = sp;
-        R(*const sr)[64] = sf;
+        // variable at a time to bound the live registers.  The wave has read
+        // the variable's stencil row before any lane writes it, so the face
+        // states of face jf+1/2 go over that row: this lane's upper-face
+        // value and, by shuffle, the next lane's lower-face value; each lane
+        // reads back only its own slots.
         R cf[kNCoef];
         load_coef<R, WO, UNI>(a, min(jf, a.m + 1), cf);
         const R eps = (R)a.eps;
@@ -535,14 +540,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
 #pragma unroll
             for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + w];
             weno_cell<WO, R, true, true>(st, cf, eps, vl, vr);
-            sl[v][lane] = vl;
-            sr[v][lane] = vr;
+            const R vn = __shfl_down(vl, 1, 64);
+            sp[v][lane] = vr;
+            sp[v][64 + lane] = vn;
         }
-        __syncthreads();
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            qL[v] = sr[v][lane];
-            qR[v] = sl[v][min(lane + 1, 63)];
+            qL[v] = sp[v][lane];
+            qR[v] = sp[v][64 + lane];
         }
     } else if (jf <= a.m) {
         R cfR[kNCoef], cfL[kNCoef];
@@ -559,6 +564,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
             for (int w = 0; w < WO; ++w) st[w] = sp[v][lane + 1 + w];
             weno_cell<WO, R, true, false>(st, cfL, eps, qR[v], dummy);
         }
+    }
+    if constexpr (kShfl) {
+        // every lane solves its face (lanes past the last face on values
+        // nobody uses), then the upper flux of each cell comes from the
+        // next lane
+        R f[S], us;
+        if (jf == -1 && a.bc_beg == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qL[v] = qR[v];
+        }
+        if (jf == a.m && a.bc_end == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qR[v] = qL[v];
+        }
+        hllc_face<NF, D, 0, R>(qL, qR, a.eos, f, us);
+        R fp[S];
+#pragma unroll
+        for (int v = 0; v < S; ++v) fp[v] = __shfl_down(f[v], 1, 64);
+        const R usp = __shfl_down(us, 1, 64);
+        const int i = x0 + lane;
+        if (lane < TW && i <= a.m) {
+            FinishIn<S> fin;
+            finish_load_staged<NF, D>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), cin, 1, fin);
+            finish_cell<NF, D, R>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), a.inv_ds[i], f, fp, us, usp,
+                                  fin);
+        }
+        return;
     }
     if (lane < TW + 1 && jf <= a.m) {
         R f[S], us;
