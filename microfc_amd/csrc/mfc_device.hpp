@@ -66,17 +66,6 @@ __device__ __forceinline__ double frcp(double b) {
     return __builtin_fma(r1, e1, r1);
 }
 
-// Reciprocal to within about 20 ulp (measured on gfx950: v_rcp_f64 alone is
-// within 5e-8 relative, one Newton step brings it to 2.2e-15): the
-// contracted form's 1/beta_k^2, whose error only moves the weights of the
-// candidate polynomials, i.e. a face value by that error times the spread
-// of the candidates (the normalisation 1/sum(alpha), which scales the whole
-// value, takes frcp).
-__device__ __forceinline__ double frcp1(double b) {
-    const double r = __builtin_amdgcn_rcp(b);
-    return __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
-}
-
 // Square root to within about one ulp from v_rsq_f64 (5e-8 relative): one
 // Newton step on the reciprocal square root, then one on the root itself.
 // 0 gives 0 and a negative argument NaN, like sqrt; no range scaling, so
@@ -200,15 +189,16 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
         const R pL0 = vj + cf[kCbL + 0] * d0, pL1 = vj + cf[kCbL + 2] * dm1;
         const R pR0 = vj + cf[kCbR + 0] * d0, pR1 = vj + cf[kCbR + 2] * dm1;
         if constexpr (kFastF64 && sizeof(R) == 8) {
-            // alpha_k = d_k / beta_k^2 and omega_k = alpha_k / sum(alpha) as
-            // products with the reciprocals (shared by both faces)
-            const R r0 = frcp1(bb0), r1 = frcp1(bb1);
+            // omega_k = alpha_k / sum(alpha) with alpha_k = d_k / beta_k^2 is
+            // unchanged when every alpha is multiplied by beta_0^2 beta_1^2:
+            // alpha'_k = d_k * (the other beta^2), and one reciprocal of the
+            // sum per face
             if constexpr (WANT_L) {
-                const R a0 = cf[kDL + 0] * r0, a1 = cf[kDL + 1] * r1;
+                const R a0 = cf[kDL + 0] * bb1, a1 = cf[kDL + 1] * bb0;
                 vL = (a0 * pL0 + a1 * pL1) * frcp(a0 + a1);
             }
             if constexpr (WANT_R) {
-                const R a0 = cf[kDR + 0] * r0, a1 = cf[kDR + 1] * r1;
+                const R a0 = cf[kDR + 0] * bb1, a1 = cf[kDR + 1] * bb0;
                 vR = (a0 * pR0 + a1 * pR1) * frcp(a0 + a1);
             }
             return;
@@ -248,13 +238,17 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
             pR[2] = vj + cf[kCbR + 4] * dm1 + cf[kCbR + 5] * dm2;
         }
         if constexpr (kFastF64 && sizeof(R) == 8) {
-            const R r0 = frcp1(bb0), r1 = frcp1(bb1), r2 = frcp1(bb2);
+            // the weights scaled by beta_0^2 beta_1^2 beta_2^2 (the ratios are
+            // unchanged): alpha'_k = d_k * the product of the other two
+            // beta^2, no reciprocal per candidate (weno_eps >= 1e-50 keeps
+            // the products normal)
+            const R q12 = bb1 * bb2, q02 = bb0 * bb2, q01 = bb0 * bb1;
             if constexpr (WANT_L) {
-                const R a0 = cf[kDL + 0] * r0, a1 = cf[kDL + 1] * r1, a2 = cf[kDL + 2] * r2;
+                const R a0 = cf[kDL + 0] * q12, a1 = cf[kDL + 1] * q02, a2 = cf[kDL + 2] * q01;
                 vL = (a0 * pL[0] + a1 * pL[1] + a2 * pL[2]) * frcp(a0 + a1 + a2);
             }
             if constexpr (WANT_R) {
-                const R a0 = cf[kDR + 0] * r0, a1 = cf[kDR + 1] * r1, a2 = cf[kDR + 2] * r2;
+                const R a0 = cf[kDR + 0] * q12, a1 = cf[kDR + 1] * q02, a2 = cf[kDR + 2] * q01;
                 vR = (a0 * pR[0] + a1 * pR[1] + a2 * pR[2]) * frcp(a0 + a1 + a2);
             }
             return;

@@ -209,11 +209,17 @@ int check_params(const mfc_params &p) {
         return fail(-1, "viscous kernels are built for 1..%d fluids", kViscFluidsMax);
     // The FP64 weight divisions (ndiv) equal IEEE division while the squared
     // smoothness indicators stay within [1e-200, 1e100], which weno_eps >=
-    // 1e-100 guarantees from below (the contracted form's reciprocals of
-    // beta^2 need the same range to stay normal).
+    // 1e-100 guarantees from below.
     if (f64 && p.weno_order > 1 && !(p.weno_eps >= 1e-100))
         return fail(-1, "weno_eps below 1e-100 is not supported in FP64 (the weight division is exact for "
                         "weno_eps >= 1e-100)");
+    // The contracted form scales the WENO5 weights by the product of the
+    // three squared smoothness indicators; its pairwise products (>=
+    // weno_eps^4) stay normal for weno_eps >= 1e-50 (and finite while the
+    // stencil differences stay below about 1e38).
+    if (p.precision == MFC_PREC_FP64_FAST && p.weno_order > 1 && !(p.weno_eps >= 1e-50))
+        return fail(-1, "weno_eps below 1e-50 is not supported in the contracted FP64 form (the scaled weights "
+                        "need weno_eps^4 normal)");
     // In FP32, beta^2 >= weno_eps^2 must stay a normal float and d/beta^2 finite.
     if (!f64 && p.weno_order > 1 && !(p.weno_eps >= 1e-18))
         return fail(-1, "weno_eps below 1e-18 is not supported in the FP32 forms (its square underflows)");

@@ -60,9 +60,10 @@ def state_linf(prm, a, b, q):
 def scaled_linf(prm, a, b):
     """Per-variable L-inf error of a against b, normalised by a physical
     scale: max|b_v| for partial densities and energy, the largest momentum
-    component for every momentum component, and the velocity-divergence
-    scale max|b_mass| / max|rho| for the volume fractions (whose reference
-    rhs is round-off small in single-fluid regions)."""
+    component for every momentum component, and for the volume fractions
+    (whose reference rhs is round-off small in single-fluid regions) the
+    larger of their own max|b_v| and the largest partial-density rhs
+    max|b_mass|."""
     nf, D = prm.num_fluids, prm.num_dims
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -5,7 +5,8 @@ clipped to [0, 1] at each stage) against the FP64 oracle.
 The error against FP64 is printed and bounded per result:
   * one RHS: per-variable L-inf error <= 1e-3 of the variable's scale
     (helpers.scaled_linf: max|rhs| of the variable, the largest momentum
-    component for momenta, the velocity-divergence scale for alpha);
+    component for momenta, for alpha the larger of its own and the largest
+    partial-density rhs);
   * 20 RK3 steps: per-variable L1 error <= 1e-5 of the variable's L1 norm
     (momentum components share one norm);
   * discrete conservation of mass/momentum/energy with periodic boundaries

@@ -9,8 +9,9 @@ Bars, with the error printed per case:
   * one RHS: relative L-infinity <= 1e-12 per variable, except the volume
     fractions, whose reference RHS is a near-cancellation of the flux
     difference and the alpha*div(u) source (zero up to rounding in
-    single-fluid regions and away from interfaces): they are measured on
-    the velocity-divergence scale max|mass rhs| / max|rho|, <= 1e-12;
+    single-fluid regions and away from interfaces): they are measured
+    against the larger of their own largest RHS and the largest
+    partial-density RHS (helpers.scaled_linf), <= 1e-12;
   * 40 steps of the 2D shock-bubble and the 3D shock-droplet: the two
     trajectories stay within 1e-9 relative L-infinity (per-step rounding
     differences that the shocks and interfaces amplify over time);
@@ -57,7 +58,7 @@ def test_fp64_fast_rhs(gpu, case):
         err = scaled_linf(prm, got, ref)
         adv = prm.num_fluids + prm.num_dims + 1
         err[:adv] = rel[:adv]
-        say(f"  {name} rhs: rel L-inf per variable (volume fractions on the divergence scale) "
+        say(f"  {name} rhs: rel L-inf per variable (volume fractions on the partial-density rhs scale) "
             f"{np.array2string(err, precision=2)}; plain rel {np.array2string(rel, precision=2)}")
         assert np.all(np.isfinite(got)) and err.max() <= TOL, err
     finally:
