@@ -120,12 +120,23 @@ __device__ __forceinline__ void cons_to_prim(const double *c, double *p, const E
         pinf = pinf + c[Y::ADV + i] * eos.pi_infs[i];
     }
     rho = max_ref(rho, 1e-16);
+    if constexpr (kFastF64) {
+        // 1/rho and 1/Gamma from one reciprocal of their product
+        const double r = frcp(rho * gam), irho = gam * r, igam = rho * r;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        p[Y::MOM + d] = bdiv(c[Y::MOM + d], rho);
-        dyn = dyn + 0.5 * c[Y::MOM + d] * p[Y::MOM + d];
+        for (int d = 0; d < D; ++d) {
+            p[Y::MOM + d] = c[Y::MOM + d] * irho;
+            dyn = dyn + 0.5 * c[Y::MOM + d] * p[Y::MOM + d];
+        }
+        p[Y::E] = (c[Y::E] - dyn - pinf) * igam;
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            p[Y::MOM + d] = bdiv(c[Y::MOM + d], rho);
+            dyn = dyn + 0.5 * c[Y::MOM + d] * p[Y::MOM + d];
+        }
+        p[Y::E] = bdiv(c[Y::E] - dyn - pinf, gam);
     }
-    p[Y::E] = bdiv(c[Y::E] - dyn - pinf, gam);
 #pragma unroll
     for (int i = 0; i < NF; ++i) p[Y::ADV + i] = c[Y::ADV + i];
 }
@@ -303,10 +314,20 @@ __device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &e
     }
     const R E_L = gamma_L * pres_L + pi_inf_L + R(0.5) * rho_L * vel_L_rms;
     const R E_R = gamma_R * pres_R + pi_inf_R + R(0.5) * rho_R * vel_R_rms;
-    const R H_L = bdiv(E_L + pres_L, rho_L);
-    const R H_R = bdiv(E_R + pres_R, rho_R);
-    const R c_L = bsqrt(bdiv(H_L - R(0.5) * vel_L_rms, gamma_L));
-    const R c_R = bsqrt(bdiv(H_R - R(0.5) * vel_R_rms, gamma_R));
+    R c_L, c_R;
+    if constexpr (kFastF64 && sizeof(R) == 8) {
+        // c^2 = (H - |u|^2/2)/Gamma = (E + p - rho |u|^2/2)/(rho Gamma), both
+        // sides from one reciprocal of the product of their denominators
+        const R gL = rho_L * gamma_L, gR = rho_R * gamma_R;
+        const R rg = frcp(gL * gR);
+        c_L = bsqrt((E_L + pres_L - R(0.5) * rho_L * vel_L_rms) * (gR * rg));
+        c_R = bsqrt((E_R + pres_R - R(0.5) * rho_R * vel_R_rms) * (gL * rg));
+    } else {
+        const R H_L = bdiv(E_L + pres_L, rho_L);
+        const R H_R = bdiv(E_R + pres_R, rho_R);
+        c_L = bsqrt(bdiv(H_L - R(0.5) * vel_L_rms, gamma_L));
+        c_R = bsqrt(bdiv(H_R - R(0.5) * vel_R_rms, gamma_R));
+    }
     const R uL = qL[Y::MOM + DIR], uR = qR[Y::MOM + DIR];
     const R s_L = min_ref(uL - c_L, uR - c_R);
     const R s_R = max_ref(uR + c_R, uL + c_L);
@@ -314,8 +335,21 @@ __device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &e
                        rho_L * (s_L - uL) - rho_R * (s_R - uR));
     const R s_M = min_ref(R(0), s_L);
     const R s_P = max_ref(R(0), s_R);
-    const R xi_L = bdiv(s_L - uL, s_L - s_S);
-    const R xi_R = bdiv(s_R - uR, s_R - s_S);
+    // 1/(s_L - u_L), 1/(s_R - u_R), 1/(s_L - s_S), 1/(s_R - s_S): in the
+    // contracted form from one reciprocal of their product
+    R xi_L, xi_R, iwL = R(0), iwR = R(0);
+    if constexpr (kFastF64 && sizeof(R) == 8) {
+        const R dL = s_L - uL, dR = s_R - uR, eL = s_L - s_S, eR = s_R - s_S;
+        const R pd = dL * dR, pe = eL * eR;
+        const R r4 = frcp(pd * pe);
+        xi_L = dL * (pd * eR * r4);
+        xi_R = dR * (pd * eL * r4);
+        iwL = dR * pe * r4;
+        iwR = dL * pe * r4;
+    } else {
+        xi_L = bdiv(s_L - uL, s_L - s_S);
+        xi_R = bdiv(s_R - uR, s_R - s_S);
+    }
     // Fortran sign(0.5, s_S) takes the sign bit of s_S, so -0.0 gives -0.5.
     const R hs = copysign(R(0.5), s_S);
     const R xi_M = (R(0.5) + hs);
@@ -332,8 +366,16 @@ __device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &e
             xi_M * (rho_L * (uL * vLd + s_M * (xi_L * (flg * s_S + (R(1) - flg) * vLd) - vLd)) + flg * (pres_L)) +
             xi_P * (rho_R * (uR * vRd + s_P * (xi_R * (flg * s_S + (R(1) - flg) * vRd) - vRd)) + flg * (pres_R));
     }
-    flux[Y::E] = xi_M * (uL * (E_L + pres_L) + s_M * (xi_L * (E_L + (s_S - uL) * (rho_L * s_S + bdiv(pres_L, s_L - uL))) - E_L)) +
-                 xi_P * (uR * (E_R + pres_R) + s_P * (xi_R * (E_R + (s_S - uR) * (rho_R * s_S + bdiv(pres_R, s_R - uR))) - E_R));
+    R pdL, pdR; // p_L/(s_L - u_L), p_R/(s_R - u_R)
+    if constexpr (kFastF64 && sizeof(R) == 8) {
+        pdL = pres_L * iwL;
+        pdR = pres_R * iwR;
+    } else {
+        pdL = bdiv(pres_L, s_L - uL);
+        pdR = bdiv(pres_R, s_R - uR);
+    }
+    flux[Y::E] = xi_M * (uL * (E_L + pres_L) + s_M * (xi_L * (E_L + (s_S - uL) * (rho_L * s_S + pdL)) - E_L)) +
+                 xi_P * (uR * (E_R + pres_R) + s_P * (xi_R * (E_R + (s_S - uR) * (rho_R * s_S + pdR)) - E_R));
 #pragma unroll
     for (int i = 0; i < NF; ++i) flux[Y::ADV + i] = xi_M * qL[Y::ADV + i] * mL + xi_P * qR[Y::ADV + i] * mR;
     // dir_flg = 1 for the normal component: xi_M*(u_L + 1*s_M*(xi_L-1)) + ...
