@@ -254,11 +254,19 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
             // beta^2, no reciprocal per candidate (weno_eps >= 1e-50 keeps
             // the products normal)
             const R q12 = bb1 * bb2, q02 = bb0 * bb2, q01 = bb0 * bb1;
-            if constexpr (WANT_L) {
+            if constexpr (WANT_L && WANT_R) {
+                // both faces' normalisations from one reciprocal of the
+                // product of the two sums
+                const R a0 = cf[kDL + 0] * q12, a1 = cf[kDL + 1] * q02, a2 = cf[kDL + 2] * q01;
+                const R b0 = cf[kDR + 0] * q12, b1 = cf[kDR + 1] * q02, b2 = cf[kDR + 2] * q01;
+                const R sL = a0 + a1 + a2, sR = b0 + b1 + b2;
+                const R r = frcp(sL * sR);
+                vL = (a0 * pL[0] + a1 * pL[1] + a2 * pL[2]) * (sR * r);
+                vR = (b0 * pR[0] + b1 * pR[1] + b2 * pR[2]) * (sL * r);
+            } else if constexpr (WANT_L) {
                 const R a0 = cf[kDL + 0] * q12, a1 = cf[kDL + 1] * q02, a2 = cf[kDL + 2] * q01;
                 vL = (a0 * pL[0] + a1 * pL[1] + a2 * pL[2]) * frcp(a0 + a1 + a2);
-            }
-            if constexpr (WANT_R) {
+            } else if constexpr (WANT_R) {
                 const R a0 = cf[kDR + 0] * q12, a1 = cf[kDR + 1] * q02, a2 = cf[kDR + 2] * q01;
                 vR = (a0 * pR[0] + a1 * pR[1] + a2 * pR[2]) * frcp(a0 + a1 + a2);
             }

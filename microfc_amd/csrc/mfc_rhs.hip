@@ -214,12 +214,13 @@ int check_params(const mfc_params &p) {
         return fail(-1, "weno_eps below 1e-100 is not supported in FP64 (the weight division is exact for "
                         "weno_eps >= 1e-100)");
     // The contracted form scales the WENO5 weights by the product of the
-    // three squared smoothness indicators; its pairwise products (>=
-    // weno_eps^4) stay normal for weno_eps >= 1e-50 (and finite while the
-    // stencil differences stay below about 1e38).
-    if (p.precision == MFC_PREC_FP64_FAST && p.weno_order > 1 && !(p.weno_eps >= 1e-50))
-        return fail(-1, "weno_eps below 1e-50 is not supported in the contracted FP64 form (the scaled weights "
-                        "need weno_eps^4 normal)");
+    // three squared smoothness indicators and normalises both faces of a
+    // cell with one reciprocal of the product of their weight sums (>=
+    // weno_eps^8 / 100): normal for weno_eps >= 1e-25, finite while the
+    // stencil differences stay below about 1e18.
+    if (p.precision == MFC_PREC_FP64_FAST && p.weno_order > 1 && !(p.weno_eps >= 1e-25))
+        return fail(-1, "weno_eps below 1e-25 is not supported in the contracted FP64 form (the scaled weights "
+                        "and the product of a cell's two weight sums must stay normal)");
     // In FP32, beta^2 >= weno_eps^2 must stay a normal float and d/beta^2 finite.
     if (!f64 && p.weno_order > 1 && !(p.weno_eps >= 1e-18))
         return fail(-1, "weno_eps below 1e-18 is not supported in the FP32 forms (its square underflows)");

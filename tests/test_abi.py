@@ -59,7 +59,7 @@ def _init(**kw):
     (dict(precision=2, weno_order=3), "weno_order = 5"),
     (dict(precision=7), "precision"),
     (dict(precision=1, weno_eps=1e-20), "weno_eps below 1e-18"),
-    (dict(precision=3, weno_eps=1e-60), "weno_eps below 1e-50"),
+    (dict(precision=3, weno_eps=1e-30), "weno_eps below 1e-25"),
 ])
 def test_invalid_cases_are_rejected(kw, msg):
     rc, err = _init(**kw)
