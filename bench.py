@@ -41,11 +41,12 @@ FP64_PEAK_TFLOPS = 78.6
 _P512 = "profiles/sweep_kernels_shock_droplet_3d_512_fp64_fast.md"
 PROFILED = {
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_x"): {
-        "traffic": 23.13e9 + 8.70e9, "valu_busy": 0.79, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
+        "traffic": 23.13e9 + 8.60e9, "valu_busy": 0.69,
+        "limiter": "VALU issue (FP64) and the RK stage's loads, not HBM bandwidth", "profile": _P512},
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_y"): {
-        "traffic": 17.44e9 + 8.59e9, "valu_busy": 0.85, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
+        "traffic": 17.44e9 + 8.59e9, "valu_busy": 0.80, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_z"): {
-        "traffic": 8.99e9 + 8.59e9, "valu_busy": 0.83, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
+        "traffic": 8.98e9 + 8.59e9, "valu_busy": 0.78, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
     ("shock_bubble_2d", 2048, "fp64", "sweep_y"): {
         "traffic": 678.5e6 + 236.6e6, "valu_busy": 0.79, "limiter": "VALU issue (FP64), not HBM",
         "profile": "profiles/sweep_kernels_shock_bubble_2d_2048_fp64.md"},
