@@ -64,7 +64,7 @@ FORMS = {"fp64": "plain FP64: the reference's evaluation order, no FMA contracti
 def sweep_order(prm):
     """The library's sweep order: x, y, z as the reference adds them; the
     contracted FP64 form of an inviscid case runs z, y, x, so that the RK
-    stage is fused into the x sweep."""
+    stage's output is written by the x sweep."""
     names = ["sweep_x", "sweep_y", "sweep_z"][:prm.num_dims]
     return names[::-1] if prm.precision == 3 and not prm.viscous else names
 
@@ -72,8 +72,10 @@ def sweep_order(prm):
 def sweep_bytes_per_step(prm):
     """Algorithmic HBM bytes per RK3 step of each kernel, per cell: the
     stencil input is read once (S doubles), rhs is written by the first
-    sweep, read and written by middle ones, read by the last, which reads the
-    RK base (stages 2 and 3) and writes the stage output.  Viscous runs use
+    sweep, read and written by middle ones, read by the last, which writes
+    the stage output; the RK base (stages 2 and 3) is read by the last sweep,
+    or, in the contracted FP64 form of a 2D/3D case, which spreads the stage
+    over the sweeps, by the first.  Viscous runs use
     the viscous kernels in place of the sweeps: per direction the face kernel
     reads the stencil input and writes a face record (HLLC flux, contact
     velocity, viscous flux: S+2+D doubles), the finish reads it and the
@@ -84,6 +86,7 @@ def sweep_bytes_per_step(prm):
     S, D, NF = prm.sys_size, prm.num_dims, prm.num_fluids
     out = {}
     names = sweep_order(prm)
+    fold = prm.precision == 3 and D > 1
     if prm.viscous:
         total = 0
         for d in range(D):
@@ -106,9 +109,9 @@ def sweep_bytes_per_step(prm):
             w = 0
             if not first:
                 r += S  # rhs partial
+            if stage > 0 and (first if fold else last):
+                r += S  # RK base q1
             if last:
-                if stage > 0:
-                    r += S  # RK base q1
                 w += S  # stage output
             else:
                 w += S  # rhs

@@ -833,6 +833,19 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
                             hipGetErrorString(hipGetLastError()));
         }
     }
+    // The contracted form spreads an RK stage over its sweeps (out = cb*base
+    // + ci*in + cdt*L; the first sweep reads the base, the last the stage
+    // input's own cells, which it stages anyway), so the x sweep does not
+    // carry a fourth array.
+    const bool fold = reverse && rk != kRkNone && G.D > 1;
+    double cb = 1.0, ci = 0.0, cdt = dt;
+    if (rk == kRk2Final) {
+        cb = 0.5, ci = 0.5, cdt = 0.5 * dt;
+    } else if (rk == kRk3Stage2) {
+        cb = 0.75, ci = 0.25, cdt = 0.25 * dt;
+    } else if (rk == kRk3Stage3) {
+        cb = 1.0 / 3.0, ci = 2.0 / 3.0, cdt = 2.0 * dt / 3.0;
+    }
     for (int k = 0; k < G.D; ++k) {
         const int d = reverse ? G.D - 1 - k : k;
         SweepArgs a = sweep_args(d);
@@ -841,11 +854,17 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         a.accumulate = k > 0;
         a.dt = dt;
         const bool last = k == G.D - 1;
-        a.rk = last ? rk : kRkNone;
-        if (a.rk != kRkNone) {
+        a.rk = last && !fold ? rk : kRkNone;
+        if (a.rk != kRkNone || fold) {
             a.qbase = *qbase;
             a.qout = *qout;
             a.base_is_in = qbase->v[0] == qin.v[0];
+        }
+        if (fold) {
+            a.fold = k == 0 ? 1 : last ? 3 : 2;
+            a.cb = cb;
+            a.ci = ci;
+            a.cdt = cdt;
         }
         if (int rc = G.visc ? viscous(d, qin, rhs, a) : sweep(d, a)) return rc;
     }
@@ -879,6 +898,12 @@ int mfc_rhs_init(const mfc_params *prm) {
     G.nyg = prm->n > 0 ? prm->n + 2 * G.b + 1 : 1;
     G.nzg = prm->p > 0 ? prm->p + 2 * G.b + 1 : 1;
     G.ng = G.nxg * G.nyg * G.nzg;
+    // The sweeps address a block's cells as a uniform base plus a 32-bit
+    // byte offset per lane (Off in mfc_sweep.hpp); a z-sweep block spans at
+    // most 40 ghosted planes
+    if (G.nzg > 1 && 40L * G.nxg * G.nyg * (long)sizeof(double) >= (1L << 32))
+        return fail(-1, "a z plane of %ld x %ld ghosted cells exceeds the sweeps' 32-bit block offsets", (long)G.nxg,
+                    (long)G.nyg);
     G.nint = (long)(prm->m + 1) * (prm->n + 1) * (prm->p + 1);
     G.dir[0].s = prm->m;
     G.dir[1].s = prm->n;

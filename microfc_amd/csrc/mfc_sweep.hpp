@@ -61,6 +61,13 @@ struct SweepArgs {
     int rk;               // RkKind
     int accumulate;       // 0: first direction assigns rhs, 1: adds
     int base_is_in;       // the RK base is the stage input (first stages)
+    // Contracted FP64 form, 2D/3D: an RK stage out = cb*base + ci*in +
+    // cdt*L spread over the sweeps (rk = kRkNone in each).  The first sweep
+    // stores cb*base + cdt*L_d into rhs, the middle one adds cdt*L_d, the
+    // last writes rhs + cdt*L_d + ci*in to qout, so the base is read by the
+    // first sweep instead of the last.
+    int fold;             // 0: off, 1: first sweep, 2: middle, 3: last
+    double cb, ci, cdt;
     int chunk, nchunk;    // cells per tile along the sweep and tile count (k_sweep_t)
     int ntile;            // x tiles (k_sweep_x) or x column groups (k_sweep_t)
     long nblocks;
@@ -114,6 +121,26 @@ __device__ __forceinline__ long xcd_logical_block(long bid, long nblocks) {
 template <typename R>
 __device__ __forceinline__ double to_d(R x) { return (double)x; }
 
+// An element offset split into a part that is the same on every lane of a
+// block and a 32-bit per-lane part.  el() adds the uniform part to the
+// array's address in scalar registers and the lane part as a 32-bit byte
+// offset, so the load or store takes the scalar-base global address form
+// with no 64-bit address arithmetic per lane.  The lane part must stay
+// below 2^29 elements (the host checks the largest block span).
+struct Off {
+    long base;    // elements, uniform
+    unsigned rel; // elements, per lane
+};
+__device__ __forceinline__ Off lane_off(long o) { return Off{o, 0u}; }
+template <typename T>
+__device__ __forceinline__ T &el(T *p, Off o) {
+    unsigned b = o.rel * (unsigned)sizeof(T);
+    // opaque to the optimiser, so the 32-bit offset is not widened before
+    // the address is formed (a widened one takes a 64-bit VALU add)
+    asm("" : "+v"(b));
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(p + o.base) + b);
+}
+
 // Operands of finish_cell that come from HBM: the rhs accumulated by the
 // earlier sweeps, the RK base and the stage input (the volume fractions of
 // the stage input feed the alpha*div(u) source in every sweep).
@@ -122,48 +149,62 @@ struct FinishIn {
     double rhs[S], base[S], in[S];
 };
 
-template <int NF, int D>
-__device__ __forceinline__ void finish_load(const SweepArgs &a, long go, long ro, FinishIn<Sys<NF, D>::S> &f) {
+// Which finish modes a sweep kernel can meet.  The contracted FP64 form of a
+// 2D/3D case never passes an RK stage to a sweep: it spreads the stage over
+// the sweeps (SweepArgs::fold), its x sweep, always the last, meets folds 0
+// and 3, its y/z sweeps folds 0, 1 and 2.  Every other form and the 1D
+// contracted x sweep meet rk and never fold.  Fixing the set at compile
+// time keeps the modes a kernel cannot meet out of its code (and its
+// registers).
+enum FinishModes { kModesAll = 0, kModesFoldLast = 1, kModesFoldFirst = 2 };
+template <int D, bool LAST>
+constexpr int finish_modes() {
+    return !kFastF64 || D == 1 ? kModesAll : LAST ? kModesFoldLast : kModesFoldFirst;
+}
+
+template <int NF, int D, int M>
+__device__ __forceinline__ void finish_load(const SweepArgs &a, Off go, Off ro, FinishIn<Sys<NF, D>::S> &f) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     if (a.accumulate) {
 #pragma unroll
-        for (int v = 0; v < S; ++v) f.rhs[v] = a.rhs.v[v][ro];
+        for (int v = 0; v < S; ++v) f.rhs[v] = el(a.rhs.v[v], ro);
     }
-    if (a.rk == kRkNone || a.rk == kRkEuler) {
+    const bool fold_in = M == kModesFoldLast && a.fold == 3 && a.ci != 0.0;
+    if ((a.rk == kRkNone || a.rk == kRkEuler) && !fold_in) {
 #pragma unroll
-        for (int i = 0; i < NF; ++i) f.in[Y::ADV + i] = a.qin.v[Y::ADV + i][go];
+        for (int i = 0; i < NF; ++i) f.in[Y::ADV + i] = el(a.qin.v[Y::ADV + i], go);
     } else {
 #pragma unroll
-        for (int v = 0; v < S; ++v) f.in[v] = a.qin.v[v][go];
+        for (int v = 0; v < S; ++v) f.in[v] = el(a.qin.v[v], go);
     }
-    if (a.rk != kRkNone) {
+    if ((M == kModesAll && a.rk != kRkNone) || (M == kModesFoldFirst && a.fold == 1)) {
 #pragma unroll
-        for (int v = 0; v < S; ++v) f.base[v] = a.qbase.v[v][go];
+        for (int v = 0; v < S; ++v) f.base[v] = el(a.qbase.v[v], go);
     }
 }
 
 // The same with the stage input of the cell taken from the copy a sweep
 // staged in LDS (element v at cin[v * stride]); the RK base too when it is
 // the stage input.
-template <int NF, int D>
-__device__ __forceinline__ void finish_load_staged(const SweepArgs &a, long go, long ro, const double *cin, int stride,
+template <int NF, int D, int M>
+__device__ __forceinline__ void finish_load_staged(const SweepArgs &a, Off go, Off ro, const double *cin, int stride,
                                                    FinishIn<Sys<NF, D>::S> &f) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     if (a.accumulate) {
 #pragma unroll
-        for (int v = 0; v < S; ++v) f.rhs[v] = a.rhs.v[v][ro];
+        for (int v = 0; v < S; ++v) f.rhs[v] = el(a.rhs.v[v], ro);
     }
 #pragma unroll
     for (int v = 0; v < S; ++v) f.in[v] = cin[v * stride];
-    if (a.rk != kRkNone) {
+    if ((M == kModesAll && a.rk != kRkNone) || (M == kModesFoldFirst && a.fold == 1)) {
         if (a.base_is_in) {
 #pragma unroll
             for (int v = 0; v < S; ++v) f.base[v] = f.in[v];
         } else {
 #pragma unroll
-            for (int v = 0; v < S; ++v) f.base[v] = a.qbase.v[v][go];
+            for (int v = 0; v < S; ++v) f.base[v] = el(a.qbase.v[v], go);
         }
     }
 }
@@ -203,19 +244,48 @@ __device__ __forceinline__ void rk_combine(const SweepArgs &a, const FinishIn<S>
 
 // Finishes one cell: flux divergence, alpha*div(u) source, then either the
 // rhs store or the fused RK stage (m_rhs.fpp:565-652, m_time_steppers.fpp).
-template <int NF, int D, typename R>
-__device__ __forceinline__ void finish_cell(const SweepArgs &a, long go, long ro, double inv,
+template <int NF, int D, typename R, int M>
+__device__ __forceinline__ void finish_cell(const SweepArgs &a, Off go, Off ro, double inv,
                                             const R *Fm, const R *Fp, R usm, R usp,
                                             const FinishIn<Sys<NF, D>::S> &f) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     double r[S];
+    const double dus = to_d(usp) - to_d(usm);
+    if constexpr (M != kModesAll) {
+        if (a.fold) {
+            // sc*dF: this direction's part of cdt*L
+            const double sc = a.cdt * inv;
+#pragma unroll
+            for (int v = 0; v < S; ++v) {
+                const double df = to_d(Fm[v]) - to_d(Fp[v]);
+                if constexpr (M == kModesFoldFirst)
+                    r[v] = a.fold == 1 ? a.cb * f.base[v] + sc * df : f.rhs[v] + sc * df;
+                else
+                    r[v] = f.rhs[v] + sc * df;
+            }
+#pragma unroll
+            for (int i = 0; i < NF; ++i) r[Y::ADV + i] = r[Y::ADV + i] + sc * f.in[Y::ADV + i] * dus;
+            if constexpr (M == kModesFoldFirst) {
+#pragma unroll
+                for (int v = 0; v < S; ++v) el(a.rhs.v[v], ro) = r[v];
+            } else if (a.ci != 0.0) {
+#pragma unroll
+                for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = r[v] + a.ci * f.in[v];
+            } else {
+                // Euler: the stage input is not part of the output (and
+                // finish_load leaves its non-advected entries unset)
+#pragma unroll
+                for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = r[v];
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int v = 0; v < S; ++v) {
         const double df = to_d(Fm[v]) - to_d(Fp[v]);
         r[v] = a.accumulate ? f.rhs[v] + inv * df : inv * df;
     }
-    const double dus = to_d(usp) - to_d(usm);
 #pragma unroll
     for (int i = 0; i < NF; ++i) r[Y::ADV + i] = r[Y::ADV + i] + inv * f.in[Y::ADV + i] * dus;
     if constexpr (!std::is_same<R, double>::value) {
@@ -227,30 +297,36 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, long go, long ro
 #pragma unroll
             for (int i = 0; i < NF; ++i) o[Y::ADV + i] = min(max(o[Y::ADV + i], 0.0), 1.0);
 #pragma unroll
-            for (int v = 0; v < S; ++v) a.qout.v[v][go] = o[v];
+            for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = o[v];
             return;
         }
+    }
+    if constexpr (M != kModesAll) {
+        // no RK stage reaches these sweeps
+#pragma unroll
+        for (int v = 0; v < S; ++v) el(a.rhs.v[v], ro) = r[v];
+        return;
     }
     switch (a.rk) {
     case kRkNone:
 #pragma unroll
-        for (int v = 0; v < S; ++v) a.rhs.v[v][ro] = r[v];
+        for (int v = 0; v < S; ++v) el(a.rhs.v[v], ro) = r[v];
         break;
     case kRkEuler:
 #pragma unroll
-        for (int v = 0; v < S; ++v) a.qout.v[v][go] = f.base[v] + a.dt * r[v];
+        for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = f.base[v] + a.dt * r[v];
         break;
     case kRk2Final:
 #pragma unroll
-        for (int v = 0; v < S; ++v) a.qout.v[v][go] = (f.base[v] + f.in[v] + a.dt * r[v]) / 2.0;
+        for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = (f.base[v] + f.in[v] + a.dt * r[v]) / 2.0;
         break;
     case kRk3Stage2:
 #pragma unroll
-        for (int v = 0; v < S; ++v) a.qout.v[v][go] = (3.0 * f.base[v] + f.in[v] + a.dt * r[v]) / 4.0;
+        for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = (3.0 * f.base[v] + f.in[v] + a.dt * r[v]) / 4.0;
         break;
     default:
 #pragma unroll
-        for (int v = 0; v < S; ++v) a.qout.v[v][go] = third(f.base[v] + 2.0 * f.in[v] + 2.0 * a.dt * r[v]);
+        for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = third(f.base[v] + 2.0 * f.in[v] + 2.0 * a.dt * r[v]);
         break;
     }
 }
@@ -415,6 +491,7 @@ template <int NF, int D, int WO, typename R, bool ML, bool UNI>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 8))) void k_sweep_x(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
+    constexpr int FM = finish_modes<D, true>();
     constexpr int PN = (WO - 1) / 2;
     constexpr int NC = 64 + 2 * PN + 1;
     constexpr int TW = x_tile_cells<ML, R, UNI>();
@@ -462,13 +539,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
     } else {
         constexpr int NIT = (NC + 63) / 64;
         double cv[NIT][S];
+        // staging entry c is cell x0-1-PN+c (entry 1+PN: cell x0)
+        const long sb = ghost_off(a, x0 - 1 - PN, j, k);
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int c = lane + 64 * it, x = x0 - 1 - PN + c;
             const bool ok = c < NC && x <= a.m + a.b;
-            const long go = ghost_off(a, ok ? x : x0, j, k);
+            const Off go{sb, (unsigned)(ok ? c : 1 + PN)};
 #pragma unroll
-            for (int v = 0; v < S; ++v) cv[it][v] = ok ? a.qin.v[v][go] : 0.0;
+            for (int v = 0; v < S; ++v) cv[it][v] = ok ? el(a.qin.v[v], go) : 0.0;
         }
         // cell x0+lane is staging entry lane+1+PN: loaded by lane
         // (lane+1+PN) mod 64 in pass 0, or in pass 1 for the last 1+PN lanes
@@ -585,10 +664,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
         const R usp = __shfl_down(us, 1, 64);
         const int i = x0 + lane;
         if (lane < TW && i <= a.m) {
+            const Off go{ghost_off(a, x0, j, k), (unsigned)lane}, ro{interior_off(a, x0, j, k), (unsigned)lane};
             FinishIn<S> fin;
-            finish_load_staged<NF, D>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), cin, 1, fin);
-            finish_cell<NF, D, R>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), a.inv_ds[i], f, fp, us, usp,
-                                  fin);
+            finish_load_staged<NF, D, FM>(a, go, ro, cin, 1, fin);
+            finish_cell<NF, D, R, FM>(a, go, ro, a.inv_ds[i], f, fp, us, usp, fin);
         }
         return;
     }
@@ -618,13 +697,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
             Fm[v] = sf[v][lane];
             Fp[v] = sf[v][lane + 1];
         }
+        const Off go{ghost_off(a, x0, j, k), (unsigned)lane}, ro{interior_off(a, x0, j, k), (unsigned)lane};
         FinishIn<S> fin;
         if constexpr (kStagedFinish)
-            finish_load_staged<NF, D>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), cin, 1, fin);
+            finish_load_staged<NF, D, FM>(a, go, ro, cin, 1, fin);
         else
-            finish_load<NF, D>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), fin);
-        finish_cell<NF, D, R>(a, ghost_off(a, i, j, k), interior_off(a, i, j, k), a.inv_ds[i], Fm, Fp,
-                              sf[S][lane], sf[S][lane + 1], fin);
+            finish_load<NF, D, FM>(a, go, ro, fin);
+        finish_cell<NF, D, R, FM>(a, go, ro, a.inv_ds[i], Fm, Fp, sf[S][lane], sf[S][lane + 1], fin);
     }
 }
 
@@ -662,6 +741,7 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
     static_assert(!ML, "the learned form runs the cell-parallel sweep");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
+    constexpr int FM = finish_modes<D, false>();
     constexpr int kTS = t_faces<ML>();
     constexpr int NT = t_threads<ML>();
     constexpr int PN = (WO - 1) / 2;
@@ -806,11 +886,12 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
         }
         FinishIn<S> fin;
         if constexpr (KEEP)
-            finish_load_staged<NF, D>(a, gcell(i, cc), rcell(i, cc), &lcons[0][ts][tx], (kTS - 1) * kTX, fin);
+            finish_load_staged<NF, D, FM>(a, lane_off(gcell(i, cc)), lane_off(rcell(i, cc)), &lcons[0][ts][tx],
+                                          (kTS - 1) * kTX, fin);
         else
-            finish_load<NF, D>(a, gcell(i, cc), rcell(i, cc), fin);
-        finish_cell<NF, D, R>(a, gcell(i, cc), rcell(i, cc), a.inv_ds[cc], Fm, Fp, sf[S][ts][tx],
-                              sf[S][ts + 1][tx], fin);
+            finish_load<NF, D, FM>(a, lane_off(gcell(i, cc)), lane_off(rcell(i, cc)), fin);
+        finish_cell<NF, D, R, FM>(a, lane_off(gcell(i, cc)), lane_off(rcell(i, cc)), a.inv_ds[cc], Fm, Fp,
+                                  sf[S][ts][tx], sf[S][ts + 1][tx], fin);
     }
 }
 
@@ -837,6 +918,7 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
     static_assert(t_cell_parallel<ML, R, UNI>(), "cell-parallel forms only");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
+    constexpr int FM = finish_modes<D, false>();
     constexpr int XC = tc_cols<R>(DIR), kTS = tc_rows<R>(DIR), NT = XC * kTS;
     constexpr int TC = t_tile_cells<ML, R, UNI>(DIR);
     constexpr int PN = (WO - 1) / 2;
@@ -874,6 +956,13 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
         return (DIR == 1) ? interior_off(a, i, c, t) : interior_off(a, i, t, c);
     };
 
+    // Offsets within the block's cells: staged cell (x0 + ex, y0-1-PN + er)
+    // at sb + ex + er*gp, interior cell (x0 + ex, y0 + er) at rb + ex + er*rp
+    // (uniform bases, 32-bit lane parts)
+    const long sb = gcell(x0, y0 - 1 - PN), rb = rcell(x0, y0);
+    const unsigned gp = (unsigned)((DIR == 1) ? a.nxg : a.nxg * a.nyg);
+    const unsigned rp = (unsigned)((DIR == 1) ? a.m + 1 : (a.m + 1) * (a.n + 1));
+
     // 1. stage primitive rows y0-1-PN .. y0+kTS-2+PN of columns x0..x0+XC-1
     // (loads along x, every load issued before the first conversion)
     {
@@ -885,9 +974,9 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
             const int ex = e % XC, er = e / XC;
             const int c = y0 - 1 - PN + er;
             const bool ok = e < NR * XC && c <= s + a.b;
-            const long go = gcell(min(x0 + ex, a.m), ok ? c : y0);
+            const Off go{sb, (unsigned)(min(x0 + ex, a.m) - x0) + (unsigned)(ok ? er : 1 + PN) * gp};
 #pragma unroll
-            for (int v = 0; v < S; ++v) cv[it][v] = ok ? a.qin.v[v][go] : 0.0;
+            for (int v = 0; v < S; ++v) cv[it][v] = ok ? el(a.qin.v[v], go) : 0.0;
         }
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
@@ -984,13 +1073,13 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
             Fm[v] = fl[v][fs][fx];
             Fp[v] = fl[v][fs + 1][fx];
         }
+        const Off go{sb, (unsigned)fx + (unsigned)(fs + 1 + PN) * gp}, ro{rb, (unsigned)fx + (unsigned)fs * rp};
         FinishIn<S> fin;
         if constexpr (KEEP)
-            finish_load_staged<NF, D>(a, gcell(i, cc), rcell(i, cc), &lcons[0][fx][fs], XC * KP, fin);
+            finish_load_staged<NF, D, FM>(a, go, ro, &lcons[0][fx][fs], XC * KP, fin);
         else
-            finish_load<NF, D>(a, gcell(i, cc), rcell(i, cc), fin);
-        finish_cell<NF, D, R>(a, gcell(i, cc), rcell(i, cc), a.inv_ds[cc], Fm, Fp, fl[S][fs][fx], fl[S][fs + 1][fx],
-                              fin);
+            finish_load<NF, D, FM>(a, go, ro, fin);
+        finish_cell<NF, D, R, FM>(a, go, ro, a.inv_ds[cc], Fm, Fp, fl[S][fs][fx], fl[S][fs + 1][fx], fin);
     }
 }
 
