@@ -75,7 +75,7 @@ def sweep_bytes_per_step(prm):
     sweep, read and written by middle ones, read by the last, which writes
     the stage output; the RK base (stages 2 and 3) is read by the last sweep,
     or, in the contracted FP64 form of a 2D/3D case, which spreads the stage
-    over the sweeps, by the first.  Viscous runs use
+    over the sweeps, by the y sweep.  Viscous runs use
     the viscous kernels in place of the sweeps: per direction the face kernel
     reads the stencil input and writes a face record (HLLC flux, contact
     velocity, viscous flux: S+2+D doubles), the finish reads it and the
@@ -109,7 +109,7 @@ def sweep_bytes_per_step(prm):
             w = 0
             if not first:
                 r += S  # rhs partial
-            if stage > 0 and (first if fold else last):
+            if stage > 0 and (nm == "sweep_y" if fold else last):
                 r += S  # RK base q1
             if last:
                 w += S  # stage output

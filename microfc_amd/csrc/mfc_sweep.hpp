@@ -63,10 +63,11 @@ struct SweepArgs {
     int base_is_in;       // the RK base is the stage input (first stages)
     // Contracted FP64 form, 2D/3D: an RK stage out = cb*base + ci*in +
     // cdt*L spread over the sweeps (rk = kRkNone in each).  The first sweep
-    // stores cb*base + cdt*L_d into rhs, the middle one adds cdt*L_d, the
-    // last writes rhs + cdt*L_d + ci*in to qout, so the base is read by the
-    // first sweep instead of the last.
+    // stores cdt*L_d into rhs, the middle one adds cdt*L_d, the last writes
+    // rhs + cdt*L_d + ci*in to qout, and one sweep before the last also adds
+    // cb*base, so the last sweep does not read the base.
     int fold;             // 0: off, 1: first sweep, 2: middle, 3: last
+    int fold_base;        // this sweep adds cb*base
     double cb, ci, cdt;
     int chunk, nchunk;    // cells per tile along the sweep and tile count (k_sweep_t)
     int ntile;            // x tiles (k_sweep_x) or x column groups (k_sweep_t)
@@ -178,7 +179,7 @@ __device__ __forceinline__ void finish_load(const SweepArgs &a, Off go, Off ro, 
 #pragma unroll
         for (int v = 0; v < S; ++v) f.in[v] = el(a.qin.v[v], go);
     }
-    if ((M == kModesAll && a.rk != kRkNone) || (M == kModesFoldFirst && a.fold == 1)) {
+    if ((M == kModesAll && a.rk != kRkNone) || (M == kModesFoldFirst && a.fold_base)) {
 #pragma unroll
         for (int v = 0; v < S; ++v) f.base[v] = el(a.qbase.v[v], go);
     }
@@ -198,7 +199,7 @@ __device__ __forceinline__ void finish_load_staged(const SweepArgs &a, Off go, O
     }
 #pragma unroll
     for (int v = 0; v < S; ++v) f.in[v] = cin[v * stride];
-    if ((M == kModesAll && a.rk != kRkNone) || (M == kModesFoldFirst && a.fold == 1)) {
+    if ((M == kModesAll && a.rk != kRkNone) || (M == kModesFoldFirst && a.fold_base)) {
         if (a.base_is_in) {
 #pragma unroll
             for (int v = 0; v < S; ++v) f.base[v] = f.in[v];
@@ -206,6 +207,30 @@ __device__ __forceinline__ void finish_load_staged(const SweepArgs &a, Off go, O
 #pragma unroll
             for (int v = 0; v < S; ++v) f.base[v] = el(a.qbase.v[v], go);
         }
+    }
+}
+
+// The same for a contracted 2D/3D y/z sweep (folds 0, 1, 2), which keeps
+// only the stage input's volume fractions in LDS (element i at
+// cadv[i * stride]): the finish needs no other stage-input value, and the
+// sweep that adds the RK base reads it from HBM (from L2 in the first
+// stage, whose base is the input this block staged).  Keeping all S
+// variables took 36.9 KB of LDS per 256-thread block in 3D, four blocks
+// per CU; the volume fractions alone leave 25.3 KB, five.
+template <int NF, int D>
+__device__ __forceinline__ void finish_load_adv(const SweepArgs &a, Off go, Off ro, const double *cadv, int stride,
+                                                FinishIn<Sys<NF, D>::S> &f) {
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    if (a.accumulate) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) f.rhs[v] = el(a.rhs.v[v], ro);
+    }
+#pragma unroll
+    for (int i = 0; i < NF; ++i) f.in[Y::ADV + i] = cadv[i * stride];
+    if (a.fold_base) {
+#pragma unroll
+        for (int v = 0; v < S; ++v) f.base[v] = el(a.qbase.v[v], go);
     }
 }
 
@@ -259,9 +284,15 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, Off go, Off ro, 
 #pragma unroll
             for (int v = 0; v < S; ++v) {
                 const double df = to_d(Fm[v]) - to_d(Fp[v]);
-                if constexpr (M == kModesFoldFirst)
-                    r[v] = a.fold == 1 ? a.cb * f.base[v] + sc * df : f.rhs[v] + sc * df;
-                else
+                if constexpr (M == kModesFoldFirst) {
+                    // sc*dF enters through the last fused multiply-add: in
+                    // a momentum component that is small against the
+                    // pressure, dF is a cancellation whose separate
+                    // rounding doubled the error against the oracle
+                    double acc = a.fold == 1 ? 0.0 : f.rhs[v];
+                    if (a.fold_base) acc = fma(a.cb, f.base[v], acc);
+                    r[v] = fma(sc, df, acc);
+                } else
                     r[v] = f.rhs[v] + sc * df;
             }
 #pragma unroll
@@ -932,9 +963,12 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
     // while four 256-thread blocks (or the same threads in fewer blocks)
     // still fit a CU (not in the learned form, whose MLP staging takes
     // that LDS)
-    constexpr bool KEEP = !ML && sizeof(R) * WORDS + 8 * S * XC * KP <= 40 * 1024 * (NT / 256);
+    // (the contracted 2D/3D sweeps keep only the volume fractions)
+    constexpr bool ADVK = FM == kModesFoldFirst;
+    constexpr int KS = ADVK ? NF : S;
+    constexpr bool KEEP = !ML && sizeof(R) * WORDS + 8 * KS * XC * KP <= 40 * 1024 * (NT / 256);
     __shared__ R lds[WORDS];
-    __shared__ double lcons[KEEP ? S : 1][XC][KEEP ? KP : 1];
+    __shared__ double lcons[KEEP ? KS : 1][XC][KEEP ? KP : 1];
     R(*sp)[XC][P] = reinterpret_cast<R(*)[XC][P]>(lds);
     R(*fl)[kTS][DP] = reinterpret_cast<R(*)[kTS][DP]>(lds);
 
@@ -990,7 +1024,7 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
                 if constexpr (KEEP) {
                     if (er > PN && er <= PN + TC) {
 #pragma unroll
-                        for (int v = 0; v < S; ++v) lcons[v][ex][er - PN - 1] = cv[it][v];
+                        for (int v = 0; v < KS; ++v) lcons[v][ex][er - PN - 1] = cv[it][ADVK ? Y::ADV + v : v];
                     }
                 }
             }
@@ -1075,7 +1109,9 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
         }
         const Off go{sb, (unsigned)fx + (unsigned)(fs + 1 + PN) * gp}, ro{rb, (unsigned)fx + (unsigned)fs * rp};
         FinishIn<S> fin;
-        if constexpr (KEEP)
+        if constexpr (KEEP && ADVK)
+            finish_load_adv<NF, D>(a, go, ro, &lcons[0][fx][fs], XC * KP, fin);
+        else if constexpr (KEEP)
             finish_load_staged<NF, D, FM>(a, go, ro, &lcons[0][fx][fs], XC * KP, fin);
         else
             finish_load<NF, D, FM>(a, go, ro, fin);
