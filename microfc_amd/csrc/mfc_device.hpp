@@ -233,9 +233,22 @@ __device__ __forceinline__ void weno_cell(const R *v, const R *cf, R eps, R &vL,
     } else {
         const R vj = v[2];
         const R d1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];
-        const R b0 = cf[kBeta + 0] * d1 * d1 + cf[kBeta + 1] * d1 * d0 + cf[kBeta + 2] * d0 * d0 + eps;
-        const R b1 = cf[kBeta + 3] * d0 * d0 + cf[kBeta + 4] * d0 * dm1 + cf[kBeta + 5] * dm1 * dm1 + eps;
-        const R b2 = cf[kBeta + 6] * dm1 * dm1 + cf[kBeta + 7] * dm1 * dm2 + cf[kBeta + 8] * dm2 * dm2 + eps;
+        R b0, b1, b2;
+        if constexpr ((kFastF64 && sizeof(R) == 8) || sizeof(R) == 4) {
+            // contracted forms: c0 a^2 + c1 a b + c2 b^2 + eps as
+            // a (c0 a + c1 b) + (c2 b b + eps), five operations where the
+            // reference's order takes seven
+            auto beta = [&](R c0, R c1, R c2, R a, R b) {
+                return __builtin_fma(a, __builtin_fma(c1, b, c0 * a), __builtin_fma(c2 * b, b, eps));
+            };
+            b0 = beta(cf[kBeta + 0], cf[kBeta + 1], cf[kBeta + 2], d1, d0);
+            b1 = beta(cf[kBeta + 3], cf[kBeta + 4], cf[kBeta + 5], d0, dm1);
+            b2 = beta(cf[kBeta + 6], cf[kBeta + 7], cf[kBeta + 8], dm1, dm2);
+        } else {
+            b0 = cf[kBeta + 0] * d1 * d1 + cf[kBeta + 1] * d1 * d0 + cf[kBeta + 2] * d0 * d0 + eps;
+            b1 = cf[kBeta + 3] * d0 * d0 + cf[kBeta + 4] * d0 * dm1 + cf[kBeta + 5] * dm1 * dm1 + eps;
+            b2 = cf[kBeta + 6] * dm1 * dm1 + cf[kBeta + 7] * dm1 * dm2 + cf[kBeta + 8] * dm2 * dm2 + eps;
+        }
         const R bb0 = b0 * b0, bb1 = b1 * b1, bb2 = b2 * b2;
         R pL[3], pR[3];
         if constexpr (WANT_L) {
@@ -343,6 +356,43 @@ __device__ __forceinline__ void hllc_face(const R *qL, const R *qR, const Eos &e
                        rho_L * (s_L - uL) - rho_R * (s_R - uR));
     const R s_M = min_ref(R(0), s_L);
     const R s_P = max_ref(R(0), s_R);
+    if constexpr (((kFastF64 && sizeof(R) == 8) || sizeof(R) == 4) && !VSRC) {
+        // The weights xi_M = 1/2 + sign(1/2, s_S) and xi_P = 1/2 - sign(1/2,
+        // s_S) below are 1 and 0 or 0 and 1 by the sign bit of s_S, so for
+        // finite states the flux is the upwind side's expression and the
+        // other side adds exact zeros.  The contracted forms evaluate the
+        // upwind side only; the contact velocity is then its m.
+        const bool lft = !signbit(s_S);
+        const R rho = lft ? rho_L : rho_R, u = lft ? uL : uR, p = lft ? pres_L : pres_R, E = lft ? E_L : E_R;
+        const R sw = lft ? s_L : s_R, sm = lft ? s_M : s_P;
+        R xi, pd; // (s - u)/(s - s_S), p/(s - u)
+        if constexpr (kFastF64 && sizeof(R) == 8) {
+            // both from one reciprocal of (s - u)(s - s_S)
+            const R du = sw - u, ds = sw - s_S;
+            const R r = frcp(du * ds);
+            xi = du * du * r;
+            pd = p * ds * r;
+        } else {
+            xi = bdiv(sw - u, sw - s_S);
+            pd = bdiv(p, sw - u);
+        }
+        const R m = u + sm * (xi - R(1));
+#pragma unroll
+        for (int i = 0; i < NF; ++i) flux[i] = (lft ? qL[i] : qR[i]) * m;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const R v = lft ? qL[Y::MOM + d] : qR[Y::MOM + d];
+            if (d == DIR)
+                flux[Y::MOM + d] = rho * (u * v + sm * (xi * s_S - v)) + p;
+            else
+                flux[Y::MOM + d] = rho * (u * v + sm * (xi * v - v));
+        }
+        flux[Y::E] = u * (E + p) + sm * (xi * (E + (s_S - u) * (rho * s_S + pd)) - E);
+#pragma unroll
+        for (int i = 0; i < NF; ++i) flux[Y::ADV + i] = (lft ? qL[Y::ADV + i] : qR[Y::ADV + i]) * m;
+        ustar = m;
+        return;
+    }
     // 1/(s_L - u_L), 1/(s_R - u_R), 1/(s_L - s_S), 1/(s_R - s_S): in the
     // contracted form from one reciprocal of their product
     R xi_L, xi_R, iwL = R(0), iwR = R(0);
