@@ -803,13 +803,16 @@ int viscous(int d, const Fields &qin, const Fields &rhs, const SweepArgs &sa) {
 // Viscous runs use the viscous kernels for each direction instead, with the
 // same finish semantics.
 int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, const Fields *qout, double dt) {
-    // The plain forms add the directions in the reference's order x, y, z
-    // (m_rhs.fpp:469-656; it fixes the rounding).  The contracted FP64 form
-    // runs them z, y, x: the sum is the same up to rounding, and the RK stage
-    // (base, rhs and stage output traffic) then rides on the x sweep, whose
-    // loads are contiguous rows, instead of on the z sweep's plane-strided
-    // tiles.
-    const bool reverse = G.prm.precision == MFC_PREC_FP64_FAST && !G.visc;
+    // The plain and learned forms add the directions in the reference's
+    // order x, y, z (m_rhs.fpp:469-656; it fixes the rounding).  The
+    // contracted FP64 form, and the FP32-bulk form in 2D, run them in the
+    // reverse order: the sum is the same up to rounding, and the RK stage's
+    // output is then written by the x sweep, whose loads are contiguous
+    // rows.  (The 3D FP32-bulk z sweep is as slow as the first sweep as it
+    // is carrying the stage: 512^3 52.6 ms per step reversed, 51.8 not.)
+    const bool reverse = (G.prm.precision == MFC_PREC_FP64_FAST ||
+                          (G.prm.precision == MFC_PREC_FP32_BULK && G.D == 2)) &&
+                         !G.visc;
     if (G.wrf) {
         // cell-centred velocity gradients of the stage input (s_get_viscous,
         // m_rhs.fpp:456-464), before the directions

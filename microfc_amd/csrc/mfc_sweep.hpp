@@ -150,17 +150,21 @@ struct FinishIn {
     double rhs[S], base[S], in[S];
 };
 
-// Which finish modes a sweep kernel can meet.  The contracted FP64 form of a
-// 2D/3D case never passes an RK stage to a sweep: it spreads the stage over
-// the sweeps (SweepArgs::fold), its x sweep, always the last, meets folds 0
-// and 3, its y/z sweeps folds 0, 1 and 2.  Every other form and the 1D
-// contracted x sweep meet rk and never fold.  Fixing the set at compile
-// time keeps the modes a kernel cannot meet out of its code (and its
-// registers).
+// Which finish modes a sweep kernel can meet.  The contracted FP64 form of
+// a 2D/3D case and the FP32-bulk form of a 2D case never pass an RK stage
+// to a sweep: they spread the stage over the sweeps (SweepArgs::fold),
+// their x sweep, always the last, meets folds 0 and 3, their y/z sweeps
+// folds 0, 1 and 2.  The other forms and cases meet rk and never fold.
+// Fixing the set at compile time keeps the modes a kernel cannot meet out
+// of its code (and its registers).
 enum FinishModes { kModesAll = 0, kModesFoldLast = 1, kModesFoldFirst = 2 };
-template <int D, bool LAST>
+template <int D, typename R, bool ML>
+constexpr bool folded_form() {
+    return (kFastF64 && sizeof(R) == 8 && D > 1) || (sizeof(R) == 4 && !ML && D == 2);
+}
+template <int D, bool LAST, typename R, bool ML>
 constexpr int finish_modes() {
-    return !kFastF64 || D == 1 ? kModesAll : LAST ? kModesFoldLast : kModesFoldFirst;
+    return !folded_form<D, R, ML>() ? kModesAll : LAST ? kModesFoldLast : kModesFoldFirst;
 }
 
 template <int NF, int D, int M>
@@ -300,15 +304,21 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, Off go, Off ro, 
             if constexpr (M == kModesFoldFirst) {
 #pragma unroll
                 for (int v = 0; v < S; ++v) el(a.rhs.v[v], ro) = r[v];
-            } else if (a.ci != 0.0) {
-#pragma unroll
-                for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = r[v] + a.ci * f.in[v];
-            } else {
-                // Euler: the stage input is not part of the output (and
-                // finish_load leaves its non-advected entries unset)
-#pragma unroll
-                for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = r[v];
+                return;
             }
+            if (a.ci != 0.0) {
+#pragma unroll
+                for (int v = 0; v < S; ++v) r[v] = r[v] + a.ci * f.in[v];
+            }
+            // (Euler: the stage input is not part of the output, and
+            // finish_load leaves its non-advected entries unset)
+            if constexpr (!std::is_same<R, double>::value) {
+                // FP32 bulk: volume fractions clipped to [0, 1] as below
+#pragma unroll
+                for (int i = 0; i < NF; ++i) r[Y::ADV + i] = min(max(r[Y::ADV + i], 0.0), 1.0);
+            }
+#pragma unroll
+            for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = r[v];
             return;
         }
     }
@@ -522,7 +532,7 @@ template <int NF, int D, int WO, typename R, bool ML, bool UNI>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 8))) void k_sweep_x(const SweepArgs a) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
-    constexpr int FM = finish_modes<D, true>();
+    constexpr int FM = finish_modes<D, true, R, ML>();
     constexpr int PN = (WO - 1) / 2;
     constexpr int NC = 64 + 2 * PN + 1;
     constexpr int TW = x_tile_cells<ML, R, UNI>();
@@ -772,7 +782,7 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
     static_assert(!ML, "the learned form runs the cell-parallel sweep");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
-    constexpr int FM = finish_modes<D, false>();
+    constexpr int FM = finish_modes<D, false, R, ML>();
     constexpr int kTS = t_faces<ML>();
     constexpr int NT = t_threads<ML>();
     constexpr int PN = (WO - 1) / 2;
@@ -949,7 +959,7 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
     static_assert(t_cell_parallel<ML, R, UNI>(), "cell-parallel forms only");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
-    constexpr int FM = finish_modes<D, false>();
+    constexpr int FM = finish_modes<D, false, R, ML>();
     constexpr int XC = tc_cols<R>(DIR), kTS = tc_rows<R>(DIR), NT = XC * kTS;
     constexpr int TC = t_tile_cells<ML, R, UNI>(DIR);
     constexpr int PN = (WO - 1) / 2;
