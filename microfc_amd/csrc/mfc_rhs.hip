@@ -853,6 +853,12 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
     } else if (rk == kRk3Stage3) {
         cb = 1.0 / 3.0, ci = 2.0 / 3.0, cdt = 2.0 * dt / 3.0;
     }
+    if (fold && qbase->v[0] == qin.v[0]) {
+        // base = in (first stages): (cb + ci)*in, added by the last sweep
+        // from its staged cells, and no sweep reads the base
+        ci += cb;
+        cb = 0.0;
+    }
     for (int k = 0; k < G.D; ++k) {
         const int d = reverse ? G.D - 1 - k : k;
         SweepArgs a = sweep_args(d);
@@ -869,7 +875,7 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         }
         if (fold) {
             a.fold = k == 0 ? 1 : last ? 3 : 2;
-            a.fold_base = k == (G.D == 3 ? kFoldBaseSweep3D : 0);
+            a.fold_base = cb != 0.0 && k == (G.D == 3 ? kFoldBaseSweep3D : 0);
             a.cb = cb;
             a.ci = ci;
             a.cdt = cdt;
