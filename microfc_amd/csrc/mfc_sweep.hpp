@@ -586,9 +586,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 
         for (int it = 0; it < NIT; ++it) {
             const int c = lane + 64 * it, x = x0 - 1 - PN + c;
             const bool ok = c < NC && x <= a.m + a.b;
+            // entries past the ghost layers take cell x0's values (their
+            // faces are never finished; a select per value would cost VALU)
             const Off go{sb, (unsigned)(ok ? c : 1 + PN)};
 #pragma unroll
-            for (int v = 0; v < S; ++v) cv[it][v] = ok ? el(a.qin.v[v], go) : 0.0;
+            for (int v = 0; v < S; ++v) cv[it][v] = el(a.qin.v[v], go);
         }
         // cell x0+lane is staging entry lane+1+PN: loaded by lane
         // (lane+1+PN) mod 64 in pass 0, or in pass 1 for the last 1+PN lanes
@@ -854,9 +856,11 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
             const int ex = e % kTX, er = e / kTX;
             const int c = y0 - 1 - PN + er;
             const bool ok = e < NR * kTX && c <= s + a.b;
+            // rows past the ghost layers take row y0's values (their faces
+            // are never finished)
             const long go = gcell(min(x0 + ex, a.m), ok ? c : y0);
 #pragma unroll
-            for (int v = 0; v < S; ++v) cv[it][v] = ok ? a.qin.v[v][go] : 0.0;
+            for (int v = 0; v < S; ++v) cv[it][v] = a.qin.v[v][go];
         }
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
@@ -1018,9 +1022,11 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
             const int ex = e % XC, er = e / XC;
             const int c = y0 - 1 - PN + er;
             const bool ok = e < NR * XC && c <= s + a.b;
+            // rows past the ghost layers take row y0's values (their faces
+            // are never finished; a select per value would cost VALU)
             const Off go{sb, (unsigned)(min(x0 + ex, a.m) - x0) + (unsigned)(ok ? er : 1 + PN) * gp};
 #pragma unroll
-            for (int v = 0; v < S; ++v) cv[it][v] = ok ? el(a.qin.v[v], go) : 0.0;
+            for (int v = 0; v < S; ++v) cv[it][v] = el(a.qin.v[v], go);
         }
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
