@@ -977,8 +977,10 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
     // while four 256-thread blocks (or the same threads in fewer blocks)
     // still fit a CU (not in the learned form, whose MLP staging takes
     // that LDS)
-    // (the contracted 2D/3D sweeps keep only the volume fractions)
-    constexpr bool ADVK = FM == kModesFoldFirst;
+    // (the contracted 2D/3D sweeps, and the 3D y sweep of the forms that
+    // sweep x, y, z, which is never the last, keep only the volume
+    // fractions)
+    constexpr bool ADVK = !ML && (FM == kModesFoldFirst || (FM == kModesAll && D == 3 && DIR == 1));
     constexpr int KS = ADVK ? NF : S;
     constexpr bool KEEP = !ML && sizeof(R) * WORDS + 8 * KS * XC * KP <= 40 * 1024 * (NT / 256);
     __shared__ R lds[WORDS];
