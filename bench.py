@@ -39,17 +39,24 @@ FP64_PEAK_TFLOPS = 78.6
 # GRBM_GUI_ACTIVE / 8)), with the profile summary they come from.  Other
 # workloads report null.
 _P512 = "profiles/sweep_kernels_shock_droplet_3d_512_fp64_fast.md"
+_P2048 = "profiles/sweep_kernels_shock_bubble_2d_2048_fp64_fast.md"
 PROFILED = {
-    ("shock_droplet_3d", 512, "fp64_fast", "sweep_x"): {
-        "traffic": 23.13e9 + 8.60e9, "valu_busy": 0.69,
-        "limiter": "VALU issue (FP64) and the RK stage's loads, not HBM bandwidth", "profile": _P512},
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_y"): {
-        "traffic": 17.44e9 + 8.59e9, "valu_busy": 0.80, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
+        "traffic": 24.05e9 + 8.59e9, "valu_busy": 0.60,
+        "limiter": "HBM bandwidth in RK stages 2 and 3 (35.9 GB at 5.2 TB/s, VALU 53 % busy); VALU issue in "
+                   "stage 1 (75 % busy)", "profile": _P512},
+    ("shock_droplet_3d", 512, "fp64_fast", "sweep_x"): {
+        "traffic": 17.31e9 + 8.64e9, "valu_busy": 0.69,
+        "limiter": "HBM bandwidth (25.9 GB at 5.0 TB/s) with VALU 69 % busy", "profile": _P512},
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_z"): {
-        "traffic": 8.98e9 + 8.59e9, "valu_busy": 0.78, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
-    ("shock_bubble_2d", 2048, "fp64", "sweep_y"): {
-        "traffic": 678.5e6 + 236.6e6, "valu_busy": 0.79, "limiter": "VALU issue (FP64), not HBM",
-        "profile": "profiles/sweep_kernels_shock_bubble_2d_2048_fp64.md"},
+        "traffic": 9.09e9 + 8.59e9, "valu_busy": 0.80, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
+    ("shock_bubble_2d", 2048, "fp64_fast", "sweep_y"): {
+        "traffic": 471.8e6 + 234.9e6, "valu_busy": 0.70,
+        "limiter": "VALU issue in RK stage 1 (78 % busy); HBM in stages 2 and 3 (0.80 GB at 4.8 TB/s)",
+        "profile": _P2048},
+    ("shock_bubble_2d", 2048, "fp64_fast", "sweep_x"): {
+        "traffic": 471.1e6 + 238.9e6, "valu_busy": 0.62,
+        "limiter": "HBM bandwidth (0.71 GB at 4.4 TB/s) with VALU 62 % busy", "profile": _P2048},
 }
 
 DEFAULT_PRECISION = "fp64_fast"

@@ -127,12 +127,12 @@ __device__ __forceinline__ double to_d(R x) { return (double)x; }
 // array's address in scalar registers and the lane part as a 32-bit byte
 // offset, so the load or store takes the scalar-base global address form
 // with no 64-bit address arithmetic per lane.  The lane part must stay
-// below 2^29 elements (the host checks the largest block span).
+// below 2^29 elements (the host checks the largest block span).  Kernels
+// whose offsets have no uniform part pass a plain element offset.
 struct Off {
     long base;    // elements, uniform
     unsigned rel; // elements, per lane
 };
-__device__ __forceinline__ Off lane_off(long o) { return Off{o, 0u}; }
 template <typename T>
 __device__ __forceinline__ T &el(T *p, Off o) {
     unsigned b = o.rel * (unsigned)sizeof(T);
@@ -140,6 +140,10 @@ __device__ __forceinline__ T &el(T *p, Off o) {
     // the address is formed (a widened one takes a 64-bit VALU add)
     asm("" : "+v"(b));
     return *reinterpret_cast<T *>(reinterpret_cast<char *>(p + o.base) + b);
+}
+template <typename T>
+__device__ __forceinline__ T &el(T *p, long o) {
+    return p[o];
 }
 
 // Operands of finish_cell that come from HBM: the rhs accumulated by the
@@ -167,8 +171,8 @@ constexpr int finish_modes() {
     return !folded_form<D, R, ML>() ? kModesAll : LAST ? kModesFoldLast : kModesFoldFirst;
 }
 
-template <int NF, int D, int M>
-__device__ __forceinline__ void finish_load(const SweepArgs &a, Off go, Off ro, FinishIn<Sys<NF, D>::S> &f) {
+template <int NF, int D, int M, typename O>
+__device__ __forceinline__ void finish_load(const SweepArgs &a, O go, O ro, FinishIn<Sys<NF, D>::S> &f) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     if (a.accumulate) {
@@ -192,8 +196,8 @@ __device__ __forceinline__ void finish_load(const SweepArgs &a, Off go, Off ro, 
 // The same with the stage input of the cell taken from the copy a sweep
 // staged in LDS (element v at cin[v * stride]); the RK base too when it is
 // the stage input.
-template <int NF, int D, int M>
-__device__ __forceinline__ void finish_load_staged(const SweepArgs &a, Off go, Off ro, const double *cin, int stride,
+template <int NF, int D, int M, typename O>
+__device__ __forceinline__ void finish_load_staged(const SweepArgs &a, O go, O ro, const double *cin, int stride,
                                                    FinishIn<Sys<NF, D>::S> &f) {
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
@@ -273,8 +277,8 @@ __device__ __forceinline__ void rk_combine(const SweepArgs &a, const FinishIn<S>
 
 // Finishes one cell: flux divergence, alpha*div(u) source, then either the
 // rhs store or the fused RK stage (m_rhs.fpp:565-652, m_time_steppers.fpp).
-template <int NF, int D, typename R, int M>
-__device__ __forceinline__ void finish_cell(const SweepArgs &a, Off go, Off ro, double inv,
+template <int NF, int D, typename R, int M, typename O>
+__device__ __forceinline__ void finish_cell(const SweepArgs &a, O go, O ro, double inv,
                                             const R *Fm, const R *Fp, R usm, R usp,
                                             const FinishIn<Sys<NF, D>::S> &f) {
     using Y = Sys<NF, D>;
@@ -931,12 +935,11 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
         }
         FinishIn<S> fin;
         if constexpr (KEEP)
-            finish_load_staged<NF, D, FM>(a, lane_off(gcell(i, cc)), lane_off(rcell(i, cc)), &lcons[0][ts][tx],
-                                          (kTS - 1) * kTX, fin);
+            finish_load_staged<NF, D, FM>(a, gcell(i, cc), rcell(i, cc), &lcons[0][ts][tx], (kTS - 1) * kTX, fin);
         else
-            finish_load<NF, D, FM>(a, lane_off(gcell(i, cc)), lane_off(rcell(i, cc)), fin);
-        finish_cell<NF, D, R, FM>(a, lane_off(gcell(i, cc)), lane_off(rcell(i, cc)), a.inv_ds[cc], Fm, Fp,
-                                  sf[S][ts][tx], sf[S][ts + 1][tx], fin);
+            finish_load<NF, D, FM>(a, gcell(i, cc), rcell(i, cc), fin);
+        finish_cell<NF, D, R, FM>(a, gcell(i, cc), rcell(i, cc), a.inv_ds[cc], Fm, Fp, sf[S][ts][tx],
+                                  sf[S][ts + 1][tx], fin);
     }
 }
 
