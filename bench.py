@@ -60,6 +60,9 @@ PROFILED = {
 }
 
 DEFAULT_PRECISION = "fp64_fast"
+# RK-base variables the contracted form's 3D z sweep adds (the y sweep adds
+# the rest; the library's split)
+BASE_ON_Z = 4
 FORMS = {"fp64": "plain FP64: the reference's evaluation order, no FMA contraction, IEEE division (bit-exact "
                  "against the FP64 oracle)",
          "fp64_fast": "contracted FP64: FMA contraction, reciprocal-multiply division, WENO weights normalised "
@@ -84,7 +87,8 @@ def sweep_bytes_per_step(prm):
     sweep, read and written by middle ones, read by the last, which writes
     the stage output; the RK base (stages 2 and 3) is read by the last sweep,
     or, in the contracted FP64 form of a 2D/3D case and the FP32-bulk form
-    of a 2D one, which spread the stage over the sweeps, by the y sweep.  Viscous runs use the
+    of a 2D one, which spread the stage over the sweeps, by the y sweep (in
+    3D its first BASE_ON_Z variables by the z sweep).  Viscous runs use the
     viscous kernels in place of the sweeps: per direction the face kernel
     reads the stencil input and writes a face record (HLLC flux, contact
     velocity, viscous flux: S+2+D doubles), the finish reads it and the
@@ -118,7 +122,11 @@ def sweep_bytes_per_step(prm):
             w = 0
             if not first:
                 r += S  # rhs partial
-            if stage > 0 and (nm == "sweep_y" if fold else last):
+            if stage > 0 and fold:
+                # RK base q1, split between z and y in the contracted 3D form
+                r += {"sweep_z": BASE_ON_Z, "sweep_y": S - BASE_ON_Z}.get(nm, 0) if D == 3 else (
+                    S if nm == "sweep_y" else 0)
+            elif stage > 0 and last:
                 r += S  # RK base q1
             if last:
                 w += S  # stage output

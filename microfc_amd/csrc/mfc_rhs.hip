@@ -51,10 +51,12 @@ double *g_scratch_dev[kMaxDevices] = {};
 
 constexpr int kNumFluidsBuilt = 10;   // the FP64 forms (every num_fluids the reference allows)
 constexpr int kNumFluidsBuiltF32 = 4; // the FP32-bulk and learned forms
-// The sweep of a contracted 3D RK stage that adds cb*base (0: z, 1: y).
-// Measured at 512^3: y 53.0 ms per step, z 53.5 (z's finish cells lie in
-// 14 planes per block, y's in one).
-constexpr int kFoldBaseSweep3D = 1;
+// Variables of the RK base that the z sweep of a contracted 3D RK stage
+// adds (the y sweep adds the rest).  The step moves the same bytes for any
+// split (512^3: 50.6-50.7 ms per step with 0, 2, 3 or 4); half on each
+// balances the two sweeps (launches of 5.5 and 5.8 ms instead of 5.0 and
+// 6.3 with the whole base on y).
+constexpr int kBaseOnZ = 4;
 
 thread_local std::string g_err;
 
@@ -841,9 +843,9 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         }
     }
     // The contracted form spreads an RK stage over its sweeps (out = cb*base
-    // + ci*in + cdt*L; the y sweep reads the base, the last the stage
-    // input's own cells, which it stages anyway), so the x sweep does not
-    // carry a fourth array.
+    // + ci*in + cdt*L; the sweeps before the last read the base, the last
+    // the stage input's own cells, which it stages anyway), so the x sweep
+    // does not carry a fourth array.
     const bool fold = reverse && rk != kRkNone && G.D > 1;
     double cb = 1.0, ci = 0.0, cdt = dt;
     if (rk == kRk2Final) {
@@ -875,7 +877,11 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         }
         if (fold) {
             a.fold = k == 0 ? 1 : last ? 3 : 2;
-            a.fold_base = cb != 0.0 && k == (G.D == 3 ? kFoldBaseSweep3D : 0);
+            // cb*base: in 3D variables [0, kBaseOnZ) on the z sweep, the rest
+            // on y; in 2D all on y (the first sweep)
+            const int split = G.D == 3 ? kBaseOnZ : 0;
+            if (cb != 0.0 && k == 0 && G.D == 3) a.base_lo = 0, a.base_hi = split;
+            if (cb != 0.0 && k == G.D - 2) a.base_lo = split, a.base_hi = G.S;
             a.cb = cb;
             a.ci = ci;
             a.cdt = cdt;

@@ -64,10 +64,11 @@ struct SweepArgs {
     // Contracted FP64 form, 2D/3D: an RK stage out = cb*base + ci*in +
     // cdt*L spread over the sweeps (rk = kRkNone in each).  The first sweep
     // stores cdt*L_d into rhs, the middle one adds cdt*L_d, the last writes
-    // rhs + cdt*L_d + ci*in to qout, and one sweep before the last also adds
-    // cb*base, so the last sweep does not read the base.
+    // rhs + cdt*L_d + ci*in to qout, and the sweeps before the last add
+    // cb*base, each for a range of the variables, so the last sweep does
+    // not read the base.
     int fold;             // 0: off, 1: first sweep, 2: middle, 3: last
-    int fold_base;        // this sweep adds cb*base
+    int base_lo, base_hi; // this sweep adds cb*base of variables [base_lo, base_hi)
     double cb, ci, cdt;
     int chunk, nchunk;    // cells per tile along the sweep and tile count (k_sweep_t)
     int ntile;            // x tiles (k_sweep_x) or x column groups (k_sweep_t)
@@ -187,9 +188,13 @@ __device__ __forceinline__ void finish_load(const SweepArgs &a, O go, O ro, Fini
 #pragma unroll
         for (int v = 0; v < S; ++v) f.in[v] = el(a.qin.v[v], go);
     }
-    if ((M == kModesAll && a.rk != kRkNone) || (M == kModesFoldFirst && a.fold_base)) {
+    if (M == kModesAll && a.rk != kRkNone) {
 #pragma unroll
         for (int v = 0; v < S; ++v) f.base[v] = el(a.qbase.v[v], go);
+    } else if (M == kModesFoldFirst) {
+#pragma unroll
+        for (int v = 0; v < S; ++v)
+            if (v >= a.base_lo && v < a.base_hi) f.base[v] = el(a.qbase.v[v], go);
     }
 }
 
@@ -207,7 +212,7 @@ __device__ __forceinline__ void finish_load_staged(const SweepArgs &a, O go, O r
     }
 #pragma unroll
     for (int v = 0; v < S; ++v) f.in[v] = cin[v * stride];
-    if ((M == kModesAll && a.rk != kRkNone) || (M == kModesFoldFirst && a.fold_base)) {
+    if (M == kModesAll && a.rk != kRkNone) {
         if (a.base_is_in) {
 #pragma unroll
             for (int v = 0; v < S; ++v) f.base[v] = f.in[v];
@@ -215,14 +220,19 @@ __device__ __forceinline__ void finish_load_staged(const SweepArgs &a, O go, O r
 #pragma unroll
             for (int v = 0; v < S; ++v) f.base[v] = el(a.qbase.v[v], go);
         }
+    } else if (M == kModesFoldFirst) {
+        // (a spread stage never adds a base that is its input: the last
+        // sweep adds (cb + ci)*in instead)
+#pragma unroll
+        for (int v = 0; v < S; ++v)
+            if (v >= a.base_lo && v < a.base_hi) f.base[v] = el(a.qbase.v[v], go);
     }
 }
 
 // The same for a contracted 2D/3D y/z sweep (folds 0, 1, 2), which keeps
 // only the stage input's volume fractions in LDS (element i at
 // cadv[i * stride]): the finish needs no other stage-input value, and the
-// sweep that adds the RK base reads it from HBM (from L2 in the first
-// stage, whose base is the input this block staged).  Keeping all S
+// RK base it adds (stages 2 and 3) comes from HBM.  Keeping all S
 // variables took 36.9 KB of LDS per 256-thread block in 3D, four blocks
 // per CU; the volume fractions alone leave 25.3 KB, five.
 template <int NF, int D>
@@ -236,10 +246,9 @@ __device__ __forceinline__ void finish_load_adv(const SweepArgs &a, Off go, Off 
     }
 #pragma unroll
     for (int i = 0; i < NF; ++i) f.in[Y::ADV + i] = cadv[i * stride];
-    if (a.fold_base) {
 #pragma unroll
-        for (int v = 0; v < S; ++v) f.base[v] = el(a.qbase.v[v], go);
-    }
+    for (int v = 0; v < S; ++v)
+        if (v >= a.base_lo && v < a.base_hi) f.base[v] = el(a.qbase.v[v], go);
 }
 
 // x/3 of the third RK3 stage: the IEEE quotient in the plain form, a
@@ -298,7 +307,7 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, O go, O ro, doub
                     // pressure, dF is a cancellation whose separate
                     // rounding doubled the error against the oracle
                     double acc = a.fold == 1 ? 0.0 : f.rhs[v];
-                    if (a.fold_base) acc = fma(a.cb, f.base[v], acc);
+                    if (v >= a.base_lo && v < a.base_hi) acc = fma(a.cb, f.base[v], acc);
                     r[v] = fma(sc, df, acc);
                 } else
                     r[v] = f.rhs[v] + sc * df;
