@@ -60,9 +60,10 @@ PROFILED = {
 }
 
 DEFAULT_PRECISION = "fp64_fast"
-# RK-base variables the contracted form's 3D z sweep adds (the y sweep adds
-# the rest; the library's split)
+# RK-base variables the contracted form's 3D z sweep adds (the y sweep, or
+# the fused x and y sweep, adds the rest; the library's split)
 BASE_ON_Z = 4
+BASE_ON_Z_FUSED = 4
 FORMS = {"fp64": "plain FP64: the reference's evaluation order, no FMA contraction, IEEE division (bit-exact "
                  "against the FP64 oracle)",
          "fp64_fast": "contracted FP64: FMA contraction, reciprocal-multiply division, WENO weights normalised "
@@ -81,7 +82,7 @@ def sweep_order(prm):
     return names[::-1] if rev else names
 
 
-def sweep_bytes_per_step(prm):
+def sweep_bytes_per_step(prm, fused=False):
     """Algorithmic HBM bytes per RK3 step of each kernel, per cell: the
     stencil input is read once (S doubles), rhs is written by the first
     sweep, read and written by middle ones, read by the last, which writes
@@ -95,7 +96,10 @@ def sweep_bytes_per_step(prm):
     volume fractions, and reads/writes rhs like a sweep (or, in the last
     direction, reads the RK base and stage input and writes the output).
     With weno_Re_flux the velocity-gradient planes and their face averages
-    add their passes."""
+    add their passes.  fused: the contracted form's fused x and y sweep
+    (sweep_xy, after the z sweep in 3D) reads the stencil input once, the
+    z sweep's rhs in 3D and its part of the RK base, and writes the stage
+    output."""
     S, D, NF = prm.sys_size, prm.num_dims, prm.num_fluids
     out = {}
     names = sweep_order(prm)
@@ -113,6 +117,12 @@ def sweep_bytes_per_step(prm):
                 per += (NF + 2 * D) + 3 * D * D
             total += 3 * 8 * per
         out["viscous"] = total
+        return out
+    if fused:
+        zb = BASE_ON_Z_FUSED if D == 3 else 0
+        if D == 3:
+            out["sweep_z"] = 8 * (3 * 2 * S + 2 * zb)
+        out["sweep_xy"] = 8 * (3 * (S + (S if D == 3 else 0) + S) + 2 * (S - zb))
         return out
     for d, nm in enumerate(names):
         first, last = d == 0, d == D - 1
@@ -269,7 +279,7 @@ def main():
     # so a launch is charged one stage's bytes of all cells.
     step_bytes = 64 * S * ncell
     stage_bytes = step_bytes / 3
-    per_step = sweep_bytes_per_step(prm)
+    per_step = sweep_bytes_per_step(prm, fused=bool(timing.get("sweep_xy", (0, 0))[1]))
     launched = {k: v for k, v in timing.items() if v[1] and k in per_step}
     dom = max(launched, key=lambda k: launched[k][0])
     dom_ms, dom_n = launched[dom]

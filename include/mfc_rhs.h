@@ -201,7 +201,8 @@ const char *mfc_mlp_last_error(void);
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * every launch.  Kernel ids: 0 bc_fill, 1 cons_to_prim, 2 sweep_x, 3 sweep_y,
  * 4 sweep_z, 5 icfl, 6 rk1_update, 7 viscous (one direction of a viscous
- * run: face fluxes and finish).  The reference's only timing is the mean
+ * run: face fluxes and finish), 8 sweep_xy (the fused x and y sweep of the
+ * contracted FP64 form on uniform grids).  The reference's only timing is the mean
  * CPU time per step (src/simulation/m_time_steppers.fpp:281,352-358). */
 int mfc_timing_enable(int on);
 int mfc_timing_reset(void);

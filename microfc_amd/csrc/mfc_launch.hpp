@@ -8,7 +8,7 @@
 namespace mfc {
 
 struct SweepLaunch {
-    int dir;       // 0, 1, 2
+    int dir;       // 0, 1, 2; 3: the fused x and y sweep (k_sweep_xy)
     int num_dims;  // 1..3
     int weno_order;
     int precision; // MFC_PREC_*
@@ -25,7 +25,10 @@ int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
     template <int D, int WO, typename R, bool ML, bool UNI>                                             \
     static int launch_nf##NF##_dwrm(int dir, const SweepArgs &a, hipStream_t st) {                      \
         const dim3 grid((unsigned)a.nblocks);                                                           \
-        if (dir == 0) {                                                                                 \
+        if (dir == 3) {                                                                                 \
+            if constexpr (kFastF64 && D >= 2 && WO == 5 && sizeof(R) == 8 && !ML && UNI) hipLaunchKernelGGL((k_sweep_xy<NF, D>), grid, dim3(512), 0, st, a); \
+            else return -1;                                                                             \
+        } else if (dir == 0) {                                                                          \
             hipLaunchKernelGGL((k_sweep_x<NF, D, WO, R, ML, UNI>), grid, dim3(64), 0, st, a);                \
         } else if (dir == 1) {                                                                          \
             if constexpr (D >= 2 && t_cell_parallel<ML, R, UNI>()) hipLaunchKernelGGL((k_sweep_tc<NF, D, 1, WO, R, ML, UNI>), grid, dim3(tc_cols<R>(1) * tc_rows<R>(1)), 0, st, a); \

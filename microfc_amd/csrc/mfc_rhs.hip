@@ -57,6 +57,9 @@ constexpr int kNumFluidsBuiltF32 = 4; // the FP32-bulk and learned forms
 // balances the two sweeps (launches of 5.5 and 5.8 ms instead of 5.0 and
 // 6.3 with the whole base on y).
 constexpr int kBaseOnZ = 4;
+// The fused x and y sweep, and the RK-base variables the z sweep adds before it.
+constexpr bool kFuseXY = true;
+constexpr int kBaseOnZFused = 4;
 
 thread_local std::string g_err;
 
@@ -111,9 +114,9 @@ struct State {
 
 // ---------------------------------------------------------------------------
 // Optional per-kernel timing with HIP events on the library's stream.
-enum KernelId { kKBcFill = 0, kKConsToPrim, kKSweepX, kKSweepY, kKSweepZ, kKIcfl, kKRk1, kKViscous, kKCount };
+enum KernelId { kKBcFill = 0, kKConsToPrim, kKSweepX, kKSweepY, kKSweepZ, kKIcfl, kKRk1, kKViscous, kKSweepXY, kKCount };
 const char *const kKernelNames[kKCount] = {"bc_fill", "cons_to_prim", "sweep_x", "sweep_y", "sweep_z", "icfl",
-                                           "rk1_update", "viscous"};
+                                           "rk1_update", "viscous", "sweep_xy"};
 
 struct Timing {
     bool on = false;
@@ -727,13 +730,28 @@ SweepArgs sweep_args(int d) {
     return a;
 }
 
+// Arguments of the fused x and y sweep: the x direction's, the y
+// direction's cell widths, boundary codes and coefficients, and its tiles.
+SweepArgs xy_args() {
+    SweepArgs a = sweep_args(0);
+    a.inv_ds2 = G.dir[1].inv_ds;
+    a.bc2_beg = G.prm.bc_y_beg;
+    a.bc2_end = G.prm.bc_y_end;
+    for (int q = 0; q < kNCoef; ++q) a.ucf2[q] = G.dir[1].ucf[q];
+    a.ntile = (G.prm.m + 1 + kXyCellsX - 1) / kXyCellsX;
+    a.nchunk = (G.prm.n + 1 + kXyCellsY - 1) / kXyCellsY;
+    a.nblocks = (long)a.ntile * a.nchunk * (G.prm.p + 1);
+    return a;
+}
+
+// d = 0, 1, 2: one direction; d = 3: the fused x and y sweep.
 int sweep(int d, SweepArgs &a) {
     SweepLaunch L{d, G.D, G.prm.weno_order, G.prm.precision};
     if (G.prm.precision == MFC_PREC_FP32_LEARNED) {
         a.mlp = mlp_frags();
         if (!a.mlp) return fail(-4, "learned weights requested but mfc_mlp_init has not been called");
     }
-    TimedScope ts(kKSweepX + d);
+    TimedScope ts(d == 3 ? kKSweepXY : kKSweepX + d);
     int rc;
     const int form = G.prm.precision == MFC_PREC_FP64        ? kFormF64
                      : G.prm.precision == MFC_PREC_FP64_FAST ? kFormF64Fast
@@ -860,6 +878,39 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         // from its staged cells, and no sweep reads the base
         ci += cb;
         cb = 0.0;
+    }
+    // The contracted form on grids uniform in x and y, WENO5, fuses the y and
+    // x sweeps (k_sweep_xy): in 3D after the z sweep, in 2D alone.
+    const bool fuse = reverse && kFuseXY && G.D >= 2 && G.prm.precision == MFC_PREC_FP64_FAST &&
+                      G.prm.weno_order == 5 && G.dir[0].uniform && G.dir[1].uniform;
+    if (fuse) {
+        const int split = G.D == 3 ? kBaseOnZFused : 0;
+        auto stage = [&](SweepArgs &a, int fold_kind, int lo, int hi) {
+            a.qin = qin;
+            a.rhs = rhs;
+            a.dt = dt;
+            a.rk = kRkNone;
+            if (fold) {
+                a.qbase = *qbase;
+                a.qout = *qout;
+                a.base_is_in = qbase->v[0] == qin.v[0];
+                a.fold = fold_kind;
+                if (cb != 0.0) a.base_lo = lo, a.base_hi = hi;
+                a.cb = cb;
+                a.ci = ci;
+                a.cdt = cdt;
+            }
+        };
+        if (G.D == 3) {
+            SweepArgs a = sweep_args(2);
+            stage(a, 1, 0, split);
+            a.accumulate = 0;
+            if (int rc = sweep(2, a)) return rc;
+        }
+        SweepArgs a = xy_args();
+        stage(a, 3, split, G.S);
+        a.accumulate = G.D == 3;
+        return sweep(3, a);
     }
     for (int k = 0; k < G.D; ++k) {
         const int d = reverse ? G.D - 1 - k : k;

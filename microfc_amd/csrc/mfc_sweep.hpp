@@ -80,6 +80,11 @@ struct SweepArgs {
     int uniform;
     double ucf[kNCoef];
     float ucf32[kNCoef];
+    // k_sweep_xy: the y direction's cell widths, boundary codes and uniform
+    // coefficients (the fields above are x's)
+    const double *inv_ds2;
+    int bc2_beg, bc2_end;
+    double ucf2[kNCoef];
 };
 
 // Threads per column of the cell-parallel y/z tile by form.  Measured on
@@ -97,6 +102,10 @@ constexpr int tc_rows_of(bool f64, bool fast_f64, int dir) { return f64 && !fast
 // 7.5 % faster at 512^3, 5 % at 256^3; 64 columns and FP64 gain nothing),
 // else 16.
 constexpr int tc_cols_of(bool f64, bool fast_f64, int dir) { return !f64 && dir == 2 ? 32 : 16; }
+
+// Cells per tile of the fused x and y sweep (k_sweep_xy): 30 along x (32
+// x-phase lanes per row), 14 along y (16 y-phase lanes per column).
+constexpr int kXyCellsX = 30, kXyCellsY = 14;
 
 // Kernels and their helpers belong to the unit's arithmetic form (see
 // mfc_device.hpp); the argument types above are shared by every form.
@@ -1146,6 +1155,236 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
         else
             finish_load<NF, D, FM>(a, go, ro, fin);
         finish_cell<NF, D, R, FM>(a, go, ro, a.inv_ds[cc], Fm, Fp, fl[S][fs][fx], fl[S][fs + 1][fx], fin);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused x and y sweep of the contracted FP64 form (WENO5 on grids uniform in
+// x and y): a 512-thread block finishes a tile of XT = 30 x YT = 14 cells of
+// one z plane, so neither the rhs between the two directions nor a second
+// pass over the stencil input goes through HBM.
+//  1. The 36 x 20 cells x0-3..x0+32, y0-3..y0+16 are staged (cons->prim) in
+//     LDS as columns of odd pitch; the tile's central 32 x 14 cells first,
+//     one per thread of threads 0..447, which keep their conservative
+//     values for the finish, then the halo ring.
+//  2. y faces: 32 columns of 16 lanes (cell y0-1+lane of column x0+col,
+//     both faces; the right state of face y+1/2 from the next lane), HLLC
+//     of face y0-1+lane+1/2 by every lane.
+//  3. x faces: 14 rows of 32 lanes (cell x0-1+lane of row y0+row, both
+//     faces, the right state from the next lane), HLLC of face
+//     x0-1+lane+1/2; the lower flux of a cell from the previous lane.
+//  4. The y fluxes go to LDS over the dead staged cells and
+//  5. every x-phase lane with a tile cell finishes it: both directions'
+//     flux differences and alpha*div(u) sources, then the rhs store or the
+//     spread RK stage (rhs from the z sweep in 3D, the rest of the base,
+//     ci*in from the kept conservative values).
+template <int NF, int D>
+__device__ __forceinline__ void xy_halo_cell(int h, int &cx, int &ry) {
+    // the 272 halo cells of the 36 x 20 staging: rows 0-2 and 17-19 whole,
+    // then columns 0, 1, 34, 35 of rows 3-16
+    if (h < 6 * 36) {
+        const int band = h / 36;
+        cx = h % 36;
+        ry = band < 3 ? band : band + 14;
+    } else {
+        const int hh = h - 6 * 36, q = hh % 4;
+        ry = 3 + hh / 4;
+        cx = q < 2 ? q : q + 32;
+    }
+}
+
+template <int NF, int D>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_xy(const SweepArgs a) {
+    static_assert(kFastF64, "the contracted FP64 form only");
+    using Y = Sys<NF, D>;
+    constexpr int S = Y::S;
+    constexpr int PN = 2;
+    constexpr int XT = kXyCellsX, XL = XT + 2; // finished cells per row, x-phase lanes per row
+    constexpr int YT = kXyCellsY, YK = YT + 2; // finished rows, y-phase lanes per column
+    constexpr int YC = 32;              // y-phase columns (XT of them finished)
+    constexpr int NCX = XL + 2 * PN;    // staged columns x0-1-PN .. x0+XL-2+PN
+    constexpr int NRY = YK + 2 * PN;    // staged rows y0-1-PN .. y0+YK-2+PN
+    constexpr int P = NRY | 1;          // odd column pitch
+    constexpr int NT = 512;
+    constexpr int NCEN = XL * YT;       // central cells, the x phase's own
+    constexpr int NHALO = NCX * NRY - NCEN;
+    static_assert(XL == 32 && YK == 16 && NCEN <= NT && NHALO <= 2 * NT - NCEN && YC * YK == NT, "tile shape");
+    // y-flux rows padded to YC + 1: the 16 lanes of a column write one
+    // column of the rows (with YC, all to one bank)
+    constexpr int FP = YC + 1;
+    constexpr int SP_WORDS = S * NCX * P, FY_WORDS = (S + 1) * (YT + 1) * FP;
+    static_assert(FY_WORDS <= SP_WORDS, "y fluxes fit over the staged cells");
+    __shared__ double lds[SP_WORDS];
+    double(*sp)[NCX][P] = reinterpret_cast<double(*)[NCX][P]>(lds);
+    double(*fyl)[YT + 1][FP] = reinterpret_cast<double(*)[YT + 1][FP]>(lds);
+
+    const long bid = xcd_logical_block(blockIdx.x, a.nblocks);
+    const int xt = (int)(bid % a.ntile);
+    const long rest = bid / a.ntile;
+    const int yt = (int)(rest % a.nchunk);
+    const int k = (int)(rest / a.nchunk);
+    const int x0 = xt * XT, y0 = yt * YT;
+    const int tid = threadIdx.x;
+    const unsigned gp = (unsigned)a.nxg;
+
+    // 1. staging; cells past the ghost layers take the clamped cell's values
+    // (their faces are never finished)
+    double cin[S];
+    {
+        const long sb = ghost_off(a, x0 - 1 - PN, y0 - 1 - PN, k);
+        auto soff = [&](int cx, int ry) -> Off {
+            const int x = min(x0 - 1 - PN + cx, a.m + a.b), y = min(y0 - 1 - PN + ry, a.n + a.b);
+            return Off{sb, (unsigned)(x - (x0 - 1 - PN)) + (unsigned)(y - (y0 - 1 - PN)) * gp};
+        };
+        int cx[2], ry[2];
+        if (tid < NCEN) {
+            cx[0] = tid % XL + PN;
+            ry[0] = tid / XL + 1 + PN;
+        } else {
+            xy_halo_cell<NF, D>(tid - NCEN, cx[0], ry[0]);
+        }
+        const int h1 = tid + (NT - NCEN);
+        const bool two = h1 < NHALO;
+        if (two)
+            xy_halo_cell<NF, D>(h1, cx[1], ry[1]);
+        else
+            cx[1] = cx[0], ry[1] = ry[0];
+        double cv[2][S];
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const Off o = soff(cx[it], ry[it]);
+#pragma unroll
+            for (int v = 0; v < S; ++v) cv[it][v] = el(a.qin.v[v], o);
+        }
+#pragma unroll
+        for (int v = 0; v < S; ++v) cin[v] = cv[0][v];
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            if (it == 0 || two) {
+                double pr[S];
+                cons_to_prim<NF, D>(cv[it], pr, a.eos);
+#pragma unroll
+                for (int v = 0; v < S; ++v) sp[v][cx[it]][ry[it]] = pr[v];
+            }
+        }
+    }
+    __syncthreads();
+
+    const double eps = a.eps;
+    // 2. y faces: column yc (cells x0+yc), lane yr: cell y0-1+yr
+    double fy[S], usy;
+    {
+        const int yc = tid / YK, yr = tid % YK, cx = yc + 1 + PN;
+        double cf[kNCoef], qL[S], qR[S];
+#pragma unroll
+        for (int q = 0; q < kNCoef; ++q) cf[q] = a.ucf2[q];
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            double st[5], vl;
+#pragma unroll
+            for (int w = 0; w < 5; ++w) st[w] = sp[v][cx][yr + w];
+            weno_cell<5, double, true, true>(st, cf, eps, vl, qL[v]);
+            qR[v] = __shfl_down(vl, 1u, YK);
+        }
+        const int jf = y0 - 1 + yr;
+        if (jf == -1 && a.bc2_beg == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qL[v] = qR[v];
+        }
+        if (jf == a.n && a.bc2_end == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qR[v] = qL[v];
+        }
+        hllc_face<NF, D, 1, double>(qL, qR, a.eos, fy, usy);
+    }
+
+    // 3. x faces: row xr (cells y0+xr), lane c: cell x0-1+c (wave 7 has none)
+    const int xr = tid / XL, c = tid % XL;
+    double fx[S], usx = 0.0, fxm[S], usxm = 0.0;
+    if (tid < NCEN) {
+        const int ry = xr + 1 + PN;
+        double cf[kNCoef], qL[S], qR[S];
+#pragma unroll
+        for (int q = 0; q < kNCoef; ++q) cf[q] = a.ucf[q];
+#pragma unroll
+        for (int v = 0; v < S; ++v) {
+            double st[5], vl;
+#pragma unroll
+            for (int w = 0; w < 5; ++w) st[w] = sp[v][c + w][ry];
+            weno_cell<5, double, true, true>(st, cf, eps, vl, qL[v]);
+            qR[v] = __shfl_down(vl, 1u, XL);
+        }
+        const int jf = x0 - 1 + c;
+        if (jf == -1 && a.bc_beg == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qL[v] = qR[v];
+        }
+        if (jf == a.m && a.bc_end == -4) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) qR[v] = qL[v];
+        }
+        hllc_face<NF, D, 0, double>(qL, qR, a.eos, fx, usx);
+#pragma unroll
+        for (int v = 0; v < S; ++v) fxm[v] = __shfl_up(fx[v], 1u, XL);
+        usxm = __shfl_up(usx, 1u, XL);
+    }
+    __syncthreads();
+
+    // 4. y fluxes of faces y0-1/2 .. y0+YT-1/2 over the staged cells
+    {
+        const int yc = tid / YK, yr = tid % YK;
+        if (yr <= YT) {
+#pragma unroll
+            for (int v = 0; v < S; ++v) fyl[v][yr][yc] = fy[v];
+            fyl[S][yr][yc] = usy;
+        }
+    }
+    __syncthreads();
+
+    // 5. finish cell (x0-1+c, y0+xr), c = 1..XT
+    const int i = x0 - 1 + c, j = y0 + xr;
+    if (tid < NCEN && c >= 1 && c <= XT && i <= a.m && j <= a.n) {
+        const int yc = c - 1;
+        const Off go{ghost_off(a, x0, y0, k), (unsigned)yc + (unsigned)xr * gp};
+        const Off ro{interior_off(a, x0, y0, k), (unsigned)yc + (unsigned)xr * (unsigned)(a.m + 1)};
+        double dfy[S];
+#pragma unroll
+        for (int v = 0; v < S; ++v) dfy[v] = fyl[v][xr][yc] - fyl[v][xr + 1][yc];
+        const double dusy = fyl[S][xr + 1][yc] - fyl[S][xr][yc], dusx = usx - usxm;
+        const double ix = a.inv_ds[i], iy = a.inv_ds2[j];
+        double r[S];
+        if (a.fold) {
+            // spread RK stage: rhs (z's part in 3D) + the rest of cb*base +
+            // cdt*(L_x + L_y) + ci*in
+            const double sx = a.cdt * ix, sy = a.cdt * iy;
+#pragma unroll
+            for (int v = 0; v < S; ++v) {
+                double acc = a.accumulate ? el(a.rhs.v[v], ro) : 0.0;
+                if (v >= a.base_lo && v < a.base_hi) acc = fma(a.cb, el(a.qbase.v[v], go), acc);
+                acc = fma(sy, dfy[v], acc);
+                r[v] = fma(sx, fxm[v] - fx[v], acc);
+            }
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                r[Y::ADV + q] = r[Y::ADV + q] + (sx * dusx + sy * dusy) * cin[Y::ADV + q];
+            if (a.ci != 0.0) {
+#pragma unroll
+                for (int v = 0; v < S; ++v) r[v] = r[v] + a.ci * cin[v];
+            }
+#pragma unroll
+            for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = r[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < S; ++v) {
+                const double acc = a.accumulate ? el(a.rhs.v[v], ro) : 0.0;
+                r[v] = acc + iy * dfy[v] + ix * (fxm[v] - fx[v]);
+            }
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                r[Y::ADV + q] = r[Y::ADV + q] + (ix * dusx + iy * dusy) * cin[Y::ADV + q];
+#pragma unroll
+            for (int v = 0; v < S; ++v) el(a.rhs.v[v], ro) = r[v];
+        }
     }
 }
 

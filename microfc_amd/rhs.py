@@ -41,7 +41,7 @@ EXPORTED = [
     "mfc_ml_restrict", "mfc_ml_samples", "mfc_ml_restrict3", "mfc_ml_samples3", "mfc_mlp_last_error",
 ]
 KERNEL_IDS = {"bc_fill": 0, "cons_to_prim": 1, "sweep_x": 2, "sweep_y": 3, "sweep_z": 4, "icfl": 5,
-              "rk1_update": 6, "viscous": 7}
+              "rk1_update": 6, "viscous": 7, "sweep_xy": 8}
 
 
 class MfcError(RuntimeError):

@@ -266,6 +266,27 @@ def small_cases():
     prm, g, q = multi_fluid_3d(5)
     out.append(("fluids5_3d_stretched", prm, g, q))
     out.extend(walls_3d_cases())
+    out.extend(xy_tile_cases())
+    return out
+
+
+def xy_tile_cases():
+    """Grids uniform bit for bit in x and y (power-of-two cell counts), so the
+    contracted form takes its fused x and y sweep: cell counts that leave
+    partial 30 x 14 tiles, every boundary code in x and y, RK2, one to three
+    fluids."""
+    out = []
+    prm, g, q = field_3d_case(31, 31, 31, (-2, -4, -1), (-4, -2, -1))
+    out.append(("xy_tiles_3d_walls", prm, g, q))
+    prm, g, q = field_3d_case(63, 31, 31, (-1, -1, -3), (-1, -1, -3), time_stepper=2)
+    out.append(("xy_tiles_3d_periodic_rk2", prm, g, q))
+    prm, g, q = ic.shock_bubble_2d(N=64)
+    prm.bc_beg = (-2, -4, -3)
+    prm.bc_end = (-4, -2, -3)
+    g = ic.uniform_grids(prm)
+    out.append(("xy_tiles_2d_walls", prm, g, perturbed(prm, q)))
+    prm, g, q = multi_fluid_2d(3, N=64)
+    out.append(("xy_tiles_fluids3_2d", prm, g, q))
     return out
 
 
