@@ -41,29 +41,23 @@ FP64_PEAK_TFLOPS = 78.6
 _P512 = "profiles/sweep_kernels_shock_droplet_3d_512_fp64_fast.md"
 _P2048 = "profiles/sweep_kernels_shock_bubble_2d_2048_fp64_fast.md"
 PROFILED = {
-    ("shock_droplet_3d", 512, "fp64_fast", "sweep_y"): {
-        "traffic": 24.05e9 + 8.59e9, "valu_busy": 0.60,
-        "limiter": "HBM bandwidth in RK stages 2 and 3 (35.9 GB at 5.2 TB/s, VALU 53 % busy); VALU issue in "
-                   "stage 1 (75 % busy)", "profile": _P512},
-    ("shock_droplet_3d", 512, "fp64_fast", "sweep_x"): {
-        "traffic": 17.31e9 + 8.64e9, "valu_busy": 0.69,
-        "limiter": "HBM bandwidth (25.9 GB at 5.0 TB/s) with VALU 69 % busy", "profile": _P512},
+    ("shock_droplet_3d", 512, "fp64_fast", "sweep_xy"): {
+        "traffic": 18.89e9 + 8.60e9, "valu_busy": 0.80,
+        "limiter": "VALU issue (FP64), not HBM (27.5 GB at 3.1 TB/s)", "profile": _P512},
     ("shock_droplet_3d", 512, "fp64_fast", "sweep_z"): {
-        "traffic": 9.09e9 + 8.59e9, "valu_busy": 0.80, "limiter": "VALU issue (FP64), not HBM", "profile": _P512},
-    ("shock_bubble_2d", 2048, "fp64_fast", "sweep_y"): {
-        "traffic": 471.8e6 + 234.9e6, "valu_busy": 0.70,
-        "limiter": "VALU issue in RK stage 1 (78 % busy); HBM in stages 2 and 3 (0.80 GB at 4.8 TB/s)",
-        "profile": _P2048},
-    ("shock_bubble_2d", 2048, "fp64_fast", "sweep_x"): {
-        "traffic": 471.1e6 + 238.9e6, "valu_busy": 0.62,
-        "limiter": "HBM bandwidth (0.71 GB at 4.4 TB/s) with VALU 62 % busy", "profile": _P2048},
+        "traffic": 13.86e9 + 8.59e9, "valu_busy": 0.69,
+        "limiter": "VALU issue in RK stage 1 (80 % busy); with 6 RK-base variables in stages 2 and 3, 24.8 GB at "
+                   "4.2 TB/s", "profile": _P512},
+    ("shock_bubble_2d", 2048, "fp64_fast", "sweep_xy"): {
+        "traffic": 453.1e6 + 236.0e6, "valu_busy": 0.67,
+        "limiter": "VALU issue and latency (63-74 % busy, four waves per SIMD), not HBM", "profile": _P2048},
 }
 
 DEFAULT_PRECISION = "fp64_fast"
 # RK-base variables the contracted form's 3D z sweep adds (the y sweep, or
 # the fused x and y sweep, adds the rest; the library's split)
 BASE_ON_Z = 4
-BASE_ON_Z_FUSED = 4
+BASE_ON_Z_FUSED = 6
 FORMS = {"fp64": "plain FP64: the reference's evaluation order, no FMA contraction, IEEE division (bit-exact "
                  "against the FP64 oracle)",
          "fp64_fast": "contracted FP64: FMA contraction, reciprocal-multiply division, WENO weights normalised "

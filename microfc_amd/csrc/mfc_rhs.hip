@@ -57,9 +57,11 @@ constexpr int kNumFluidsBuiltF32 = 4; // the FP32-bulk and learned forms
 // balances the two sweeps (launches of 5.5 and 5.8 ms instead of 5.0 and
 // 6.3 with the whole base on y).
 constexpr int kBaseOnZ = 4;
-// The fused x and y sweep, and the RK-base variables the z sweep adds before it.
+// The fused x and y sweep, and the RK-base variables the z sweep adds before
+// it (512^3: 47.7, 46.0, 44.9, 44.8, 45.9 ms per step with 0, 2, 4, 6, 8 of
+// the 8; the fused sweep is bound by VALU issue, the z sweep less so).
 constexpr bool kFuseXY = true;
-constexpr int kBaseOnZFused = 4;
+constexpr int kBaseOnZFused = 6;
 
 thread_local std::string g_err;
 
