@@ -881,10 +881,12 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         ci += cb;
         cb = 0.0;
     }
-    // The contracted form on grids uniform in x and y, WENO5, fuses the y and
-    // x sweeps (k_sweep_xy): in 3D after the z sweep, in 2D alone.
+    // The contracted form on grids uniform in x and y, WENO5, one or two
+    // fluids, fuses the y and x sweeps (k_sweep_xy): in 3D after the z sweep,
+    // in 2D alone.
     const bool fuse = reverse && kFuseXY && G.D >= 2 && G.prm.precision == MFC_PREC_FP64_FAST &&
-                      G.prm.weno_order == 5 && G.dir[0].uniform && G.dir[1].uniform;
+                      G.prm.weno_order == 5 && G.prm.num_fluids <= kXyMaxFluids && G.dir[0].uniform &&
+                      G.dir[1].uniform;
     if (fuse) {
         const int split = G.D == 3 ? kBaseOnZFused : 0;
         auto stage = [&](SweepArgs &a, int fold_kind, int lo, int hi) {

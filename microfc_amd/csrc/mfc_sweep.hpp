@@ -104,8 +104,11 @@ constexpr int tc_rows_of(bool f64, bool fast_f64, int dir) { return f64 && !fast
 constexpr int tc_cols_of(bool f64, bool fast_f64, int dir) { return !f64 && dir == 2 ? 32 : 16; }
 
 // Cells per tile of the fused x and y sweep (k_sweep_xy): 30 along x (32
-// x-phase lanes per row), 14 along y (16 y-phase lanes per column).
-constexpr int kXyCellsX = 30, kXyCellsY = 14;
+// x-phase lanes per row), 14 along y (16 y-phase lanes per column).  It is
+// built for one or two fluids: from three on, its staged tile (8 B x 720
+// cells x S) and the per-lane face states outgrow the LDS and registers of
+// two blocks per CU (scratch spills, up to 145 KB of LDS at ten fluids).
+constexpr int kXyCellsX = 30, kXyCellsY = 14, kXyMaxFluids = 2;
 
 // Kernels and their helpers belong to the unit's arithmetic form (see
 // mfc_device.hpp); the argument types above are shared by every form.

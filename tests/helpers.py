@@ -270,12 +270,39 @@ def small_cases():
     return out
 
 
+def one_fluid_3d_case(m, n, p, bcb, bce, seed=9):
+    """One gas in 3D on a uniform grid: density, pressure and every velocity
+    component varying smoothly, periodic in all three directions."""
+    ga, pia = ic.AIR
+    prm = Params(m=m, n=n, p=p, num_fluids=1, weno_order=5, gammas=[ga], pi_infs=[pia], bc_beg=tuple(bcb),
+                 bc_end=tuple(bce))
+    g = ic.uniform_grids(prm)
+    rng = np.random.default_rng(seed)
+    Z, Y, X = np.meshgrid((np.arange(p + 1) + 0.5) / (p + 1), (np.arange(n + 1) + 0.5) / (n + 1),
+                          (np.arange(m + 1) + 0.5) / (m + 1), indexing="ij")
+    ph = rng.uniform(0, 2 * np.pi, 5)
+    prim = np.zeros(prm.interior_shape)
+    prim[0] = 1.2 * (1 + 0.3 * np.sin(2 * np.pi * (X + Y) + ph[0]))
+    prim[1] = 30.0 * np.sin(2 * np.pi * Y + ph[1])
+    prim[2] = -20.0 * np.cos(2 * np.pi * Z + ph[2])
+    prim[3] = 25.0 * np.sin(2 * np.pi * X + ph[3])
+    prim[4] = 1e5 * (1.0 + 0.5 * np.cos(2 * np.pi * (X - Z) + ph[4]))
+    prim[5] = 1.0
+    q = np.zeros(prm.ghost_shape)
+    b = prm.buff_size
+    q[:, b:b + p + 1, b:b + n + 1, b:b + m + 1] = O.prim_to_cons(prm, prim)
+    return prm, g, q
+
+
 def xy_tile_cases():
     """Grids uniform bit for bit in x and y (power-of-two cell counts), so the
-    contracted form takes its fused x and y sweep: cell counts that leave
-    partial 30 x 14 tiles, every boundary code in x and y, RK2, one to three
-    fluids."""
+    contracted form takes its fused x and y sweep (one or two fluids): cell
+    counts that leave partial 30 x 14 tiles, every boundary code in x and y,
+    RK2, one fluid in 3D; and three fluids on such a grid, which keep the
+    separate sweeps."""
     out = []
+    prm, g, q = one_fluid_3d_case(31, 31, 31, (-1, -1, -1), (-1, -1, -1))
+    out.append(("xy_tiles_3d_one_fluid", prm, g, q))
     prm, g, q = field_3d_case(31, 31, 31, (-2, -4, -1), (-4, -2, -1))
     out.append(("xy_tiles_3d_walls", prm, g, q))
     prm, g, q = field_3d_case(63, 31, 31, (-1, -1, -3), (-1, -1, -3), time_stepper=2)
