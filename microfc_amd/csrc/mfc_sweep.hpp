@@ -109,6 +109,7 @@ constexpr int tc_cols_of(bool f64, bool fast_f64, int dir) { return !f64 && dir 
 // cells x S) and the per-lane face states outgrow the LDS and registers of
 // two blocks per CU (scratch spills, up to 145 KB of LDS at ten fluids).
 constexpr int kXyCellsX = 30, kXyCellsY = 14, kXyMaxFluids = 2;
+constexpr int kXyThreads = (kXyCellsX + 2) * (kXyCellsY + 2);
 
 // Kernels and their helpers belong to the unit's arithmetic form (see
 // mfc_device.hpp); the argument types above are shared by every form.
@@ -1181,37 +1182,40 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
 //     flux differences and alpha*div(u) sources, then the rhs store or the
 //     spread RK stage (rhs from the z sweep in 3D, the rest of the base,
 //     ci*in from the kept conservative values).
-template <int NF, int D>
+template <int NCX, int NRY, int PN>
 __device__ __forceinline__ void xy_halo_cell(int h, int &cx, int &ry) {
-    // the 272 halo cells of the 36 x 20 staging: rows 0-2 and 17-19 whole,
-    // then columns 0, 1, 34, 35 of rows 3-16
-    if (h < 6 * 36) {
-        const int band = h / 36;
-        cx = h % 36;
-        ry = band < 3 ? band : band + 14;
+    // the halo ring of the NCX x NRY staging around the central cells
+    // (columns PN..NCX-PN-1 of rows PN+1..NRY-PN-2): the PN+1 rows at
+    // either end whole, then the PN columns at either end of the rows
+    // between
+    constexpr int ROWS = PN + 1;
+    if (h < 2 * ROWS * NCX) {
+        const int band = h / NCX;
+        cx = h % NCX;
+        ry = band < ROWS ? band : band + NRY - 2 * ROWS;
     } else {
-        const int hh = h - 6 * 36, q = hh % 4;
-        ry = 3 + hh / 4;
-        cx = q < 2 ? q : q + 32;
+        const int hh = h - 2 * ROWS * NCX, q = hh % (2 * PN);
+        ry = ROWS + hh / (2 * PN);
+        cx = q < PN ? q : q + NCX - 2 * PN;
     }
 }
 
 template <int NF, int D>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_xy(const SweepArgs a) {
+__global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_xy(const SweepArgs a) {
     static_assert(kFastF64, "the contracted FP64 form only");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int PN = 2;
     constexpr int XT = kXyCellsX, XL = XT + 2; // finished cells per row, x-phase lanes per row
     constexpr int YT = kXyCellsY, YK = YT + 2; // finished rows, y-phase lanes per column
-    constexpr int YC = 32;              // y-phase columns (XT of them finished)
+    constexpr int YC = XL;              // y-phase columns (XT of them finished)
     constexpr int NCX = XL + 2 * PN;    // staged columns x0-1-PN .. x0+XL-2+PN
     constexpr int NRY = YK + 2 * PN;    // staged rows y0-1-PN .. y0+YK-2+PN
     constexpr int P = NRY | 1;          // odd column pitch
-    constexpr int NT = 512;
+    constexpr int NT = kXyThreads;
     constexpr int NCEN = XL * YT;       // central cells, the x phase's own
     constexpr int NHALO = NCX * NRY - NCEN;
-    static_assert(XL == 32 && YK == 16 && NCEN <= NT && NHALO <= 2 * NT - NCEN && YC * YK == NT, "tile shape");
+    static_assert(XL == 32 && NCEN <= NT && NHALO <= 2 * NT - NCEN && YC * YK == NT, "tile shape");
     // y-flux rows padded to YC + 1: the 16 lanes of a column write one
     // column of the rows (with YC, all to one bank)
     constexpr int FP = YC + 1;
@@ -1244,12 +1248,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             cx[0] = tid % XL + PN;
             ry[0] = tid / XL + 1 + PN;
         } else {
-            xy_halo_cell<NF, D>(tid - NCEN, cx[0], ry[0]);
+            xy_halo_cell<NCX, NRY, PN>(tid - NCEN, cx[0], ry[0]);
         }
         const int h1 = tid + (NT - NCEN);
         const bool two = h1 < NHALO;
         if (two)
-            xy_halo_cell<NF, D>(h1, cx[1], ry[1]);
+            xy_halo_cell<NCX, NRY, PN>(h1, cx[1], ry[1]);
         else
             cx[1] = cx[0], ry[1] = ry[0];
         double cv[2][S];
