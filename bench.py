@@ -68,11 +68,10 @@ FORMS = {"fp64": "plain FP64: the reference's evaluation order, no FMA contracti
 
 def sweep_order(prm):
     """The library's sweep order: x, y, z as the reference adds them; the
-    contracted FP64 form of an inviscid case, and the FP32-bulk form of a 2D
-    one, run them in reverse, so that the RK stage's output is written by
-    the x sweep."""
+    contracted FP64 and FP32-bulk forms of an inviscid case run them in
+    reverse, so that the RK stage's output is written by the x sweep."""
     names = ["sweep_x", "sweep_y", "sweep_z"][:prm.num_dims]
-    rev = (prm.precision == 3 or (prm.precision == 1 and prm.num_dims == 2)) and not prm.viscous
+    rev = prm.precision in (1, 3) and not prm.viscous
     return names[::-1] if rev else names
 
 
@@ -81,23 +80,23 @@ def sweep_bytes_per_step(prm, fused=False):
     stencil input is read once (S doubles), rhs is written by the first
     sweep, read and written by middle ones, read by the last, which writes
     the stage output; the RK base (stages 2 and 3) is read by the last sweep,
-    or, in the contracted FP64 form of a 2D/3D case and the FP32-bulk form
-    of a 2D one, which spread the stage over the sweeps, by the y sweep (in
-    3D its first BASE_ON_Z variables by the z sweep).  Viscous runs use the
+    or, in the contracted FP64 and FP32-bulk forms of a 2D/3D case, which
+    spread the stage over the sweeps, by the y sweep (in 3D its first
+    BASE_ON_Z variables by the z sweep).  Viscous runs use the
     viscous kernels in place of the sweeps: per direction the face kernel
     reads the stencil input and writes a face record (HLLC flux, contact
     velocity, viscous flux: S+2+D doubles), the finish reads it and the
     volume fractions, and reads/writes rhs like a sweep (or, in the last
     direction, reads the RK base and stage input and writes the output).
     With weno_Re_flux the velocity-gradient planes and their face averages
-    add their passes.  fused: the contracted form's fused x and y sweep
+    add their passes.  fused: the contracted and FP32-bulk forms' fused x and y sweep
     (sweep_xy, after the z sweep in 3D) reads the stencil input once, the
     z sweep's rhs in 3D and its part of the RK base, and writes the stage
     output."""
     S, D, NF = prm.sys_size, prm.num_dims, prm.num_fluids
     out = {}
     names = sweep_order(prm)
-    fold = D > 1 and (prm.precision == 3 or (prm.precision == 1 and D == 2))
+    fold = D > 1 and prm.precision in (1, 3)
     if prm.viscous:
         total = 0
         for d in range(D):

@@ -739,7 +739,10 @@ SweepArgs xy_args() {
     a.inv_ds2 = G.dir[1].inv_ds;
     a.bc2_beg = G.prm.bc_y_beg;
     a.bc2_end = G.prm.bc_y_end;
-    for (int q = 0; q < kNCoef; ++q) a.ucf2[q] = G.dir[1].ucf[q];
+    for (int q = 0; q < kNCoef; ++q) {
+        a.ucf2[q] = G.dir[1].ucf[q];
+        a.ucf2_32[q] = (float)G.dir[1].ucf[q];
+    }
     a.ntile = (G.prm.m + 1 + kXyCellsX - 1) / kXyCellsX;
     a.nchunk = (G.prm.n + 1 + kXyCellsY - 1) / kXyCellsY;
     a.nblocks = (long)a.ntile * a.nchunk * (G.prm.p + 1);
@@ -827,14 +830,12 @@ int viscous(int d, const Fields &qin, const Fields &rhs, const SweepArgs &sa) {
 int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, const Fields *qout, double dt) {
     // The plain and learned forms add the directions in the reference's
     // order x, y, z (m_rhs.fpp:469-656; it fixes the rounding).  The
-    // contracted FP64 form, and the FP32-bulk form in 2D, run them in the
-    // reverse order: the sum is the same up to rounding, and the RK stage's
-    // output is then written by the x sweep, whose loads are contiguous
-    // rows.  (The 3D FP32-bulk z sweep is as slow as the first sweep as it
-    // is carrying the stage: 512^3 52.6 ms per step reversed, 51.8 not.)
-    const bool reverse = (G.prm.precision == MFC_PREC_FP64_FAST ||
-                          (G.prm.precision == MFC_PREC_FP32_BULK && G.D == 2)) &&
-                         !G.visc;
+    // contracted FP64 and FP32-bulk forms run them in the reverse order: the
+    // sum is the same up to rounding, and the RK stage's output is then
+    // written by the x sweep (or the fused x and y sweep), whose loads are
+    // contiguous rows.
+    const bool reverse =
+        (G.prm.precision == MFC_PREC_FP64_FAST || G.prm.precision == MFC_PREC_FP32_BULK) && !G.visc;
     if (G.wrf) {
         // cell-centred velocity gradients of the stage input (s_get_viscous,
         // m_rhs.fpp:456-464), before the directions
@@ -881,10 +882,11 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         ci += cb;
         cb = 0.0;
     }
-    // The contracted form on grids uniform in x and y, WENO5, one or two
-    // fluids, fuses the y and x sweeps (k_sweep_xy): in 3D after the z sweep,
-    // in 2D alone.
-    const bool fuse = reverse && kFuseXY && G.D >= 2 && G.prm.precision == MFC_PREC_FP64_FAST &&
+    // The contracted and FP32-bulk forms on grids uniform in x and y, WENO5,
+    // one or two fluids, fuse the y and x sweeps (k_sweep_xy): in 3D after
+    // the z sweep, in 2D alone.
+    const bool fuse = reverse && kFuseXY && G.D >= 2 &&
+                      (G.prm.precision == MFC_PREC_FP64_FAST || G.prm.precision == MFC_PREC_FP32_BULK) &&
                       G.prm.weno_order == 5 && G.prm.num_fluids <= kXyMaxFluids && G.dir[0].uniform &&
                       G.dir[1].uniform;
     if (fuse) {

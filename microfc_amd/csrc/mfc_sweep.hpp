@@ -85,6 +85,7 @@ struct SweepArgs {
     const double *inv_ds2;
     int bc2_beg, bc2_end;
     double ucf2[kNCoef];
+    float ucf2_32[kNCoef];
 };
 
 // Threads per column of the cell-parallel y/z tile by form.  Measured on
@@ -168,9 +169,9 @@ struct FinishIn {
     double rhs[S], base[S], in[S];
 };
 
-// Which finish modes a sweep kernel can meet.  The contracted FP64 form of
-// a 2D/3D case and the FP32-bulk form of a 2D case never pass an RK stage
-// to a sweep: they spread the stage over the sweeps (SweepArgs::fold),
+// Which finish modes a sweep kernel can meet.  The contracted FP64 and
+// FP32-bulk forms of a 2D/3D case never pass an RK stage to a sweep: they
+// spread the stage over the sweeps (SweepArgs::fold),
 // their x sweep, always the last, meets folds 0 and 3, their y/z sweeps
 // folds 0, 1 and 2.  The other forms and cases meet rk and never fold.
 // Fixing the set at compile time keeps the modes a kernel cannot meet out
@@ -178,7 +179,7 @@ struct FinishIn {
 enum FinishModes { kModesAll = 0, kModesFoldLast = 1, kModesFoldFirst = 2 };
 template <int D, typename R, bool ML>
 constexpr bool folded_form() {
-    return (kFastF64 && sizeof(R) == 8 && D > 1) || (sizeof(R) == 4 && !ML && D == 2);
+    return (kFastF64 && sizeof(R) == 8 && D > 1) || (sizeof(R) == 4 && !ML && D > 1);
 }
 template <int D, bool LAST, typename R, bool ML>
 constexpr int finish_modes() {
@@ -1163,8 +1164,10 @@ __global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
 }
 
 // ---------------------------------------------------------------------------
-// Fused x and y sweep of the contracted FP64 form (WENO5 on grids uniform in
-// x and y): a 512-thread block finishes a tile of XT = 30 x YT = 14 cells of
+// Fused x and y sweep of the contracted FP64 and FP32-bulk forms (WENO5 on
+// grids uniform in x and y; the FP32-bulk form reconstructs and solves in
+// FP32 and finishes in FP64 with its volume-fraction clip): a 512-thread
+// block finishes a tile of XT = 30 x YT = 14 cells of
 // one z plane, so neither the rhs between the two directions nor a second
 // pass over the stencil input goes through HBM.
 //  1. The 36 x 20 cells x0-3..x0+32, y0-3..y0+16 are staged (cons->prim) in
@@ -1200,9 +1203,9 @@ __device__ __forceinline__ void xy_halo_cell(int h, int &cx, int &ry) {
     }
 }
 
-template <int NF, int D>
+template <int NF, int D, typename R>
 __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_xy(const SweepArgs a) {
-    static_assert(kFastF64, "the contracted FP64 form only");
+    static_assert((kFastF64 && sizeof(R) == 8) || sizeof(R) == 4, "the contracted FP64 and FP32-bulk forms");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int PN = 2;
@@ -1221,9 +1224,9 @@ __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8
     constexpr int FP = YC + 1;
     constexpr int SP_WORDS = S * NCX * P, FY_WORDS = (S + 1) * (YT + 1) * FP;
     static_assert(FY_WORDS <= SP_WORDS, "y fluxes fit over the staged cells");
-    __shared__ double lds[SP_WORDS];
-    double(*sp)[NCX][P] = reinterpret_cast<double(*)[NCX][P]>(lds);
-    double(*fyl)[YT + 1][FP] = reinterpret_cast<double(*)[YT + 1][FP]>(lds);
+    __shared__ R lds[SP_WORDS];
+    R(*sp)[NCX][P] = reinterpret_cast<R(*)[NCX][P]>(lds);
+    R(*fyl)[YT + 1][FP] = reinterpret_cast<R(*)[YT + 1][FP]>(lds);
 
     const long bid = xcd_logical_block(blockIdx.x, a.nblocks);
     const int xt = (int)(bid % a.ntile);
@@ -1271,26 +1274,26 @@ __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8
                 double pr[S];
                 cons_to_prim<NF, D>(cv[it], pr, a.eos);
 #pragma unroll
-                for (int v = 0; v < S; ++v) sp[v][cx[it]][ry[it]] = pr[v];
+                for (int v = 0; v < S; ++v) sp[v][cx[it]][ry[it]] = (R)pr[v];
             }
         }
     }
     __syncthreads();
 
-    const double eps = a.eps;
+    const R eps = (R)a.eps;
     // 2. y faces: column yc (cells x0+yc), lane yr: cell y0-1+yr
-    double fy[S], usy;
+    R fy[S], usy;
     {
         const int yc = tid / YK, yr = tid % YK, cx = yc + 1 + PN;
-        double cf[kNCoef], qL[S], qR[S];
+        R cf[kNCoef], qL[S], qR[S];
 #pragma unroll
-        for (int q = 0; q < kNCoef; ++q) cf[q] = a.ucf2[q];
+        for (int q = 0; q < kNCoef; ++q) cf[q] = sizeof(R) == 8 ? (R)a.ucf2[q] : (R)a.ucf2_32[q];
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            double st[5], vl;
+            R st[5], vl;
 #pragma unroll
             for (int w = 0; w < 5; ++w) st[w] = sp[v][cx][yr + w];
-            weno_cell<5, double, true, true>(st, cf, eps, vl, qL[v]);
+            weno_cell<5, R, true, true>(st, cf, eps, vl, qL[v]);
             qR[v] = __shfl_down(vl, 1u, YK);
         }
         const int jf = y0 - 1 + yr;
@@ -1302,23 +1305,23 @@ __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8
 #pragma unroll
             for (int v = 0; v < S; ++v) qR[v] = qL[v];
         }
-        hllc_face<NF, D, 1, double>(qL, qR, a.eos, fy, usy);
+        hllc_face<NF, D, 1, R>(qL, qR, a.eos, fy, usy);
     }
 
     // 3. x faces: row xr (cells y0+xr), lane c: cell x0-1+c (wave 7 has none)
     const int xr = tid / XL, c = tid % XL;
-    double fx[S], usx = 0.0, fxm[S], usxm = 0.0;
+    R fx[S], usx = R(0), fxm[S], usxm = R(0);
     if (tid < NCEN) {
         const int ry = xr + 1 + PN;
-        double cf[kNCoef], qL[S], qR[S];
+        R cf[kNCoef], qL[S], qR[S];
 #pragma unroll
-        for (int q = 0; q < kNCoef; ++q) cf[q] = a.ucf[q];
+        for (int q = 0; q < kNCoef; ++q) cf[q] = sizeof(R) == 8 ? (R)a.ucf[q] : (R)a.ucf32[q];
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            double st[5], vl;
+            R st[5], vl;
 #pragma unroll
             for (int w = 0; w < 5; ++w) st[w] = sp[v][c + w][ry];
-            weno_cell<5, double, true, true>(st, cf, eps, vl, qL[v]);
+            weno_cell<5, R, true, true>(st, cf, eps, vl, qL[v]);
             qR[v] = __shfl_down(vl, 1u, XL);
         }
         const int jf = x0 - 1 + c;
@@ -1330,7 +1333,7 @@ __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8
 #pragma unroll
             for (int v = 0; v < S; ++v) qR[v] = qL[v];
         }
-        hllc_face<NF, D, 0, double>(qL, qR, a.eos, fx, usx);
+        hllc_face<NF, D, 0, R>(qL, qR, a.eos, fx, usx);
 #pragma unroll
         for (int v = 0; v < S; ++v) fxm[v] = __shfl_up(fx[v], 1u, XL);
         usxm = __shfl_up(usx, 1u, XL);
@@ -1354,10 +1357,14 @@ __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8
         const int yc = c - 1;
         const Off go{ghost_off(a, x0, y0, k), (unsigned)yc + (unsigned)xr * gp};
         const Off ro{interior_off(a, x0, y0, k), (unsigned)yc + (unsigned)xr * (unsigned)(a.m + 1)};
-        double dfy[S];
+        // flux differences in FP64 (the FP32-bulk form converts its fluxes)
+        double dfy[S], dfx[S];
 #pragma unroll
-        for (int v = 0; v < S; ++v) dfy[v] = fyl[v][xr][yc] - fyl[v][xr + 1][yc];
-        const double dusy = fyl[S][xr + 1][yc] - fyl[S][xr][yc], dusx = usx - usxm;
+        for (int v = 0; v < S; ++v) {
+            dfy[v] = to_d(fyl[v][xr][yc]) - to_d(fyl[v][xr + 1][yc]);
+            dfx[v] = to_d(fxm[v]) - to_d(fx[v]);
+        }
+        const double dusy = to_d(fyl[S][xr + 1][yc]) - to_d(fyl[S][xr][yc]), dusx = to_d(usx) - to_d(usxm);
         const double ix = a.inv_ds[i], iy = a.inv_ds2[j];
         double r[S];
         if (a.fold) {
@@ -1369,7 +1376,7 @@ __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8
                 double acc = a.accumulate ? el(a.rhs.v[v], ro) : 0.0;
                 if (v >= a.base_lo && v < a.base_hi) acc = fma(a.cb, el(a.qbase.v[v], go), acc);
                 acc = fma(sy, dfy[v], acc);
-                r[v] = fma(sx, fxm[v] - fx[v], acc);
+                r[v] = fma(sx, dfx[v], acc);
             }
 #pragma unroll
             for (int q = 0; q < NF; ++q)
@@ -1378,13 +1385,18 @@ __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8
 #pragma unroll
                 for (int v = 0; v < S; ++v) r[v] = r[v] + a.ci * cin[v];
             }
+            if constexpr (sizeof(R) == 4) {
+                // FP32 bulk: volume fractions clipped to [0, 1], as finish_cell
+#pragma unroll
+                for (int q = 0; q < NF; ++q) r[Y::ADV + q] = min(max(r[Y::ADV + q], 0.0), 1.0);
+            }
 #pragma unroll
             for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = r[v];
         } else {
 #pragma unroll
             for (int v = 0; v < S; ++v) {
                 const double acc = a.accumulate ? el(a.rhs.v[v], ro) : 0.0;
-                r[v] = acc + iy * dfy[v] + ix * (fxm[v] - fx[v]);
+                r[v] = acc + iy * dfy[v] + ix * dfx[v];
             }
 #pragma unroll
             for (int q = 0; q < NF; ++q)
