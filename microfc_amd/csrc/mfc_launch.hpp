@@ -31,11 +31,11 @@ int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
         } else if (dir == 0) {                                                                          \
             hipLaunchKernelGGL((k_sweep_x<NF, D, WO, R, ML, UNI>), grid, dim3(64), 0, st, a);                \
         } else if (dir == 1) {                                                                          \
-            if constexpr (D >= 2 && t_cell_parallel<ML, R, UNI>()) hipLaunchKernelGGL((k_sweep_tc<NF, D, 1, WO, R, ML, UNI>), grid, dim3(tc_cols<R>(1) * tc_rows<R>(1)), 0, st, a); \
+            if constexpr (D >= 2 && t_cell_parallel<ML, R, UNI>()) hipLaunchKernelGGL((k_sweep_tc<NF, D, 1, WO, R, ML, UNI>), grid, dim3(tc_cols<R, ML>(1) * tc_rows<R>(1)), 0, st, a); \
             else if constexpr (D >= 2) hipLaunchKernelGGL((k_sweep_t<NF, D, 1, WO, R, ML, UNI>), grid, dim3(t_threads<ML>()), 0, st, a); \
             else return -1;                                                                             \
         } else {                                                                                        \
-            if constexpr (D >= 3 && t_cell_parallel<ML, R, UNI>()) hipLaunchKernelGGL((k_sweep_tc<NF, D, 2, WO, R, ML, UNI>), grid, dim3(tc_cols<R>(2) * tc_rows<R>(2)), 0, st, a); \
+            if constexpr (D >= 3 && t_cell_parallel<ML, R, UNI>()) hipLaunchKernelGGL((k_sweep_tc<NF, D, 2, WO, R, ML, UNI>), grid, dim3(tc_cols<R, ML>(2) * tc_rows<R>(2)), 0, st, a); \
             else if constexpr (D >= 3) hipLaunchKernelGGL((k_sweep_t<NF, D, 2, WO, R, ML, UNI>), grid, dim3(t_threads<ML>()), 0, st, a); \
             else return -1;                                                                             \
         }                                                                                               \

@@ -718,7 +718,7 @@ SweepArgs sweep_args(int d) {
                             ? (uni ? t_cell_parallel<false, float, true>() : t_cell_parallel<false, float, false>())
                             : (uni ? t_cell_parallel<false, double, true>() : t_cell_parallel<false, double, false>());
         const bool f64 = G.prm.precision == MFC_PREC_FP64 || G.prm.precision == MFC_PREC_FP64_FAST;
-        const int xc = !cp ? kTX : tc_cols_of(f64, G.prm.precision == MFC_PREC_FP64_FAST, d);
+        const int xc = !cp ? kTX : tc_cols_of(G.prm.precision == MFC_PREC_FP32_LEARNED, d);
         a.ntile = (G.prm.m + 1 + xc - 1) / xc;
         a.chunk = G.prm.precision == MFC_PREC_FP32_LEARNED ? t_tile_cells<true, float, true>(d)
                   : G.prm.precision == MFC_PREC_FP32_BULK

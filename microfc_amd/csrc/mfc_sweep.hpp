@@ -98,11 +98,13 @@ struct SweepArgs {
 // gains from 32 at 256^3 but loses 6 % at 512^3 (36 planes per block), and
 // in FP32, where the 3D sweeps near the HBM rate, 32 is up to 11 % slower.
 constexpr int tc_rows_of(bool f64, bool fast_f64, int dir) { return f64 && !fast_f64 && dir == 1 ? 32 : 16; }
-// x columns per cell-parallel tile by form: 32 for the FP32 z sweep, whose
-// stencil rows are planes apart (twice the cells per page touched; measured
-// 7.5 % faster at 512^3, 5 % at 256^3; 64 columns and FP64 gain nothing),
-// else 16.
-constexpr int tc_cols_of(bool f64, bool fast_f64, int dir) { return !f64 && dir == 2 ? 32 : 16; }
+// x columns per cell-parallel tile: 32 for the learned form's z sweep,
+// whose stencil rows are planes apart (twice the cells per page touched;
+// 512^3 70.3 against 71.2 ms per step), else 16.  (32 for the FP32-bulk z
+// sweep had been 7.5 % faster at 512^3 when that sweep came last and
+// carried the RK stage; as the first sweep of the reversed order it is
+// faster with 16: 5.55 against 6.18 ms.)
+constexpr int tc_cols_of(bool learned, int dir) { return learned && dir == 2 ? 32 : 16; }
 
 // Cells per tile of the fused x and y sweep (k_sweep_xy): 30 along x (32
 // x-phase lanes per row), 14 along y (16 y-phase lanes per column).  It is
@@ -798,8 +800,8 @@ constexpr bool t_cell_parallel() { return x_cell_parallel<ML, R, UNI>(); }
 // contracted FP64 differ, so the host picks by form too).
 template <typename R>
 constexpr int tc_rows(int dir) { return tc_rows_of(sizeof(R) == 8, kFastF64, dir); }
-template <typename R>
-constexpr int tc_cols(int dir) { return tc_cols_of(sizeof(R) == 8, kFastF64, dir); }
+template <typename R, bool ML>
+constexpr int tc_cols(int dir) { return tc_cols_of(ML, dir); }
 template <bool ML, typename R, bool UNI>
 constexpr int t_tile_cells(int dir) {
     return t_cell_parallel<ML, R, UNI>() ? tc_rows<R>(dir) - 2 : t_faces<ML>() - 1;
@@ -968,7 +970,7 @@ __global__ __launch_bounds__(t_threads<ML>()) __attribute__((amdgpu_waves_per_eu
 
 // ---------------------------------------------------------------------------
 // y / z sweep, cell-parallel (the forms with a cell-parallel x sweep), K =
-// tc_rows<R>(DIR) threads per column, C = tc_cols<R>(DIR) columns, C * K
+// tc_rows<R>(DIR) threads per column, C = tc_cols<R, ML>(DIR) columns, C * K
 // threads per block.  Lane l of wave w works on column
 // x0 + (64/K)w + l/K; its position ts = l%K among the column's K lanes is
 // cell y0-1+ts, of which it reconstructs both faces (one set of smoothness
@@ -984,13 +986,13 @@ __device__ __forceinline__ R row_next(R x) {
 }
 
 template <int NF, int D, int DIR, int WO, typename R, bool ML, bool UNI>
-__global__ __launch_bounds__(tc_cols<R>(DIR) * tc_rows<R>(DIR))
+__global__ __launch_bounds__((tc_cols<R, ML>(DIR) * tc_rows<R>(DIR)))
     __attribute__((amdgpu_waves_per_eu(ML ? 2 : 4, 8))) void k_sweep_tc(const SweepArgs a) {
     static_assert(t_cell_parallel<ML, R, UNI>(), "cell-parallel forms only");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int FM = finish_modes<D, false, R, ML>();
-    constexpr int XC = tc_cols<R>(DIR), kTS = tc_rows<R>(DIR), NT = XC * kTS;
+    constexpr int XC = tc_cols<R, ML>(DIR), kTS = tc_rows<R>(DIR), NT = XC * kTS;
     constexpr int TC = t_tile_cells<ML, R, UNI>(DIR);
     constexpr int PN = (WO - 1) / 2;
     constexpr int NR = kTS + 2 * PN; // staged cells y0-1-PN .. y0+kTS-2+PN
