@@ -1169,9 +1169,9 @@ __global__ __launch_bounds__((tc_cols<R, ML>(DIR) * tc_rows<R>(DIR)))
 // Fused x and y sweep of the contracted FP64 and FP32-bulk forms (WENO5 on
 // grids uniform in x and y; the FP32-bulk form reconstructs and solves in
 // FP32 and finishes in FP64 with its volume-fraction clip): a 512-thread
-// block finishes a tile of XT = 30 x YT = 14 cells of
-// one z plane, so neither the rhs between the two directions nor a second
-// pass over the stencil input goes through HBM.
+// block finishes a tile of XT = 30 x YT = 14 cells of one z plane, so
+// neither the rhs between the two directions nor a second pass over the
+// stencil input goes through HBM.
 //  1. The 36 x 20 cells x0-3..x0+32, y0-3..y0+16 are staged (cons->prim) in
 //     LDS as columns of odd pitch; the tile's central 32 x 14 cells first,
 //     one per thread of threads 0..447, which keep their conservative
