@@ -89,10 +89,12 @@ def sweep_bytes_per_step(prm, fused=False):
     volume fractions, and reads/writes rhs like a sweep (or, in the last
     direction, reads the RK base and stage input and writes the output).
     With weno_Re_flux the velocity-gradient planes and their face averages
-    add their passes.  fused: the contracted and FP32-bulk forms' fused x and y sweep
-    (sweep_xy, after the z sweep in 3D) reads the stencil input once, the
-    z sweep's rhs in 3D and its part of the RK base, and writes the stage
-    output."""
+    add their passes.  fused: the fused x and y sweep (sweep_xy) reads the
+    stencil input once; in the contracted and FP32-bulk forms it comes after
+    the z sweep in 3D, reads the z sweep's rhs and its part of the RK base
+    and writes the stage output; in the plain form it comes first, writing
+    rhs in 3D (the z sweep then carries the RK stage) and the stage output
+    in 2D."""
     S, D, NF = prm.sys_size, prm.num_dims, prm.num_fluids
     out = {}
     names = sweep_order(prm)
@@ -110,6 +112,13 @@ def sweep_bytes_per_step(prm, fused=False):
                 per += (NF + 2 * D) + 3 * D * D
             total += 3 * 8 * per
         out["viscous"] = total
+        return out
+    if fused and prm.precision == 0:
+        if D == 3:
+            out["sweep_xy"] = 8 * 3 * 2 * S
+            out["sweep_z"] = 8 * (3 * 3 * S + 2 * S)
+        else:
+            out["sweep_xy"] = 8 * (3 * 2 * S + 2 * S)
         return out
     if fused:
         zb = BASE_ON_Z_FUSED if D == 3 else 0

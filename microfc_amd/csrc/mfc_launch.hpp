@@ -26,7 +26,7 @@ int launch_sweep(const SweepLaunch &L, const SweepArgs &a, hipStream_t st);
     static int launch_nf##NF##_dwrm(int dir, const SweepArgs &a, hipStream_t st) {                      \
         const dim3 grid((unsigned)a.nblocks);                                                           \
         if (dir == 3) {                                                                                 \
-            if constexpr (((kFastF64 && sizeof(R) == 8) || sizeof(R) == 4) && NF <= kXyMaxFluids && D >= 2 && WO == 5 && !ML && UNI) hipLaunchKernelGGL((k_sweep_xy<NF, D, R>), grid, dim3(kXyThreads), 0, st, a); \
+            if constexpr (NF <= kXyMaxFluids && D >= 2 && WO == 5 && !ML && UNI) hipLaunchKernelGGL((k_sweep_xy<NF, D, R>), grid, dim3(kXyThreads), 0, st, a); \
             else return -1;                                                                             \
         } else if (dir == 0) {                                                                          \
             hipLaunchKernelGGL((k_sweep_x<NF, D, WO, R, ML, UNI>), grid, dim3(64), 0, st, a);                \

@@ -918,6 +918,36 @@ int sweeps(const Fields &qin, const Fields &rhs, int rk, const Fields *qbase, co
         a.accumulate = G.D == 3;
         return sweep(3, a);
     }
+    // The plain form fuses its first two directions the same way (bitwise
+    // the separate x and y sweeps): in 3D the z sweep then adds its part and
+    // applies the RK stage, in 2D the fused sweep applies it.
+    const bool fuse_plain = kFuseXY && !G.visc && G.D >= 2 && G.prm.precision == MFC_PREC_FP64 &&
+                            G.prm.weno_order == 5 && G.prm.num_fluids <= kXyMaxFluids && G.dir[0].uniform &&
+                            G.dir[1].uniform;
+    if (fuse_plain) {
+        auto finish = [&](SweepArgs &a, bool last) {
+            a.qin = qin;
+            a.rhs = rhs;
+            a.dt = dt;
+            a.rk = last ? rk : kRkNone;
+            if (a.rk != kRkNone) {
+                a.qbase = *qbase;
+                a.qout = *qout;
+                a.base_is_in = qbase->v[0] == qin.v[0];
+            }
+        };
+        SweepArgs a = xy_args();
+        finish(a, G.D == 2);
+        a.accumulate = 0;
+        if (int rc = sweep(3, a)) return rc;
+        if (G.D == 3) {
+            SweepArgs z = sweep_args(2);
+            finish(z, true);
+            z.accumulate = 1;
+            return sweep(2, z);
+        }
+        return 0;
+    }
     for (int k = 0; k < G.D; ++k) {
         const int d = reverse ? G.D - 1 - k : k;
         SweepArgs a = sweep_args(d);

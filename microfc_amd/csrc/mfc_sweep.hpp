@@ -1166,9 +1166,10 @@ __global__ __launch_bounds__((tc_cols<R, ML>(DIR) * tc_rows<R>(DIR)))
 }
 
 // ---------------------------------------------------------------------------
-// Fused x and y sweep of the contracted FP64 and FP32-bulk forms (WENO5 on
-// grids uniform in x and y; the FP32-bulk form reconstructs and solves in
-// FP32 and finishes in FP64 with its volume-fraction clip): a 512-thread
+// Fused x and y sweep of the plain and contracted FP64 and the FP32-bulk
+// forms (WENO5 on grids uniform in x and y; the FP32-bulk form reconstructs
+// and solves in FP32 and finishes in FP64 with its volume-fraction clip; the
+// plain form finishes as its separate x and y sweeps would): a 512-thread
 // block finishes a tile of XT = 30 x YT = 14 cells of one z plane, so
 // neither the rhs between the two directions nor a second pass over the
 // stencil input goes through HBM.
@@ -1207,7 +1208,7 @@ __device__ __forceinline__ void xy_halo_cell(int h, int &cx, int &ry) {
 
 template <int NF, int D, typename R>
 __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_xy(const SweepArgs a) {
-    static_assert((kFastF64 && sizeof(R) == 8) || sizeof(R) == 4, "the contracted FP64 and FP32-bulk forms");
+    static_assert(sizeof(R) == 8 || sizeof(R) == 4, "FP64 or FP32 reconstruction");
     using Y = Sys<NF, D>;
     constexpr int S = Y::S;
     constexpr int PN = 2;
@@ -1369,6 +1370,36 @@ __global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(4, 8
         const double dusy = to_d(fyl[S][xr + 1][yc]) - to_d(fyl[S][xr][yc]), dusx = to_d(usx) - to_d(usxm);
         const double ix = a.inv_ds[i], iy = a.inv_ds2[j];
         double r[S];
+        if constexpr (!kFastF64 && sizeof(R) == 8) {
+            // plain FP64: the x sweep's finish (the first direction: it
+            // assigns) and then the y sweep's, operation for operation, so
+            // the result is bitwise theirs; then the rhs store or the RK
+            // stage (2D, where y is the last direction)
+#pragma unroll
+            for (int v = 0; v < S; ++v) r[v] = ix * dfx[v];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) r[Y::ADV + q] = r[Y::ADV + q] + ix * cin[Y::ADV + q] * dusx;
+#pragma unroll
+            for (int v = 0; v < S; ++v) r[v] = r[v] + iy * dfy[v];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) r[Y::ADV + q] = r[Y::ADV + q] + iy * cin[Y::ADV + q] * dusy;
+            if (a.rk == kRkNone) {
+#pragma unroll
+                for (int v = 0; v < S; ++v) el(a.rhs.v[v], ro) = r[v];
+            } else {
+                FinishIn<S> f;
+#pragma unroll
+                for (int v = 0; v < S; ++v) {
+                    f.in[v] = cin[v];
+                    f.base[v] = a.base_is_in ? cin[v] : el(a.qbase.v[v], go);
+                }
+                double o[S];
+                rk_combine<S>(a, f, r, o);
+#pragma unroll
+                for (int v = 0; v < S; ++v) el(a.qout.v[v], go) = o[v];
+            }
+            return;
+        }
         if (a.fold) {
             // spread RK stage: rhs (z's part in 3D) + the rest of cb*base +
             // cdt*(L_x + L_y) + ci*in
