@@ -1,0 +1,51 @@
+"""Which sweep kernels a case runs, read from the library's per-kernel HIP-event
+timing (mfc_timing_read): the fused x and y sweep on grids uniform in x and y
+with WENO5 and one or two fluids, in the plain, contracted and FP32-bulk
+forms; the separate sweeps on stretched grids, with three fluids and in the
+learned form.  The results of every path are checked against the oracle by
+the parity, contracted-form and FP32-bulk tests; this pins that the fused
+path is the one those tests exercise on the uniform cases."""
+import pytest
+
+from microfc_amd import PREC_FP32_BULK, PREC_FP64, PREC_FP64_FAST, RhsModule
+
+from helpers import field_3d_case, multi_fluid_2d, say, stretched_grids
+
+pytestmark = pytest.mark.gpu
+
+
+def _launches(torch, prm, grids, q, precision):
+    prm.precision = precision
+    mod = RhsModule(prm, grids)
+    try:
+        qd = torch.from_numpy(q.copy()).cuda()
+        rd = mod.new_field(ghosted=False)
+        mod.timing(enable=True, reset=True)
+        mod.s_compute_rhs(qd, None, rd, 0)
+        mod.sync()
+        t = mod.timing(enable=False)
+        return {k: v[1] for k, v in t.items() if v[1]}
+    finally:
+        prm.precision = 0
+        mod.finalize()
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP64_FAST, PREC_FP32_BULK], ids=["fp64", "fast", "fp32"])
+def test_uniform_grid_runs_the_fused_sweep(gpu, precision):
+    prm, g, q = field_3d_case(31, 31, 31, (-2, -4, -1), (-4, -2, -1))
+    n = _launches(gpu, prm, g, q, precision)
+    say(f"[dispatch] uniform 3D, precision {precision}: {n}")
+    assert n.get("sweep_xy") == 1 and n.get("sweep_z") == 1
+    assert "sweep_x" not in n and "sweep_y" not in n
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP64_FAST], ids=["fp64", "fast"])
+def test_stretched_grid_and_three_fluids_run_separate_sweeps(gpu, precision):
+    prm, g, q = field_3d_case(31, 31, 31, (-2, -4, -1), (-4, -2, -1))
+    n = _launches(gpu, prm, stretched_grids(prm), q, precision)
+    say(f"[dispatch] stretched 3D, precision {precision}: {n}")
+    assert "sweep_xy" not in n and n.get("sweep_x") == 1 and n.get("sweep_y") == 1
+    prm, g, q = multi_fluid_2d(3, N=64)
+    n = _launches(gpu, prm, g, q, precision)
+    say(f"[dispatch] three fluids 2D, precision {precision}: {n}")
+    assert "sweep_xy" not in n and n.get("sweep_x") == 1 and n.get("sweep_y") == 1
