@@ -68,10 +68,10 @@ FORMS = {"fp64": "plain FP64: the reference's evaluation order, no FMA contracti
 
 def sweep_order(prm):
     """The library's sweep order: x, y, z as the reference adds them; the
-    contracted FP64 and FP32-bulk forms of an inviscid case run them in
-    reverse, so that the RK stage's output is written by the x sweep."""
+    contracted FP64, FP32-bulk and learned forms of an inviscid case run them
+    in reverse, so that the RK stage's output is written by the x sweep."""
     names = ["sweep_x", "sweep_y", "sweep_z"][:prm.num_dims]
-    rev = prm.precision in (1, 3) and not prm.viscous
+    rev = prm.precision in (1, 2, 3) and not prm.viscous
     return names[::-1] if rev else names
 
 
@@ -80,8 +80,8 @@ def sweep_bytes_per_step(prm, fused=False):
     stencil input is read once (S doubles), rhs is written by the first
     sweep, read and written by middle ones, read by the last, which writes
     the stage output; the RK base (stages 2 and 3) is read by the last sweep,
-    or, in the contracted FP64 and FP32-bulk forms of a 2D/3D case, which
-    spread the stage over the sweeps, by the y sweep (in 3D its first
+    or, in the contracted FP64, FP32-bulk and learned forms of a 2D/3D case,
+    which spread the stage over the sweeps, by the y sweep (in 3D its first
     BASE_ON_Z variables by the z sweep).  Viscous runs use the
     viscous kernels in place of the sweeps: per direction the face kernel
     reads the stencil input and writes a face record (HLLC flux, contact
@@ -98,7 +98,7 @@ def sweep_bytes_per_step(prm, fused=False):
     S, D, NF = prm.sys_size, prm.num_dims, prm.num_fluids
     out = {}
     names = sweep_order(prm)
-    fold = D > 1 and prm.precision in (1, 3)
+    fold = D > 1 and prm.precision in (1, 2, 3)
     if prm.viscous:
         total = 0
         for d in range(D):

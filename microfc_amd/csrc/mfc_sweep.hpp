@@ -98,13 +98,12 @@ struct SweepArgs {
 // gains from 32 at 256^3 but loses 6 % at 512^3 (36 planes per block), and
 // in FP32, where the 3D sweeps near the HBM rate, 32 is up to 11 % slower.
 constexpr int tc_rows_of(bool f64, bool fast_f64, int dir) { return f64 && !fast_f64 && dir == 1 ? 32 : 16; }
-// x columns per cell-parallel tile: 32 for the learned form's z sweep,
-// whose stencil rows are planes apart (twice the cells per page touched;
-// 512^3 70.3 against 71.2 ms per step), else 16.  (32 for the FP32-bulk z
-// sweep had been 7.5 % faster at 512^3 when that sweep came last and
-// carried the RK stage; as the first sweep of the reversed order it is
-// faster with 16: 5.55 against 6.18 ms.)
-constexpr int tc_cols_of(bool learned, int dir) { return learned && dir == 2 ? 32 : 16; }
+// x columns per cell-parallel tile: 16 in every form.  (32 for the FP32
+// z sweeps, whose stencil rows are planes apart, had been 7.5 % faster at
+// 512^3 while z came last and carried the RK stage; first, in the reversed
+// order, 16 is faster: FP32 bulk z sweep 5.55 against 6.18 ms per launch,
+// learned weights 65.0 against 66.1 ms per step.)
+constexpr int tc_cols_of(bool /*learned*/, int /*dir*/) { return 16; }
 
 // Cells per tile of the fused x and y sweep (k_sweep_xy): 30 along x (32
 // x-phase lanes per row), 14 along y (16 y-phase lanes per column).  It is
@@ -171,9 +170,9 @@ struct FinishIn {
     double rhs[S], base[S], in[S];
 };
 
-// Which finish modes a sweep kernel can meet.  The contracted FP64 and
-// FP32-bulk forms of a 2D/3D case never pass an RK stage to a sweep: they
-// spread the stage over the sweeps (SweepArgs::fold),
+// Which finish modes a sweep kernel can meet.  The contracted FP64, the
+// FP32-bulk and the learned forms of a 2D/3D case never pass an RK stage to
+// a sweep: they spread the stage over the sweeps (SweepArgs::fold),
 // their x sweep, always the last, meets folds 0 and 3, their y/z sweeps
 // folds 0, 1 and 2.  The other forms and cases meet rk and never fold.
 // Fixing the set at compile time keeps the modes a kernel cannot meet out
@@ -181,7 +180,7 @@ struct FinishIn {
 enum FinishModes { kModesAll = 0, kModesFoldLast = 1, kModesFoldFirst = 2 };
 template <int D, typename R, bool ML>
 constexpr bool folded_form() {
-    return (kFastF64 && sizeof(R) == 8 && D > 1) || (sizeof(R) == 4 && !ML && D > 1);
+    return (kFastF64 && sizeof(R) == 8 && D > 1) || (sizeof(R) == 4 && D > 1);
 }
 template <int D, bool LAST, typename R, bool ML>
 constexpr int finish_modes() {

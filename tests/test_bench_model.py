@@ -36,3 +36,4 @@ def test_sweep_order():
     assert bench.sweep_order(_P(3, 2, 0)) == ["sweep_x", "sweep_y", "sweep_z"]
     assert bench.sweep_order(_P(3, 2, 3)) == ["sweep_z", "sweep_y", "sweep_x"]
     assert bench.sweep_order(_P(2, 2, 1)) == ["sweep_y", "sweep_x"]
+    assert bench.sweep_order(_P(3, 2, 2)) == ["sweep_z", "sweep_y", "sweep_x"]
