@@ -89,6 +89,11 @@ int mfc_rhs_init(const mfc_params *prm);
  * s_cb is the ghosted host array x_cb(-1-b : s+b) and ds the ghosted host
  * array dx(-b : s+b) as left by s_populate_grid_variables_buffers
  * (src/simulation/m_start_up.fpp:517-655). */
+/* In the forms other than MFC_PREC_FP64, a direction whose cells' WENO
+ * coefficients all lie within 5e-13 (relative) of the first reconstructed
+ * cell's (equal widths up to the rounding of the cell boundaries) is run
+ * with the first cell's coefficients; MFC_PREC_FP64 does so only when they
+ * are equal bit for bit. */
 int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds);
 
 /* Replaces s_compute_rhs(q_cons_vf, q_prim_vf, rhs_vf, t_step)

@@ -57,6 +57,14 @@ constexpr int kNumFluidsBuiltF32 = 4; // the FP32-bulk and learned forms
 // balances the two sweeps (launches of 5.5 and 5.8 ms instead of 5.0 and
 // 6.3 with the whole base on y).
 constexpr int kBaseOnZ = 4;
+// A grid direction counts as uniform in the contracted and FP32 forms when
+// every cell's WENO coefficients are within this relative distance of the
+// first reconstructed cell's.  Equal widths on paper differ by the rounding
+// of the cell boundaries: 768 cells on [0, 1] give coefficients 2.6e-13
+// apart, which moves the RHS by about as much relative to its scale, well
+// inside the contracted form's 1e-12.  The plain FP64 form keeps the
+// per-cell coefficients unless they are equal bit for bit.
+constexpr double kUniformRelTol = 5e-13;
 // The fused x and y sweep, and the RK-base variables the z sweep adds before
 // it (512^3: 47.7, 46.0, 44.9, 44.8, 45.9 ms per step with 0, 2, 4, 6, 8 of
 // the 8; the fused sweep is bound by VALU issue, the z sweep less so).
@@ -90,7 +98,9 @@ struct Dir {
     double *inv_dsg = nullptr; // device, 1/ds(i) for i = -b..s+b (weno_Re_flux gradients)
     std::vector<double> coef_h, ds_h;
     bool ready = false;
-    bool uniform = false;  // every reconstructed cell has the same coefficients
+    // every reconstructed cell has the same coefficients: bit for bit in the
+    // plain FP64 form, to kUniformRelTol in the others
+    bool uniform = false;
     double ucf[kNCoef] = {};
     bool uniform_wide = false; // the same over every cell the coefficients are set for (-b+polyn..s+b-polyn)
 };
@@ -1063,14 +1073,19 @@ int mfc_rhs_set_grid(int dir, const double *s_cb, const double *ds) {
         if (!(ds[i + b] > 0.0)) return fail(-1, "non-positive cell width at index %d of direction %d", i, dir);
     weno_tables(s, b, G.prm.weno_order, s_cb, D.coef_h);
     // Uniform coefficients over the cells the sweeps reconstruct (-1..s+1)?
+    // Bit for bit in the plain FP64 form; within kUniformRelTol of cell -1's
+    // in the others, whose uniform-grid kernels then use cell -1's.
     {
         const long nc = s + 2 * b + 1;
+        const double tol = G.prm.precision == MFC_PREC_FP64 ? 0.0 : kUniformRelTol;
         D.uniform = true;
         for (int q = 0; q < kNCoef && D.uniform; ++q) {
             const double v0 = D.coef_h[(long)q * nc + (-1 + b)];
             D.ucf[q] = v0;
-            for (int j = 0; j <= s + 1 && D.uniform; ++j)
-                D.uniform = std::memcmp(&v0, &D.coef_h[(long)q * nc + (j + b)], sizeof(double)) == 0;
+            for (int j = 0; j <= s + 1 && D.uniform; ++j) {
+                const double v = D.coef_h[(long)q * nc + (j + b)];
+                D.uniform = std::memcmp(&v0, &v, sizeof(double)) == 0 || std::fabs(v - v0) <= tol * std::fabs(v0);
+            }
         }
     }
     // ... and over every cell with coefficients (the weno_Re_flux gradient

@@ -298,8 +298,11 @@ def xy_tile_cases():
     """Grids uniform bit for bit in x and y (power-of-two cell counts), so the
     contracted form takes its fused x and y sweep (one or two fluids): cell
     counts that leave partial 30 x 14 tiles, every boundary code in x and y,
-    RK2, one fluid in 3D; and three fluids on such a grid, which keep the
-    separate sweeps."""
+    RK2, one fluid in 3D; three fluids on such a grid, which keep the
+    separate sweeps; and grids uniform up to the rounding of their cell
+    boundaries (WENO coefficients 1e-14 to 4e-14 apart), on which the
+    contracted and FP32 forms take the uniform-grid kernels with the first
+    cell's coefficients and the plain form the per-cell ones."""
     out = []
     prm, g, q = one_fluid_3d_case(31, 31, 31, (-1, -1, -1), (-1, -1, -1))
     out.append(("xy_tiles_3d_one_fluid", prm, g, q))
@@ -314,6 +317,10 @@ def xy_tile_cases():
     out.append(("xy_tiles_2d_walls", prm, g, perturbed(prm, q)))
     prm, g, q = multi_fluid_2d(3, N=64)
     out.append(("xy_tiles_fluids3_2d", prm, g, q))
+    prm, g, q = field_3d_case(47, 39, 29, (-1, -1, -1), (-1, -1, -1))
+    out.append(("xy_tiles_3d_rounded_grid", prm, g, q))
+    prm, g, q = ic.shock_bubble_2d(N=96)
+    out.append(("xy_tiles_2d_rounded_grid", prm, g, perturbed(prm, q)))
     return out
 
 

@@ -24,7 +24,7 @@ import pytest
 from microfc_amd import PREC_FP64_FAST, RhsModule, ic
 from oracle import oracle as O
 
-from helpers import rel_linf, say, scaled_linf, small_cases, to_oracle, ulp_diff
+from helpers import perturbed, rel_linf, say, scaled_linf, small_cases, to_oracle, ulp_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -89,6 +89,48 @@ def test_fp64_fast_steps(gpu, case):
             mod.s_tvd_rk(t, (qd, q2d), None, rd, dt)
             mod.sync()
             _check(prm.interior(qd.cpu().numpy()), prm.interior(qo), f"{name} step {t}")
+    finally:
+        prm.precision = 0
+        mod.finalize()
+
+
+def test_fp64_fast_rounded_grid_768(gpu):
+    """The 2D shock-bubble at 768^2 on [0, 1]^2 (every cell perturbed): its
+    cell widths differ by the rounding of the boundaries, its WENO
+    coefficients by up to 2.6e-13 relative, and the contracted form takes the
+    fused sweep with the first cell's coefficients.  One RHS and one RK3 step
+    from the oracle's state within 1e-12 of the oracle's (the RHS's volume
+    fractions on the partial-density scale, as above)."""
+    torch = gpu
+    prm, grids, q = ic.shock_bubble_2d(N=768)
+    q = perturbed(prm, q)
+    oc = to_oracle(prm)
+    ref = O.compute_rhs(oc, grids, q)
+    prm.precision = PREC_FP64_FAST
+    mod = RhsModule(prm, grids)
+    try:
+        qd = torch.from_numpy(q.copy()).cuda()
+        rd = mod.new_field(ghosted=False)
+        mod.timing(enable=True, reset=True)
+        mod.s_compute_rhs(qd, None, rd, 0)
+        mod.sync()
+        calls = {k: v[1] for k, v in mod.timing(enable=False).items() if v[1]}
+        say(f"[fp64 fast rounded grid] bubble2d_768 launches {calls}")
+        assert calls.get("sweep_xy") == 1
+        got = rd.cpu().numpy()
+        rel = rel_linf(got, ref)
+        err = scaled_linf(prm, got, ref)
+        adv = prm.num_fluids + prm.num_dims + 1
+        err[:adv] = rel[:adv]
+        say(f"  rhs: {np.array2string(err, precision=2)}")
+        assert np.all(np.isfinite(got)) and err.max() <= TOL, err
+        dt = 0.3 * mod.cfl_dt(qd, 1.0)
+        q2d = mod.new_field()
+        qo = q.copy()
+        O.tvd_rk_step(oc, grids, dt, qo, np.zeros_like(qo), t_step=0)
+        mod.s_tvd_rk(0, (qd, q2d), None, rd, dt)
+        mod.sync()
+        _check(prm.interior(qd.cpu().numpy()), prm.interior(qo), "bubble2d_768 step")
     finally:
         prm.precision = 0
         mod.finalize()

@@ -2,12 +2,14 @@
 timing (mfc_timing_read): the fused x and y sweep on grids uniform in x and y
 with WENO5 and one or two fluids, in the plain, contracted and FP32-bulk
 forms; the separate sweeps on stretched grids, with three fluids and in the
-learned form.  The results of every path are checked against the oracle by
+learned form; on grids uniform up to the rounding of their cell boundaries
+the fused sweep in the contracted and FP32-bulk forms and the separate
+sweeps (per-cell coefficients) in the plain form.  The results of every path are checked against the oracle by
 the parity, contracted-form and FP32-bulk tests; this pins that the fused
 path is the one those tests exercise on the uniform cases."""
 import pytest
 
-from microfc_amd import PREC_FP32_BULK, PREC_FP64, PREC_FP64_FAST, RhsModule
+from microfc_amd import PREC_FP32_BULK, PREC_FP64, PREC_FP64_FAST, RhsModule, ic
 
 from helpers import field_3d_case, multi_fluid_2d, say, stretched_grids
 
@@ -49,3 +51,20 @@ def test_stretched_grid_and_three_fluids_run_separate_sweeps(gpu, precision):
     n = _launches(gpu, prm, g, q, precision)
     say(f"[dispatch] three fluids 2D, precision {precision}: {n}")
     assert "sweep_xy" not in n and n.get("sweep_x") == 1 and n.get("sweep_y") == 1
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP64_FAST, PREC_FP32_BULK], ids=["fp64", "fast", "fp32"])
+def test_rounded_uniform_grid(gpu, precision):
+    # 47 x 39 x 29 cells on [0, 1]^3: equal widths up to the rounding of the
+    # boundaries (WENO coefficients up to 2.5e-14 apart)
+    prm, g, q = field_3d_case(47, 39, 29, (-1, -1, -1), (-1, -1, -1))
+    n = _launches(gpu, prm, g, q, precision)
+    say(f"[dispatch] rounded uniform 3D, precision {precision}: {n}")
+    if precision == PREC_FP64:
+        assert "sweep_xy" not in n and n.get("sweep_x") == 1 and n.get("sweep_y") == 1
+    else:
+        assert n.get("sweep_xy") == 1 and n.get("sweep_z") == 1
+    prm, g, q = ic.shock_bubble_2d(N=96)
+    n = _launches(gpu, prm, g, q, precision)
+    say(f"[dispatch] rounded uniform 2D, precision {precision}: {n}")
+    assert ("sweep_xy" in n) == (precision != PREC_FP64)
