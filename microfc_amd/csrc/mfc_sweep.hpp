@@ -95,8 +95,10 @@ struct SweepArgs {
 // against 6.83 ms, 2048^2 0.204 against 0.219 ms: with less arithmetic per
 // cell, four 256-thread blocks per CU hide the finish's rhs traffic better
 // than two of 512).  The z sweep, whose stencil rows are planes apart,
-// gains from 32 at 256^3 but loses 6 % at 512^3 (36 planes per block), and
-// in FP32, where the 3D sweeps near the HBM rate, 32 is up to 11 % slower.
+// gained from 32 at 256^3 but lost 6 % at 512^3 (36 planes per block) while
+// it came last; first, storing rhs in the contracted form, 32 and 16 are
+// level at 512^3 (5.52 against 5.53 ms).  In FP32, where the 3D sweeps near
+// the HBM rate, 32 is up to 11 % slower.
 constexpr int tc_rows_of(bool f64, bool fast_f64, int dir) { return f64 && !fast_f64 && dir == 1 ? 32 : 16; }
 // x columns per cell-parallel tile: 16 in every form.  (32 for the FP32
 // z sweeps, whose stencil rows are planes apart, had been 7.5 % faster at
