@@ -100,7 +100,8 @@ def test_fp64_fast_rounded_grid_768(gpu):
     coefficients by up to 2.6e-13 relative, and the contracted form takes the
     fused sweep with the first cell's coefficients.  One RHS and one RK3 step
     from the oracle's state within 1e-12 of the oracle's (the RHS's volume
-    fractions on the partial-density scale, as above)."""
+    fractions on the partial-density scale, as above), and 20 further steps
+    of both within the 1e-9 trajectory bar."""
     torch = gpu
     prm, grids, q = ic.shock_bubble_2d(N=768)
     q = perturbed(prm, q)
@@ -127,10 +128,16 @@ def test_fp64_fast_rounded_grid_768(gpu):
         dt = 0.3 * mod.cfl_dt(qd, 1.0)
         q2d = mod.new_field()
         qo = q.copy()
-        O.tvd_rk_step(oc, grids, dt, qo, np.zeros_like(qo), t_step=0)
+        q2o = np.zeros_like(qo)
+        O.tvd_rk_step(oc, grids, dt, qo, q2o, t_step=0)
         mod.s_tvd_rk(0, (qd, q2d), None, rd, dt)
         mod.sync()
         _check(prm.interior(qd.cpu().numpy()), prm.interior(qo), "bubble2d_768 step")
+        for t in range(1, 21):
+            O.tvd_rk_step(oc, grids, dt, qo, q2o, t_step=t)
+            mod.s_tvd_rk(t, (qd, q2d), None, rd, dt)
+        mod.sync()
+        _check(prm.interior(qd.cpu().numpy()), prm.interior(qo), "bubble2d_768 after 21 steps", tol=1e-9)
     finally:
         prm.precision = 0
         mod.finalize()
